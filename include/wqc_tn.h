@@ -1,0 +1,216 @@
+/*
+ * wqc_tn.h — C ABI of the MI355X (gfx950) MPS tensor-network engine.
+ *
+ * This is the drop-in boundary for wqc-core's TN backend seam. Every entry point
+ * replaces one method of the reference's `MpsState` (src/tn/mps.rs) or one piece of
+ * the backend selection (src/tn/backend.rs, src/tn/engine_status.rs); the citation on
+ * each declaration names the reference item it stands in for.
+ *
+ * Conventions
+ *  - Plain C types only: opaque handles, pointers, sizes. No exceptions cross the ABI.
+ *  - Every function that can fail returns a wqc_tn_status. The thread-local message of
+ *    the last failure is available from wqc_tn_last_error().
+ *  - Complex numbers are interleaved doubles (re, im).
+ *  - Site tensors use the reference layout (src/tn/mps.rs:56): a site of shape
+ *    (l, 2, r) is row-major, element (a, s, b) at index a*2r + s*r + b.
+ *  - Qubit q of a basis index is bit q (src/tn/mps.rs:742-780).
+ *  - The library owns device memory; the caller owns every host buffer it passes.
+ *  - A state handle is single-threaded; a device handle may be shared by states
+ *    created on the same thread. Calls are synchronous at the ABI: results are ready
+ *    when a function returns.
+ */
+#ifndef WQC_TN_H
+#define WQC_TN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The reference returns Result<_, EngineError> (src/engine.rs:68-78). */
+typedef enum wqc_tn_status {
+    WQC_TN_OK = 0,
+    WQC_TN_INVALID_ARG = 1,     /* InvalidQubitCount, QubitIndexOutOfBounds, bad sizes      */
+    WQC_TN_OUT_OF_MEMORY = 2,   /* InsufficientMemory (src/tn/mps.rs:36-48), on device VRAM  */
+    WQC_TN_EXEC_FAILED = 3,     /* ExecutionFailed(String): numeric or runtime failure       */
+    WQC_TN_CANCELLED = 4,       /* cancel::check (src/cancel.rs:52-63)                       */
+    WQC_TN_BOND_EXCEEDED = 5,   /* "mps bond exceeded chi" (src/tn/mps.rs:1137-1143)         */
+    WQC_TN_NO_DEVICE = 6        /* no usable gfx950 device                                   */
+} wqc_tn_status;
+
+/* Gate kinds. The numeric values are the reference's own gate ids
+ * (Gate::to_stark_id, src/engine.rs:284-300); MEASURE/RESET/IF follow. */
+typedef enum wqc_tn_gate_kind {
+    WQC_TN_GATE_X = 1,
+    WQC_TN_GATE_Y = 2,
+    WQC_TN_GATE_Z = 3,
+    WQC_TN_GATE_H = 4,
+    WQC_TN_GATE_S = 5,
+    WQC_TN_GATE_T = 6,
+    WQC_TN_GATE_CNOT = 7,   /* q0 = control, q1 = target */
+    WQC_TN_GATE_CZ = 8,     /* q0 = control, q1 = target */
+    WQC_TN_GATE_CCNOT = 9,  /* q0, q1 = controls, q2 = target */
+    WQC_TN_GATE_RX = 10,    /* q0 = target, angle */
+    WQC_TN_GATE_RY = 11,
+    WQC_TN_GATE_RZ = 12,
+    WQC_TN_GATE_MEASURE = 13, /* no-op on the state (src/tn/mps.rs:294) */
+    WQC_TN_GATE_RESET = 14,   /* no-op on the state (src/tn/mps.rs:294) */
+    WQC_TN_GATE_IF = 15       /* no-op on the state (src/tn/mps.rs:294) */
+} wqc_tn_gate_kind;
+
+/* One gate of src/engine.rs:257-279 (Gate enum), flattened. */
+typedef struct wqc_tn_gate {
+    int32_t kind;   /* wqc_tn_gate_kind */
+    int32_t q0;
+    int32_t q1;
+    int32_t q2;
+    double angle;   /* RX/RY/RZ only */
+} wqc_tn_gate;
+
+/* Arithmetic of the two-site split.
+ *  FP64       : plain complex128 everywhere (the drop-in form).
+ *  MIXED_FP32 : complex64 GEMMs (merge, range finder, factors) and complex64 early
+ *               Jacobi sweeps; the last Jacobi sweeps, the singular values and the
+ *               truncation decision in FP64. */
+typedef enum wqc_tn_precision {
+    WQC_TN_PRECISION_FP64 = 0,
+    WQC_TN_PRECISION_MIXED_FP32 = 1
+} wqc_tn_precision;
+
+typedef struct wqc_tn_device wqc_tn_device;
+typedef struct wqc_tn_state wqc_tn_state;
+
+/* Counters and timings of one state (WorkReport fields and test counters). */
+typedef struct wqc_tn_stats {
+    uint64_t two_qubit_units;     /* apply_two_qubit_at_positions calls (src/tn/mps.rs:377) */
+    uint64_t one_qubit_gates;
+    uint64_t exact_splits;        /* svd_split_two_site (src/tn/mps.rs:1182)              */
+    uint64_t rsvd_kept;           /* RSVD_KEPT (src/tn/mps.rs:797-801)                    */
+    uint64_t rsvd_fallbacks;      /* RSVD_FALLBACKS                                       */
+    uint64_t svd_check_failures;  /* SVD_RETRIES analogue (src/tn/svd_guard.rs:141-144)   */
+    uint64_t jacobi_sweeps;       /* total outer Jacobi sweeps run                        */
+    uint64_t truncated_splits;    /* splits that dropped a singular value above 1e-12     */
+    uint64_t peak_vram_bytes;     /* vram_peak_bytes (src/tn/mps.rs:157-166)              */
+    double max_discarded_weight;  /* max over splits of sum_{k>=chi} s_k^2 / sum s_k^2     */
+    double sum_discarded_weight;
+} wqc_tn_stats;
+
+/* ---- errors ------------------------------------------------------------------ */
+
+/* Thread-local text of the last failure on this thread ("" when none). */
+const char *wqc_tn_last_error(void);
+
+/* ---- backend seam (src/tn/backend.rs:12-29, src/tn/engine_status.rs:17-117) ---- */
+
+/* Parses WQC_TN_BACKEND like tn_backend_from_env: returns "hip" for "hip"/"gpu"/"rocm",
+ * otherwise "cpu" (the reference CPU path stays the caller's). Never fails. */
+const char *wqc_tn_backend_from_env(void);
+
+/* Probe like probe_tn_engine_status: writes "hip" and the device name into `active`
+ * (capacity `cap`), the usable VRAM budget into *vram_budget_bytes (cf. the
+ * `vram_budget_bytes` of /sysinfo, src/tn/engine_status.rs:105-117). */
+int wqc_tn_engine_status(char *active, size_t cap, uint64_t *vram_budget_bytes);
+
+/* ---- device ------------------------------------------------------------------ */
+
+/* Open device `ordinal` (0 = first visible GPU). Replaces GpuMpsDevice::try_new
+ * (src/tn/gpu/device.rs:68-214). */
+int wqc_tn_device_open(int ordinal, wqc_tn_device **out);
+void wqc_tn_device_close(wqc_tn_device *dev);
+
+/* ---- state lifetime ------------------------------------------------------------ */
+
+/* MpsState::try_new_with_bond (src/tn/mps.rs:82-126): |0...0> product state, every
+ * site (1,2,1) = [1, 0]. qubit_count == 0 -> INVALID_ARG; max_bond_dim == 0 ->
+ * EXEC_FAILED; N*chi^2*32 bytes above the free VRAM -> OUT_OF_MEMORY. */
+int wqc_tn_state_new(wqc_tn_device *dev, size_t qubit_count, size_t max_bond_dim,
+                     int precision, wqc_tn_state **out);
+void wqc_tn_state_free(wqc_tn_state *st);
+
+size_t wqc_tn_qubit_count(const wqc_tn_state *st);
+/* pub max_bond_dim field (mutated by src/tn/tree.rs:288). */
+int wqc_tn_set_max_bond_dim(wqc_tn_state *st, size_t max_bond_dim);
+/* private reject_truncation flag (src/tn/mps.rs:61, set at :1270,1274). */
+int wqc_tn_set_reject_truncation(wqc_tn_state *st, int reject);
+
+/* ---- gate application ------------------------------------------------------------ */
+
+/* MpsState::apply_gate (src/tn/mps.rs:278-297). */
+int wqc_tn_apply_gate(wqc_tn_state *st, const wqc_tn_gate *gate);
+
+/* The apply loop of trace::execute_with_trace_using (src/tn/trace.rs:71-161) without
+ * trace rows: MEASURE/RESET/IF skipped, same-target same-axis rotation runs applied
+ * as one net rotation, runs of H applied one by one, and qubit-disjoint gates batched
+ * into one device launch per layer (disjoint updates touch disjoint sites, so the
+ * result equals gate-by-gate application). This is contract_slice's gate loop
+ * (src/tn/contract.rs:10-30). */
+int wqc_tn_apply_circuit(wqc_tn_state *st, const wqc_tn_gate *gates, size_t count);
+
+/* MpsState::apply_adjacent_disjoint_gates (src/tn/mps.rs:520-565). */
+int wqc_tn_apply_adjacent_disjoint_gates(wqc_tn_state *st, const wqc_tn_gate *gates,
+                                         size_t count);
+
+/* Sweep-tree replay hooks (src/tn/mps.rs:621-655). `gate` of apply_unary_at must be a
+ * one-qubit gate; apply_pair_at takes CNOT or CZ; site+1 must exist. */
+int wqc_tn_apply_unary_at(wqc_tn_state *st, size_t site, const wqc_tn_gate *gate);
+int wqc_tn_apply_tdg_at(wqc_tn_state *st, size_t site);
+int wqc_tn_apply_swap_at(wqc_tn_state *st, size_t site);
+int wqc_tn_apply_pair_at(wqc_tn_state *st, size_t site, const wqc_tn_gate *gate,
+                         int ctrl_on_left);
+int wqc_tn_apply_cnot_at(wqc_tn_state *st, size_t site, int ctrl_on_left);
+
+/* Cancellation (src/cancel.rs:22-63): sets a flag polled before every two-qubit unit
+ * and every layer; the running call then returns WQC_TN_CANCELLED. Thread-safe. */
+void wqc_tn_request_cancel(wqc_tn_state *st);
+void wqc_tn_clear_cancel(wqc_tn_state *st);
+
+/* ---- readouts ---------------------------------------------------------------------- */
+
+/* MpsState::amplitude_at_compact_zero (src/tn/mps.rs:262-276): <0...0|psi>, FP64. */
+int wqc_tn_amplitude_at_compact_zero(wqc_tn_state *st, double out[2]);
+
+/* MpsState::site_amplitudes (src/tn/mps.rs:679-693): out = {a0.re, a0.im, a1.re, a1.im}. */
+int wqc_tn_site_amplitudes(wqc_tn_state *st, size_t target, double out[4]);
+/* MpsState::control_probability (src/tn/mps.rs:695-698). */
+int wqc_tn_control_probability(wqc_tn_state *st, size_t control, double *out);
+
+/* MpsState::contract_to_statevector (src/tn/mps.rs:215-260): n <= 20, else
+ * EXEC_FAILED "full statevector projection requires qubit_count <= 20".
+ * `out` holds 2^n complex values (2^(n+1) doubles). */
+int wqc_tn_contract_to_statevector(wqc_tn_state *st, double *out, size_t len_complex);
+
+/* Bond dimensions: out[i] = right bond of site i, i < n (len >= n). */
+int wqc_tn_bond_dims(wqc_tn_state *st, size_t *out, size_t len);
+
+/* Site download/upload (the `pub sites` field, src/tn/mps.rs:56). download writes
+ * l*2*r complex values when len_complex allows and always reports l and r. upload
+ * replaces the site; its l must equal the right bond of site-1 (1 for site 0) and r
+ * the left bond of site+1 (1 for the last site), except when the caller uploads a
+ * whole chain site by site (then shapes are checked at the next gate). */
+int wqc_tn_download_site(wqc_tn_state *st, size_t site, double *out, size_t len_complex,
+                         size_t *l, size_t *r);
+int wqc_tn_upload_site(wqc_tn_state *st, size_t site, const double *data, size_t l,
+                       size_t r);
+
+/* ---- reporting ---------------------------------------------------------------------- */
+
+/* MpsState::backend_label (src/tn/mps.rs:146-155): "hip". */
+const char *wqc_tn_backend_label(const wqc_tn_state *st);
+/* MpsState::peak_vram_bytes (src/tn/mps.rs:157-166): largest device footprint of this
+ * state (sites plus split workspace), in bytes. */
+uint64_t wqc_tn_peak_vram_bytes(const wqc_tn_state *st);
+int wqc_tn_get_stats(const wqc_tn_state *st, wqc_tn_stats *out);
+
+/* Device time (ms) accumulated per kernel family since the state was created, for
+ * measurement: index 0 = one-qubit, 1 = merge, 2 = range-finder GEMMs, 3 = Jacobi,
+ * 4 = factors, 5 = readouts. Returns the number of entries written. */
+int wqc_tn_kernel_times(const wqc_tn_state *st, double *ms, size_t len);
+int wqc_tn_enable_timing(wqc_tn_state *st, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WQC_TN_H */

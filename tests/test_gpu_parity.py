@@ -1,0 +1,246 @@
+"""GPU parity: the HIP engine (through the C ABI) against the CPU FP64 oracle and the numpy
+dense statevector, on seeded inputs.
+
+Bars (plain FP64 form): amplitudes and statevectors within 1e-10 relative, identical bond
+dimensions (the same kept singular spectrum) and identical split paths. Range-finder
+splits are compared with the oracle's exact-arithmetic rank rule
+(oracle_mps.set_qr_rank_tol(1e-10)); see test_oracle.py for why the literal
+floating-point range finder of the reference has no reproducible output.
+"""
+
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import circuits as C
+import dense_ref as D
+from oracle import oracle_mps as om
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def wt():
+    import wqc_tn
+
+    wqc_tn.load()
+    return wqc_tn
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def run_both(wt, n, chi, gates, circuit=True):
+    g = wt.MpsState(n, chi)
+    o = om.OracleMps(n, chi, record=True)
+    if circuit:
+        g.apply_circuit(gates)
+        o.apply_circuit(gates)
+    else:
+        for x in gates:
+            g.apply_gate(x)
+        o.apply_gates(gates)
+    return g, o
+
+
+def test_device_and_labels(wt):
+    st = wt.MpsState(4, 8)
+    assert st.backend_label() == "hip"
+    assert st.peak_vram_bytes() > 0
+    assert st.bond_dims() == [1, 1, 1, 1]
+    assert st.amplitude_at_compact_zero() == 1.0
+
+
+def test_product_amplitudes_50_100(wt):
+    # src/tn/contract.rs:216-230
+    for n in (50, 100):
+        st = wt.MpsState(n, 128)
+        st.apply_circuit([C.one(C.H, 0)])
+        a = st.amplitude_at_compact_zero()
+        assert abs(a.real - 1 / math.sqrt(2)) < 1e-12 and abs(a.imag) < 1e-12
+
+
+def test_random_small_circuits_vs_oracle_and_dense(wt):
+    # src/tn/mps.rs:1922-1972 shape: 1-6 qubits, every gate type, CCNOT and long range.
+    from test_oracle import seeded_random_circuit
+
+    rng = C.SplitMix64(0x5EED_2026_1128)
+    worst = 0.0
+    for case in range(120):
+        n = 1 + rng.below(6)
+        depth = 1 + rng.below(24)
+        gates = seeded_random_circuit(rng, n, depth)
+        g, o = run_both(wt, n, 128, gates, circuit=False)
+        sv = g.contract_to_statevector()
+        d1 = np.max(np.abs(sv - o.contract_to_statevector()))
+        d2 = np.max(np.abs(sv - D.run(n, gates)))
+        worst = max(worst, d1)
+        assert d1 < TOL and d2 < 1e-9, (case, n, gates, d1, d2)
+        assert g.bond_dims() == o.bond_dims(), case
+    say("random circuits worst statevector diff", worst)
+
+
+def test_config1_amplitude(wt):
+    n, chi, gates = C.config1()
+    g, o = run_both(wt, n, chi, gates)
+    want = D.run(n, gates)[0]
+    a = g.amplitude_at_compact_zero()
+    say("config1", a, want, rel(a, want))
+    assert rel(a, want) <= TOL
+    assert rel(a, o.amplitude_at_compact_zero()) <= TOL
+    assert g.bond_dims() == o.bond_dims()
+    s = g.stats()
+    assert s["two_qubit_units"] == C.two_qubit_units(gates, n) == 75
+    assert np.max(np.abs(g.contract_to_statevector() - D.run(n, gates))) < 1e-12
+
+
+def test_config2_amplitude(wt):
+    n, chi, gates = C.config2()
+    g, o = run_both(wt, n, chi, gates)
+    a, b = g.amplitude_at_compact_zero(), o.amplitude_at_compact_zero()
+    say("config2", a, b, rel(a, b), g.stats())
+    assert rel(a, b) <= TOL
+    assert g.bond_dims() == o.bond_dims()
+    assert g.stats()["two_qubit_units"] == o.counters()["units"]
+
+
+@pytest.mark.parametrize("n,depth,chi", [(24, 16, 16), (20, 14, 8), (16, 12, 64)])
+def test_truncating_brickwork_vs_oracle(wt, n, depth, chi):
+    # chi below the circuit's Schmidt ranks: every split truncates through the exact SVD.
+    gates = C.brickwork(n, depth, 77 + n)
+    g, o = run_both(wt, n, chi, gates)
+    a, b = g.amplitude_at_compact_zero(), o.amplitude_at_compact_zero()
+    say("truncating", n, depth, chi, a, b, rel(a, b), g.stats()["truncated_splits"])
+    assert g.stats()["truncated_splits"] > 0
+    assert g.bond_dims() == o.bond_dims()
+    assert rel(a, b) <= TOL
+
+
+def test_disjoint_layer_equals_sequential_bitwise(wt):
+    # src/tn/mps.rs:1772-1793, and disjoint batching inside apply_circuit.
+    prep = C.brickwork(8, 4, 5)
+    layer = [C.two(C.CNOT, 0, 1), C.two(C.CZ, 3, 2), C.two(C.CNOT, 5, 4), C.two(C.CZ, 6, 7)]
+    a = wt.MpsState(8, 16)
+    b = wt.MpsState(8, 16)
+    a.apply_circuit(prep)
+    b.apply_circuit(prep)
+    for x in layer:
+        a.apply_gate(x)
+    b.apply_adjacent_disjoint_gates(layer)
+    assert np.array_equal(a.contract_to_statevector(), b.contract_to_statevector())
+
+
+def test_site_amplitudes_vs_oracle(wt):
+    n = 10
+    gates = C.brickwork(n, 6, 99)
+    g, o = run_both(wt, n, 32, gates)
+    for t in range(n):
+        ga, oa = g.site_amplitudes(t), o.site_amplitudes(t)
+        for x, y in zip(ga, oa):
+            assert abs(x - y) <= 1e-12 * max(1.0, abs(y))
+        assert abs(g.control_probability(t) - o.control_probability(t)) <= 1e-12
+
+
+def test_reject_truncation_and_errors(wt):
+    st = wt.MpsState(8, 2)
+    st.set_reject_truncation(True)
+    with pytest.raises(wt.WqcTnError) as e:
+        st.apply_circuit(C.brickwork(8, 6, 3))
+    assert e.value.code == wt.BOND_EXCEEDED and "bond exceeded" in str(e.value)
+    with pytest.raises(wt.WqcTnError) as e:
+        wt.MpsState(0, 8)
+    assert e.value.code == wt.INVALID_ARG
+    st = wt.MpsState(3, 8)
+    with pytest.raises(wt.WqcTnError) as e:
+        st.apply_gate(C.one(C.H, 3))
+    assert e.value.code == wt.INVALID_ARG
+    with pytest.raises(wt.WqcTnError) as e:
+        wt.MpsState(21, 4).contract_to_statevector()
+    assert "qubit_count <= 20" in str(e.value)
+    st = wt.MpsState(4, 8)
+    st.request_cancel()
+    with pytest.raises(wt.WqcTnError) as e:
+        st.apply_circuit([C.one(C.H, 0), C.two(C.CNOT, 0, 1)])
+    assert e.value.code == wt.CANCELLED
+    st.clear_cancel()
+    st.apply_circuit([C.one(C.H, 0), C.two(C.CNOT, 0, 1)])
+
+
+def random_mps_sites(rng, bonds, decay):
+    """Sites for an MPS with the given bonds; singular-value-like decay on the bonds."""
+    sites = []
+    for i in range(len(bonds) - 1):
+        l, r = bonds[i], bonds[i + 1]
+        t = rng.standard_normal((l, 2, r)) + 1j * rng.standard_normal((l, 2, r))
+        t *= np.exp(-decay * np.arange(r))[None, None, :]
+        t /= np.linalg.norm(t) / math.sqrt(r)
+        sites.append(t)
+    return sites
+
+
+@pytest.mark.parametrize("b,chi,decay", [(128, 128, 0.02), (128, 100, 0.01), (96, 120, 0.03)])
+def test_range_finder_split_vs_oracle_exact_rank_rule(wt, b, chi, decay):
+    # theta of 2b x 2b with ell = chi + 16 < 2b: the range-finder path (src/tn/mps.rs:937-981).
+    rng = np.random.default_rng(b * 7 + chi)
+    sites = random_mps_sites(rng, [1, b, b, b, 1], decay)
+    om.set_qr_rank_tol(1e-10)
+    try:
+        g = wt.MpsState(4, chi)
+        o = om.OracleMps(4, chi, record=True)
+        for i, s in enumerate(sites):
+            g.set_site(i, s)
+            o.set_site(i, s)
+        gate = C.two(C.CZ, 1, 2)
+        g.apply_gate(gate)
+        o.apply_gate(gate)
+        paths, chis = o.split_log()
+        st = g.stats()
+        say("rsvd", b, chi, paths, chis, st["rsvd_kept"], st["rsvd_fallbacks"])
+        assert paths[0] in (1, 2)
+        assert (st["rsvd_kept"], st["rsvd_fallbacks"]) == ((1, 0) if paths[0] == 1 else (0, 1))
+        assert g.bond_dims() == o.bond_dims()
+        k = g.bond_dims()[1]
+        pg = g.site(1).reshape(-1, k) @ g.site(2).reshape(k, -1)
+        po = o.site(1).reshape(-1, k) @ o.site(2).reshape(k, -1)
+        err = np.max(np.abs(pg - po)) / np.max(np.abs(po))
+        say("rsvd product err", err)
+        assert err <= TOL
+        a, c = g.amplitude_at_compact_zero(), o.amplitude_at_compact_zero()
+        assert rel(a, c) <= TOL
+    finally:
+        om.set_qr_rank_tol(0.0)
+
+
+@pytest.mark.parametrize("dl,dr,bond,chi", [(40, 24, 30, 64), (24, 40, 30, 64), (64, 64, 64, 48),
+                                           (1, 64, 2, 8), (64, 1, 2, 8), (33, 17, 20, 12)])
+def test_exact_split_shapes_vs_oracle(wt, dl, dr, bond, chi):
+    rng = np.random.default_rng(dl * 1000 + dr * 10 + chi)
+    sites = random_mps_sites(rng, [1, dl, bond, dr, 1], 0.05)
+    g = wt.MpsState(4, chi)
+    o = om.OracleMps(4, chi)
+    for i, s in enumerate(sites):
+        g.set_site(i, s)
+        o.set_site(i, s)
+    gate = C.two(C.CNOT, 2, 1)
+    g.apply_gate(gate)
+    o.apply_gate(gate)
+    assert g.bond_dims() == o.bond_dims()
+    k = g.bond_dims()[1]
+    pg = g.site(1).reshape(-1, k) @ g.site(2).reshape(k, -1)
+    po = o.site(1).reshape(-1, k) @ o.site(2).reshape(k, -1)
+    assert np.max(np.abs(pg - po)) <= TOL * np.max(np.abs(po))
+    # sqrt(sigma) gauge: column norms of the left factor equal row norms of the right one
+    ln = np.linalg.norm(g.site(1).reshape(-1, k), axis=0)
+    rn = np.linalg.norm(g.site(2).reshape(k, -1), axis=1)
+    assert np.allclose(ln, rn, rtol=1e-10)
+    assert np.all(np.diff(ln) <= 1e-12 * ln[0])

@@ -1,0 +1,1629 @@
+// engine.hip — host side of the MI355X MPS engine and its C ABI (include/wqc_tn.h).
+//
+// The state lives in device memory: one buffer per site tensor (l, 2, r), sized for the
+// largest bond the site can reach (min(chi, 2^i, 2^(n-i)) on each side). Gates are
+// scheduled into layers of site-disjoint operations; each layer's one-qubit updates are
+// one launch and its two-site units run as batched launches (merge + U, split). The host
+// reads back only the singular values of each split (to apply the truncation rule in
+// FP64 exactly as the reference does) and the per-sweep Jacobi convergence flags.
+//
+// Reference items restated here (paths relative to the reference repository root):
+//   MpsState::try_new_with_bond       src/tn/mps.rs:82-126
+//   apply_gate / apply_one_qubit      src/tn/mps.rs:278-351
+//   apply_two_qubit_gate (SWAP route) src/tn/mps.rs:353-375
+//   apply_two_qubit_at_positions      src/tn/mps.rs:377-429
+//   apply_adjacent_disjoint_gates     src/tn/mps.rs:520-565
+//   split_theta / rsvd_sites          src/tn/mps.rs:929-1053
+//   truncated_svd_split               src/tn/mps.rs:1124-1201
+//   checked_svd dust rule + norm check src/tn/svd_guard.rs:91-139, 242-256
+//   retained_chi / needs_f64_fallback src/tn/svd_guard.rs:12-76
+//   apply_ccnot                       src/tn/mps.rs:657-677
+//   readouts                          src/tn/mps.rs:215-276, 679-740
+//   execute_with_trace apply loop     src/tn/trace.rs:71-161
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/wqc_tn.h"
+#include "kernels.h"
+
+namespace {
+
+using wqc::FactorJob;
+using wqc::GemmJob;
+using wqc::HouseJob;
+using wqc::JacobiJob;
+using wqc::MergeJob;
+using wqc::OneQubitJob;
+using wqc::VecMatJob;
+
+thread_local std::string g_last_error;
+
+struct Failure {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw Failure{code, msg}; }
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        fail(e == hipErrorOutOfMemory ? WQC_TN_OUT_OF_MEMORY : WQC_TN_EXEC_FAILED,
+             std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPC(x) hip_check((x), #x)
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return WQC_TN_OK;
+    } catch (const Failure& e) {
+        g_last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return WQC_TN_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return WQC_TN_EXEC_FAILED;
+    }
+}
+
+// ---------------------------------------------------------------- constants of the rule
+const double SINGULAR_CUTOFF = 1e-12;       // src/tn/svd_guard.rs:12
+const double CHI_REL_GAP = 1e-3;            // src/tn/svd_guard.rs:15
+const double CUTOFF_ABS_BAND = 1e-10;       // src/tn/svd_guard.rs:18
+const double DUST_REL = 1e-14;              // src/tn/svd_guard.rs:93
+const double SVD_CHECK_TOL = 1e-9;          // src/tn/svd_guard.rs:234
+const size_t RSVD_MIN_SIDE = 160;           // src/tn/mps.rs:794
+const size_t RSVD_OVERSAMPLE = 16;          // src/tn/mps.rs:795
+// Range-finder rank rule: a column of Y whose Householder residual is at most this
+// fraction of its norm is dependent (identity reflector), as in exact arithmetic.
+const double QR_RANK_TOL = 1e-10;
+const int MAX_SWEEPS = 60;
+
+size_t retained_chi(const std::vector<double>& v, size_t max_bond, size_t rows, size_t cols) {
+    if (v.empty() || max_bond == 0 || rows == 0 || cols == 0) return 0;
+    size_t rank = 0;
+    for (double s : v)
+        if (s > SINGULAR_CUTOFF) rank++;
+    rank = std::max<size_t>(rank, 1);
+    return std::min(std::min(rank, max_bond), std::min(rows, cols));
+}
+
+bool needs_f64_fallback(const std::vector<double>& v, size_t chi) {
+    for (double s : v)
+        if (!std::isfinite(s) || s < 0.0) return true;
+    for (double s : v)
+        if (s > 0.0 && std::fabs(s - SINGULAR_CUTOFF) <= CUTOFF_ABS_BAND) return true;
+    if (chi > 0 && chi < v.size()) {
+        double kept = v[chi - 1], dropped = v[chi];
+        double scale = std::max(std::max(kept, dropped), 1e-30);
+        if (std::fabs(kept - dropped) / scale < CHI_REL_GAP) return true;
+    }
+    return false;
+}
+
+bool rsvd_worth(size_t rows, size_t cols, size_t ell) {
+    unsigned __int128 f = (unsigned __int128)rows * cols * ell;
+    unsigned __int128 side = RSVD_MIN_SIDE;
+    return f >= side * side * side;
+}
+
+// ---------------------------------------------------------------- gate tensors
+typedef double2 Mat2[4];    // row-major 2x2
+typedef double2 Mat4[16];   // [c_out][t_out][c_in][t_in]
+
+inline double2 cpx(double r, double i) { return make_double2(r, i); }
+
+bool is_one_qubit(int k) {
+    return k == WQC_TN_GATE_H || k == WQC_TN_GATE_X || k == WQC_TN_GATE_Y || k == WQC_TN_GATE_Z ||
+           k == WQC_TN_GATE_S || k == WQC_TN_GATE_T || k == WQC_TN_GATE_RX ||
+           k == WQC_TN_GATE_RY || k == WQC_TN_GATE_RZ;
+}
+
+// unary_matrix (src/tn/gates.rs:10-71); kind 0 = T-dagger (src/tn/mps.rs:1173-1180).
+void unary_matrix(int kind, double angle, Mat2 u) {
+    const double fs2 = 0.70710678118654752440;
+    for (int i = 0; i < 4; ++i) u[i] = cpx(0, 0);
+    switch (kind) {
+        case WQC_TN_GATE_H: {
+            double k = 1.0 / std::sqrt(2.0);
+            u[0] = cpx(k, 0);
+            u[1] = cpx(k, 0);
+            u[2] = cpx(k, 0);
+            u[3] = cpx(-k, -0.0);
+            break;
+        }
+        case WQC_TN_GATE_X:
+            u[1] = cpx(1, 0);
+            u[2] = cpx(1, 0);
+            break;
+        case WQC_TN_GATE_Y:
+            u[1] = cpx(-0.0, -1.0);
+            u[2] = cpx(0.0, 1.0);
+            break;
+        case WQC_TN_GATE_Z:
+            u[0] = cpx(1, 0);
+            u[3] = cpx(-1.0, -0.0);
+            break;
+        case WQC_TN_GATE_S:
+            u[0] = cpx(1, 0);
+            u[3] = cpx(0, 1);
+            break;
+        case WQC_TN_GATE_T:
+            u[0] = cpx(1, 0);
+            u[3] = cpx(fs2, fs2);
+            break;
+        case WQC_TN_GATE_RX: {
+            double s = std::sin(angle / 2.0), c = std::cos(angle / 2.0);
+            u[0] = cpx(c, 0);
+            u[1] = cpx(0, -s);
+            u[2] = cpx(0, -s);
+            u[3] = cpx(c, 0);
+            break;
+        }
+        case WQC_TN_GATE_RY: {
+            double s = std::sin(angle / 2.0), c = std::cos(angle / 2.0);
+            u[0] = cpx(c, 0);
+            u[1] = cpx(-s, 0);
+            u[2] = cpx(s, 0);
+            u[3] = cpx(c, 0);
+            break;
+        }
+        case WQC_TN_GATE_RZ: {
+            double s = std::sin(angle / 2.0), c = std::cos(angle / 2.0);
+            u[0] = cpx(c, -s);
+            u[3] = cpx(c, s);
+            break;
+        }
+        case 0:
+            u[0] = cpx(1, 0);
+            u[3] = cpx(fs2, -fs2);
+            break;
+        default:
+            fail(WQC_TN_INVALID_ARG, "not a one-qubit gate");
+    }
+}
+
+inline int m4(int co, int to, int ci, int ti) { return ((co * 2 + to) * 2 + ci) * 2 + ti; }
+
+// cnot_matrix / cz_matrix / swap_matrix (src/tn/gates.rs:73-104).
+void two_qubit_matrix(int kind, Mat4 u) {
+    for (int i = 0; i < 16; ++i) u[i] = cpx(0, 0);
+    if (kind == WQC_TN_GATE_CNOT) {
+        u[m4(0, 0, 0, 0)] = cpx(1, 0);
+        u[m4(0, 1, 0, 1)] = cpx(1, 0);
+        u[m4(1, 0, 1, 1)] = cpx(1, 0);
+        u[m4(1, 1, 1, 0)] = cpx(1, 0);
+    } else if (kind == WQC_TN_GATE_CZ) {
+        u[m4(0, 0, 0, 0)] = cpx(1, 0);
+        u[m4(0, 1, 0, 1)] = cpx(1, 0);
+        u[m4(1, 0, 1, 0)] = cpx(1, 0);
+        u[m4(1, 1, 1, 1)] = cpx(-1.0, -0.0);
+    } else {  // SWAP
+        u[m4(0, 0, 0, 0)] = cpx(1, 0);
+        u[m4(0, 1, 1, 0)] = cpx(1, 0);
+        u[m4(1, 0, 0, 1)] = cpx(1, 0);
+        u[m4(1, 1, 1, 1)] = cpx(1, 0);
+    }
+}
+const int KIND_SWAP = 100;
+
+// u4[(so*2+to)*4 + si*2+ti]: the unitary on the (left site, right site) physical pair
+// (the ctrl_is_first index mapping of src/tn/mps.rs:404-414).
+void resolve_u4(const Mat4 u, bool ctrl_is_first, double2* u4) {
+    for (int so = 0; so < 2; ++so)
+        for (int to = 0; to < 2; ++to)
+            for (int si = 0; si < 2; ++si)
+                for (int ti = 0; ti < 2; ++ti) {
+                    int co = ctrl_is_first ? so : to, tco = ctrl_is_first ? to : so;
+                    int ci = ctrl_is_first ? si : ti, tci = ctrl_is_first ? ti : si;
+                    u4[(so * 2 + to) * 4 + si * 2 + ti] = u[m4(co, tco, ci, tci)];
+                }
+}
+
+// ---------------------------------------------------------------- device buffers
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t want) {
+        if (want <= bytes) return;
+        if (p) HIPC(hipFree(p));
+        p = nullptr;
+        bytes = 0;
+        size_t sz = std::max(want, (size_t)1 << 20);
+        sz = (sz + ((size_t)1 << 20) - 1) & ~(((size_t)1 << 20) - 1);
+        HIPC(hipMalloc(&p, sz));
+        bytes = sz;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+struct OmegaEntry {
+    double2* omega_cols = nullptr;  // cols x r row-major: the independent columns of Omega
+    int* pos = nullptr;             // r positions
+    int r = 0;
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------- device handle
+struct wqc_tn_device {
+    int ordinal = 0;
+    hipDeviceProp_t prop{};
+    std::mutex mu;
+    std::map<std::tuple<size_t, size_t, size_t>, OmegaEntry> omega;
+    ~wqc_tn_device() {
+        (void)hipSetDevice(ordinal);
+        for (auto& kv : omega) {
+            if (kv.second.omega_cols) (void)hipFree(kv.second.omega_cols);
+            if (kv.second.pos) (void)hipFree(kv.second.pos);
+        }
+    }
+};
+
+namespace {
+
+// rademacher / mix_seed (src/tn/mps.rs:1103-1122), then the independent columns of
+// Omega in order (modified Gram-Schmidt with one re-orthogonalization, relative residual
+// threshold QR_RANK_TOL). A column of Omega that depends on earlier columns gives a
+// column of Y = A A^H A Omega that depends on the same earlier columns, so only these
+// columns can carry a Householder reflector.
+OmegaEntry& omega_for(wqc_tn_device* dev, size_t rows, size_t cols, size_t ell) {
+    std::lock_guard<std::mutex> lk(dev->mu);
+    auto key = std::make_tuple(rows, cols, ell);
+    auto it = dev->omega.find(key);
+    if (it != dev->omega.end()) return it->second;
+    uint64_t salt = ((uint64_t)rows * 0x9E37ull + (uint64_t)cols) * 0x9E37ull + (uint64_t)ell;
+    uint64_t state = salt | 1ull;
+    const double scale = 1.0 / std::sqrt((double)cols);
+    std::vector<double2> om(cols * ell);
+    for (size_t i = 0; i < cols * ell; ++i) {
+        state = state * 6364136223846793005ull + 1ull;
+        om[i] = make_double2((state & 0x100) == 0 ? scale : -scale, (state & 0x200) == 0 ? scale : -scale);
+    }
+    std::vector<std::vector<double2>> basis;
+    std::vector<int> pos;
+    std::vector<double2> w(cols);
+    for (size_t e = 0; e < ell; ++e) {
+        double n0 = 0.0;
+        for (size_t c = 0; c < cols; ++c) {
+            w[c] = om[c * ell + e];
+            n0 += w[c].x * w[c].x + w[c].y * w[c].y;
+        }
+        n0 = std::sqrt(n0);
+        for (int pass = 0; pass < 2; ++pass)
+            for (auto& b : basis) {
+                double dr = 0, di = 0;  // b^H w
+                for (size_t c = 0; c < cols; ++c) {
+                    dr += b[c].x * w[c].x + b[c].y * w[c].y;
+                    di += b[c].x * w[c].y - b[c].y * w[c].x;
+                }
+                for (size_t c = 0; c < cols; ++c) {
+                    w[c].x -= b[c].x * dr - b[c].y * di;
+                    w[c].y -= b[c].x * di + b[c].y * dr;
+                }
+            }
+        double nw = 0.0;
+        for (size_t c = 0; c < cols; ++c) nw += w[c].x * w[c].x + w[c].y * w[c].y;
+        nw = std::sqrt(nw);
+        if (nw > QR_RANK_TOL * n0) {
+            std::vector<double2> b(cols);
+            for (size_t c = 0; c < cols; ++c) b[c] = make_double2(w[c].x / nw, w[c].y / nw);
+            basis.push_back(std::move(b));
+            pos.push_back((int)e);
+        }
+        if (basis.size() >= std::min(cols, ell)) break;
+    }
+    OmegaEntry ent;
+    ent.r = (int)pos.size();
+    std::vector<double2> sel(cols * ent.r);
+    for (size_t c = 0; c < cols; ++c)
+        for (int k = 0; k < ent.r; ++k) sel[c * ent.r + k] = om[c * ell + pos[k]];
+    HIPC(hipMalloc(&ent.omega_cols, sel.size() * sizeof(double2)));
+    HIPC(hipMemcpy(ent.omega_cols, sel.data(), sel.size() * sizeof(double2), hipMemcpyHostToDevice));
+    HIPC(hipMalloc(&ent.pos, std::max<size_t>(1, pos.size()) * sizeof(int)));
+    HIPC(hipMemcpy(ent.pos, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice));
+    return dev->omega.emplace(key, ent).first->second;
+}
+
+struct Site {
+    double2* ptr = nullptr;
+    size_t cap = 0;  // complex elements
+    int l = 1, r = 1;
+};
+
+// One two-site unit: sites p and p+1, resolved 4x4 unitary.
+struct Unit {
+    int p;
+    double2 u4[16];
+};
+
+struct PendingOne {
+    int site;
+    double2 u[4];
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------- state
+struct wqc_tn_state {
+    wqc_tn_device* dev = nullptr;
+    int n = 0;
+    size_t chi = 0;
+    bool reject = false;
+    int precision = WQC_TN_PRECISION_FP64;
+    std::vector<Site> sites;
+    hipStream_t stream = nullptr;
+    DevBuf ws;        // split workspace
+    DevBuf tab;       // device job tables
+    char* htab = nullptr;  // pinned staging for tables
+    size_t htab_cap = 0, tab_off = 0;
+    double* hread = nullptr;  // pinned readback
+    size_t hread_cap = 0;
+    std::atomic<int> cancel{0};
+    wqc_tn_stats stats{};
+    uint64_t site_bytes = 0;
+    bool timing = false;
+    double kms[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<PendingOne> pend1;
+    std::vector<Unit> pend2;
+    std::vector<char> touched;
+
+    ~wqc_tn_state() {
+        (void)hipSetDevice(dev ? dev->ordinal : 0);
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& s : sites)
+            if (s.ptr) (void)hipFree(s.ptr);
+        ws.release();
+        tab.release();
+        if (htab) (void)hipHostFree(htab);
+        if (hread) (void)hipHostFree(hread);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // ------------------------------------------------------------ helpers
+    void note_peak() {
+        uint64_t now = site_bytes + ws.bytes + tab.bytes;
+        if (now > stats.peak_vram_bytes) stats.peak_vram_bytes = now;
+    }
+
+    size_t bond_cap(int i) const {  // bond between sites i-1 and i
+        size_t b = chi;
+        int e = std::min(i, n - i);
+        if (e < 40) b = std::min(b, (size_t)1 << e);
+        return std::max<size_t>(b, 1);
+    }
+
+    void ensure_site_cap(int i, size_t elems) {
+        Site& s = sites[i];
+        if (elems <= s.cap) return;
+        HIPC(hipStreamSynchronize(stream));
+        double2* np = nullptr;
+        HIPC(hipMalloc(&np, elems * sizeof(double2)));
+        if (s.ptr) {
+            HIPC(hipMemcpy(np, s.ptr, std::min(s.cap, elems) * sizeof(double2), hipMemcpyDeviceToDevice));
+            HIPC(hipFree(s.ptr));
+        }
+        site_bytes += (elems - s.cap) * sizeof(double2);
+        s.ptr = np;
+        s.cap = elems;
+        note_peak();
+    }
+
+    void tables_reset() { tab_off = 0; }
+
+    template <class T>
+    T* upload(const std::vector<T>& v) {
+        size_t bytes = std::max<size_t>(v.size() * sizeof(T), 16);
+        size_t off = (tab_off + 255) & ~(size_t)255;
+        if (off + bytes > htab_cap) fail(WQC_TN_EXEC_FAILED, "job table arena exhausted");
+        std::memcpy(htab + off, v.data(), v.size() * sizeof(T));
+        HIPC(hipMemcpyAsync((char*)tab.p + off, htab + off, bytes, hipMemcpyHostToDevice, stream));
+        tab_off = off + bytes;
+        return reinterpret_cast<T*>((char*)tab.p + off);
+    }
+
+    void check_launch() { HIPC(hipGetLastError()); }
+
+    void sync() { HIPC(hipStreamSynchronize(stream)); }
+
+    void check_cancel() {
+        if (cancel.load()) fail(WQC_TN_CANCELLED, "cancelled");
+    }
+
+    // ------------------------------------------------------------ lifetime
+    void init(wqc_tn_device* d, size_t qn, size_t mb, int prec) {
+        dev = d;
+        n = (int)qn;
+        chi = mb;
+        precision = prec;
+        HIPC(hipSetDevice(dev->ordinal));
+        HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        htab_cap = 8u << 20;
+        HIPC(hipHostMalloc((void**)&htab, htab_cap, hipHostMallocDefault));
+        tab.ensure(htab_cap);
+        hread_cap = 1u << 20;
+        HIPC(hipHostMalloc((void**)&hread, hread_cap * sizeof(double), hipHostMallocDefault));
+        sites.assign(n, Site{});
+        std::vector<double2> init = {make_double2(1, 0), make_double2(0, 0)};
+        for (int i = 0; i < n; ++i) {
+            size_t cap = bond_cap(i) * 2 * bond_cap(i + 1);
+            ensure_site_cap(i, cap);
+            HIPC(hipMemcpy(sites[i].ptr, init.data(), 2 * sizeof(double2), hipMemcpyHostToDevice));
+            sites[i].l = sites[i].r = 1;
+        }
+        touched.assign(n, 0);
+        note_peak();
+    }
+
+    // ------------------------------------------------------------ one-qubit batch
+    void run_one_qubit(const std::vector<PendingOne>& ops) {
+        if (ops.empty()) return;
+        tables_reset();
+        std::vector<OneQubitJob> jobs(ops.size());
+        int max_elems = 1;
+        for (size_t i = 0; i < ops.size(); ++i) {
+            Site& s = sites[ops[i].site];
+            jobs[i].site = s.ptr;
+            jobs[i].l = s.l;
+            jobs[i].r = s.r;
+            for (int k = 0; k < 4; ++k) jobs[i].u[k] = ops[i].u[k];
+            max_elems = std::max(max_elems, s.l * s.r);
+        }
+        OneQubitJob* d = upload(jobs);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timing) {
+            HIPC(hipEventCreate(&e0));
+            HIPC(hipEventCreate(&e1));
+            HIPC(hipEventRecord(e0, stream));
+        }
+        wqc::launch_one_qubit(d, (int)jobs.size(), max_elems, stream);
+        check_launch();
+        if (timing) {
+            HIPC(hipEventRecord(e1, stream));
+            HIPC(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPC(hipEventElapsedTime(&ms, e0, e1));
+            kms[0] += ms;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+        stats.one_qubit_gates += ops.size();
+        sync();
+    }
+
+    // ------------------------------------------------------------ timing helper
+    struct Timer {
+        wqc_tn_state* st;
+        int fam;
+        hipEvent_t a = nullptr, b = nullptr;
+        Timer(wqc_tn_state* s, int f) : st(s), fam(f) {
+            if (!st->timing) return;
+            HIPC(hipEventCreate(&a));
+            HIPC(hipEventCreate(&b));
+            HIPC(hipEventRecord(a, st->stream));
+        }
+        ~Timer() {
+            if (!st->timing || !a) return;
+            if (hipEventRecord(b, st->stream) == hipSuccess && hipEventSynchronize(b) == hipSuccess) {
+                float ms = 0;
+                if (hipEventElapsedTime(&ms, a, b) == hipSuccess) st->kms[fam] += ms;
+            }
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    };
+
+    // ------------------------------------------------------------ Jacobi driver
+    // Rotates the rows of every job's X until no block pair needs a rotation, then
+    // writes the row norms into sigma (device, at sig_off[j]). Returns sweeps used.
+    void jacobi(std::vector<JacobiJob>& jobs, double* d_sigma, const std::vector<long>& sig_off,
+                int* d_flags) {
+        const int nj = (int)jobs.size();
+        if (!nj) return;
+        Timer tm(this, 3);
+        JacobiJob* dj = upload(jobs);
+        int max_n = 1, max_m = 1;
+        for (auto& j : jobs) {
+            max_n = std::max(max_n, j.n);
+            max_m = std::max(max_m, j.m);
+        }
+        const int nb = (max_n + 15) / 16;
+        const int nbe = nb + (nb & 1);
+        const int rounds = nbe - 1, pairs = nbe / 2;
+        const double tol = 1e-15 * std::sqrt((double)max_m);
+        int* d_rot = d_flags;
+        int* d_done = d_flags + nj;
+        wqc::launch_fill_int(d_done, 0, nj, stream);
+        std::vector<int> done(nj, 0);
+        int active = nj;
+        for (int sweep = 0; sweep < MAX_SWEEPS && active > 0; ++sweep) {
+            check_cancel();
+            wqc::launch_fill_int(d_rot, 0, nj, stream);
+            for (int r = 0; r < rounds; ++r)
+                wqc::launch_jacobi_round(dj, nj, pairs, r, tol, d_rot, d_done, stream);
+            check_launch();
+            stats.jacobi_sweeps += active;
+            HIPC(hipMemcpyAsync(hread, d_rot, nj * sizeof(int), hipMemcpyDeviceToHost, stream));
+            sync();
+            const int* rot = reinterpret_cast<const int*>(hread);
+            bool changed = false;
+            for (int j = 0; j < nj; ++j)
+                if (!done[j] && !rot[j]) {
+                    done[j] = 1;
+                    active--;
+                    changed = true;
+                }
+            if (changed && active > 0)
+                HIPC(hipMemcpyAsync(d_done, upload_ints(done), nj * sizeof(int), hipMemcpyDeviceToDevice, stream));
+        }
+        std::vector<long> offs = sig_off;
+        long* doff = upload(offs);
+        wqc::launch_row_norms(dj, nj, max_n, d_sigma, doff, stream);
+        check_launch();
+    }
+
+    int* upload_ints(const std::vector<int>& v) { return upload(v); }
+
+    // Frobenius^2 (deterministic two-stage sum) then checked_svd's dust rule, per matrix.
+    void dust(const std::vector<double2*>& mats, const std::vector<long>& counts, double* d_fro2,
+              double* d_partial, int nblk) {
+        const int nj = (int)mats.size();
+        if (!nj) return;
+        double2** dp = upload(mats);
+        long* dc = upload(counts);
+        long mx = 1;
+        for (long c : counts) mx = std::max(mx, c);
+        wqc::launch_sumsq(dp, dc, d_partial, nj, nblk, stream);
+        wqc::launch_sumsq_final(d_partial, d_fro2, nj, nblk, stream);
+        wqc::launch_dust(dp, dc, d_fro2, DUST_REL, nj, mx, stream);
+        check_launch();
+    }
+
+    // ------------------------------------------------------------ two-site batch
+    struct JobInfo {
+        int p, dl, bond, dr, rows, cols;
+        bool rsvd;
+        size_t ell;
+        // workspace offsets (bytes)
+        size_t o_theta, o_thetaT, o_J, o_sig;
+        size_t o_Y0, o_Z, o_Y, o_V, o_T, o_Z2, o_B, o_JB, o_sigB, o_W, o_W2;
+        OmegaEntry* om;
+        size_t chi_kept;
+        bool fallback;
+    };
+
+    void run_units(std::vector<Unit>& units) {
+        if (units.empty()) return;
+        check_cancel();
+        sync();
+        tables_reset();
+        const int nu = (int)units.size();
+        std::vector<JobInfo> J(nu);
+        size_t off = 0;
+        auto take = [&](size_t bytes) {
+            size_t o = (off + 255) & ~(size_t)255;
+            off = o + bytes;
+            return o;
+        };
+        const size_t C16 = sizeof(double2);
+        for (int i = 0; i < nu; ++i) {
+            JobInfo& j = J[i];
+            j.p = units[i].p;
+            const Site& L = sites[j.p];
+            const Site& R = sites[j.p + 1];
+            if (L.r != R.l) fail(WQC_TN_EXEC_FAILED, "MPS bond layout mismatch between sites");
+            j.dl = L.l;
+            j.bond = L.r;
+            j.dr = R.r;
+            j.rows = 2 * j.dl;
+            j.cols = 2 * j.dr;
+            const size_t rows = j.rows, cols = j.cols;
+            j.ell = std::min(std::min(chi + RSVD_OVERSAMPLE, rows), cols);
+            j.rsvd = !reject && !(j.ell == 0 || j.ell >= std::min(rows, cols) || !rsvd_worth(rows, cols, j.ell));
+            j.om = nullptr;
+            j.fallback = false;
+            const size_t n_ex = std::min(rows, cols);
+            j.o_theta = take(rows * cols * C16);
+            j.o_J = take(n_ex * n_ex * C16);
+            j.o_sig = take(std::max(rows, cols) * sizeof(double));
+            if (j.rsvd) {
+                j.om = &omega_for(dev, rows, cols, j.ell);
+                const size_t r = std::max(1, j.om->r);
+                j.o_thetaT = rows > cols ? take(rows * cols * C16) : 0;
+                j.o_Y0 = take(rows * r * C16);
+                j.o_Z = take(cols * r * C16);
+                j.o_Y = take(rows * r * C16);
+                j.o_V = take(rows * r * C16);
+                j.o_T = take(r * r * C16);
+                j.o_Z2 = take(r * cols * C16);
+                j.o_B = take(j.ell * cols * C16);
+                j.o_JB = take(j.ell * j.ell * C16);
+                j.o_sigB = take(j.ell * sizeof(double));
+                j.o_W = take(j.ell * std::min(chi, j.ell) * C16);
+                j.o_W2 = take(r * std::min(chi, j.ell) * C16 * 2);
+            }
+        }
+        const int nblk = 64;
+        const size_t o_flags = take(4 * (size_t)nu * sizeof(int));
+        const size_t o_fro = take((size_t)nu * 2 * sizeof(double));
+        const size_t o_part = take((size_t)nu * 2 * nblk * sizeof(double));
+        ws.ensure(off);
+        note_peak();
+        char* base = (char*)ws.p;
+        auto P = [&](size_t o) { return reinterpret_cast<double2*>(base + o); };
+        auto PD = [&](size_t o) { return reinterpret_cast<double*>(base + o); };
+        int* d_flags = reinterpret_cast<int*>(base + o_flags);
+        double* d_fro = PD(o_fro);
+        double* d_part = PD(o_part);
+
+        // 1. merge + U (exact jobs with rows > cols store theta'^H for the row Jacobi).
+        {
+            Timer tm(this, 1);
+            std::vector<MergeJob> mj(nu);
+            int mdl = 1, mdr = 1;
+            for (int i = 0; i < nu; ++i) {
+                JobInfo& j = J[i];
+                mj[i].L = sites[j.p].ptr;
+                mj[i].R = sites[j.p + 1].ptr;
+                mj[i].out = P(j.o_theta);
+                mj[i].dl = j.dl;
+                mj[i].bond = j.bond;
+                mj[i].dr = j.dr;
+                mj[i].transpose = (!j.rsvd && j.rows > j.cols) ? 1 : 0;
+                std::memcpy(mj[i].u4, units[i].u4, sizeof(mj[i].u4));
+                mdl = std::max(mdl, j.dl);
+                mdr = std::max(mdr, j.dr);
+            }
+            wqc::launch_merge(upload(mj), nu, mdl, mdr, stream);
+            check_launch();
+        }
+        stats.two_qubit_units += nu;
+
+        std::vector<int> exact_ids, rsvd_ids;
+        for (int i = 0; i < nu; ++i) (J[i].rsvd ? rsvd_ids : exact_ids).push_back(i);
+        if (!rsvd_ids.empty()) run_rsvd(J, rsvd_ids, base, d_flags, d_fro, d_part, nblk);
+        for (int i : rsvd_ids)
+            if (J[i].fallback) exact_ids.push_back(i);
+        if (!exact_ids.empty()) run_exact(J, exact_ids, base, d_flags, d_fro + nu, d_part + (size_t)nu * nblk, nblk);
+        sync();
+    }
+
+    // Exact split (truncated_svd_split) for the listed jobs.
+    void run_exact(std::vector<JobInfo>& J, const std::vector<int>& ids, char* base, int* d_flags,
+                   double* d_fro, double* d_part, int nblk) {
+        auto P = [&](size_t o) { return reinterpret_cast<double2*>(base + o); };
+        auto PD = [&](size_t o) { return reinterpret_cast<double*>(base + o); };
+        const int ne = (int)ids.size();
+        // Fallback jobs from the range finder hold theta' untransposed.
+        for (int i : ids) {
+            JobInfo& j = J[i];
+            if (j.rsvd && j.rows > j.cols) {
+                wqc::launch_conj_transpose(P(j.o_theta), P(j.o_thetaT), j.rows, j.cols, stream);
+                check_launch();
+            }
+        }
+        std::vector<JacobiJob> jj(ne);
+        std::vector<double2*> mats(ne), Jm(ne);
+        std::vector<long> counts(ne), offs(ne);
+        std::vector<int> dims(ne);
+        int max_n = 1;
+        for (int k = 0; k < ne; ++k) {
+            JobInfo& j = J[ids[k]];
+            const bool tall = j.rows > j.cols;
+            double2* X = (j.rsvd && tall) ? P(j.o_thetaT) : P(j.o_theta);
+            jj[k].X = X;
+            jj[k].J = P(j.o_J);
+            jj[k].n = tall ? j.cols : j.rows;
+            jj[k].m = tall ? j.rows : j.cols;
+            jj[k].ldx = jj[k].m;
+            mats[k] = X;
+            Jm[k] = P(j.o_J);
+            counts[k] = (long)j.rows * j.cols;
+            dims[k] = jj[k].n;
+            max_n = std::max(max_n, jj[k].n);
+            offs[k] = (long)((j.o_sig) / sizeof(double));
+        }
+        dust(mats, counts, d_fro, d_part, nblk);
+        wqc::launch_identity(upload(Jm), upload(dims), ne, max_n, stream);
+        check_launch();
+        // sigma offsets are relative to the workspace base viewed as doubles
+        jacobi(jj, PD(0), offs, d_flags);
+        // read back sigma and |theta|_F^2
+        size_t tot = 0;
+        for (int k = 0; k < ne; ++k) tot += jj[k].n;
+        if (tot + ne > hread_cap) fail(WQC_TN_EXEC_FAILED, "readback buffer too small");
+        size_t cur = 0;
+        std::vector<size_t> hoff(ne);
+        for (int k = 0; k < ne; ++k) {
+            hoff[k] = cur;
+            HIPC(hipMemcpyAsync(hread + cur, PD(J[ids[k]].o_sig), jj[k].n * sizeof(double), hipMemcpyDeviceToHost, stream));
+            cur += jj[k].n;
+        }
+        HIPC(hipMemcpyAsync(hread + cur, d_fro, ne * sizeof(double), hipMemcpyDeviceToHost, stream));
+        sync();
+        std::vector<FactorJob> fj(ne);
+        std::vector<std::vector<int>> perms(ne);
+        long mxl = 1, mxr = 1;
+        for (int k = 0; k < ne; ++k) {
+            JobInfo& j = J[ids[k]];
+            const int nn = jj[k].n;
+            std::vector<double> sig(hread + hoff[k], hread + hoff[k] + nn);
+            const double fro2 = hread[cur + k];
+            std::vector<int> order(nn);
+            for (int i = 0; i < nn; ++i) order[i] = i;
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sig[a] > sig[b]; });
+            std::vector<double> sorted(nn);
+            double s2 = 0.0;
+            for (int i = 0; i < nn; ++i) {
+                sorted[i] = sig[order[i]];
+                s2 += sorted[i] * sorted[i];
+            }
+            if (!(std::fabs(s2 - fro2) <= SVD_CHECK_TOL * std::max(fro2, 2.2250738585072014e-308) + 1e-28)) {
+                stats.svd_check_failures++;
+                fail(WQC_TN_EXEC_FAILED, "SVD of a " + std::to_string(j.rows) + "x" + std::to_string(j.cols) +
+                                             " matrix failed its check");
+            }
+            const size_t c = retained_chi(sorted, chi, j.rows, j.cols);
+            bool dropped = false;
+            double drop = 0.0;
+            for (int i = (int)c; i < nn; ++i) {
+                if (sorted[i] > SINGULAR_CUTOFF) dropped = true;
+                drop += sorted[i] * sorted[i];
+            }
+            if (reject && dropped) fail(WQC_TN_BOND_EXCEEDED, "mps bond exceeded chi");
+            if (dropped) stats.truncated_splits++;
+            const double dw = s2 > 0 ? drop / s2 : 0.0;
+            stats.max_discarded_weight = std::max(stats.max_discarded_weight, dw);
+            stats.sum_discarded_weight += dw;
+            stats.exact_splits++;
+            perms[k].assign(order.begin(), order.begin() + c);
+            j.chi_kept = c;
+            ensure_site_cap(j.p, (size_t)j.dl * 2 * c);
+            ensure_site_cap(j.p + 1, c * 2 * (size_t)j.dr);
+            const bool tall = j.rows > j.cols;
+            fj[k].X = jj[k].X;
+            fj[k].J = jj[k].J;
+            fj[k].left = sites[j.p].ptr;
+            fj[k].right = sites[j.p + 1].ptr;
+            fj[k].perm = upload(perms[k]);
+            fj[k].sigma = PD(j.o_sig);
+            fj[k].rows = j.rows;
+            fj[k].cols = j.cols;
+            fj[k].chi = (int)c;
+            fj[k].ldx = jj[k].ldx;
+            fj[k].ldj = nn;
+            fj[k].mode = tall ? 1 : 0;
+            mxl = std::max(mxl, (long)j.rows * (long)c);
+            mxr = std::max(mxr, (long)c * j.cols);
+        }
+        {
+            Timer tm(this, 4);
+            wqc::launch_factor(upload(fj), ne, mxl, mxr, stream);
+            check_launch();
+        }
+        for (int k = 0; k < ne; ++k) {
+            JobInfo& j = J[ids[k]];
+            sites[j.p].r = (int)j.chi_kept;
+            sites[j.p + 1].l = (int)j.chi_kept;
+        }
+    }
+
+    void gemm(std::vector<GemmJob>& g) {
+        if (g.empty()) return;
+        int mm = 1, mn = 1;
+        for (auto& x : g) {
+            mm = std::max(mm, x.M);
+            mn = std::max(mn, x.N);
+        }
+        wqc::launch_gemm(upload(g), (int)g.size(), mm, mn, stream);
+        check_launch();
+    }
+
+    static GemmJob gj(const double2* A, int lda, int opA, const double2* B, int ldb, int opB, double2* C,
+                      int ldc, int M, int N, int K, double alpha, double beta) {
+        GemmJob g;
+        g.A = A;
+        g.B = B;
+        g.C = C;
+        g.M = M;
+        g.N = N;
+        g.K = K;
+        g.lda = lda;
+        g.ldb = ldb;
+        g.ldc = ldc;
+        g.opA = opA;
+        g.opB = opB;
+        g.alpha = make_double2(alpha, 0);
+        g.beta = make_double2(beta, 0);
+        return g;
+    }
+
+    // Range finder (rsvd_sites) for the listed jobs; jobs whose cut is ambiguous are
+    // marked fallback for the exact split.
+    void run_rsvd(std::vector<JobInfo>& J, const std::vector<int>& ids, char* base, int* d_flags,
+                  double* d_fro, double* d_part, int nblk) {
+        auto P = [&](size_t o) { return reinterpret_cast<double2*>(base + o); };
+        auto PD = [&](size_t o) { return reinterpret_cast<double*>(base + o); };
+        const int nr = (int)ids.size();
+        {
+            Timer tm(this, 2);
+            std::vector<GemmJob> g1, g2, g3;
+            for (int i : ids) {
+                JobInfo& j = J[i];
+                const int r = j.om->r;
+                // Y0 = A Omega_I; Z = A^H Y0; Y = A Z
+                g1.push_back(gj(P(j.o_theta), j.cols, 0, j.om->omega_cols, r, 0, P(j.o_Y0), r, j.rows, r, j.cols, 1, 0));
+                g2.push_back(gj(P(j.o_theta), j.cols, 1, P(j.o_Y0), r, 0, P(j.o_Z), r, j.cols, r, j.rows, 1, 0));
+                g3.push_back(gj(P(j.o_theta), j.cols, 0, P(j.o_Z), r, 0, P(j.o_Y), r, j.rows, r, j.cols, 1, 0));
+            }
+            gemm(g1);
+            gemm(g2);
+            gemm(g3);
+            std::vector<HouseJob> hj;
+            for (int i : ids) {
+                JobInfo& j = J[i];
+                if (j.om->r > wqc::kMaxHouseCols) fail(WQC_TN_EXEC_FAILED, "range finder rank above the QR limit");
+                HouseJob h;
+                h.Y = P(j.o_Y);
+                h.V = P(j.o_V);
+                h.T = P(j.o_T);
+                h.pos = j.om->pos;
+                h.rows = j.rows;
+                h.r = j.om->r;
+                h.pad0 = h.pad1 = 0;
+                h.tau = QR_RANK_TOL;
+                hj.push_back(h);
+            }
+            wqc::launch_householder(upload(hj), (int)hj.size(), stream);
+            check_launch();
+            // B = A[0:ell, :] - V[0:ell, :] (T^H (V^H A))
+            std::vector<GemmJob> g4, g5, g6;
+            for (int i : ids) {
+                JobInfo& j = J[i];
+                const int r = j.om->r;
+                g4.push_back(gj(P(j.o_V), r, 1, P(j.o_theta), j.cols, 0, P(j.o_Z2), j.cols, r, j.cols, j.rows, 1, 0));
+                g5.push_back(gj(P(j.o_T), r, 1, P(j.o_Z2), j.cols, 0, P(j.o_Z), j.cols, r, j.cols, r, 1, 0));
+                HIPC(hipMemcpyAsync(P(j.o_B), P(j.o_theta), j.ell * (size_t)j.cols * sizeof(double2),
+                                    hipMemcpyDeviceToDevice, stream));
+                g6.push_back(gj(P(j.o_V), r, 0, P(j.o_Z), j.cols, 0, P(j.o_B), j.cols, (int)j.ell, j.cols, r, -1, 1));
+            }
+            gemm(g4);
+            gemm(g5);
+            gemm(g6);
+        }
+        // checked_svd(B): dust rule, Jacobi on the rows of B (ell <= cols), norms.
+        std::vector<JacobiJob> jj(nr);
+        std::vector<double2*> mats(nr), Jm(nr);
+        std::vector<long> counts(nr), offs(nr);
+        std::vector<int> dims(nr);
+        int max_n = 1;
+        for (int k = 0; k < nr; ++k) {
+            JobInfo& j = J[ids[k]];
+            jj[k].X = P(j.o_B);
+            jj[k].J = P(j.o_JB);
+            jj[k].n = (int)j.ell;
+            jj[k].m = j.cols;
+            jj[k].ldx = j.cols;
+            mats[k] = P(j.o_B);
+            Jm[k] = P(j.o_JB);
+            counts[k] = (long)j.ell * j.cols;
+            dims[k] = (int)j.ell;
+            max_n = std::max(max_n, (int)j.ell);
+            offs[k] = (long)(j.o_sigB / sizeof(double));
+        }
+        dust(mats, counts, d_fro, d_part, nblk);
+        wqc::launch_identity(upload(Jm), upload(dims), nr, max_n, stream);
+        check_launch();
+        jacobi(jj, PD(0), offs, d_flags);
+        size_t cur = 0;
+        std::vector<size_t> hoff(nr);
+        for (int k = 0; k < nr; ++k) {
+            hoff[k] = cur;
+            HIPC(hipMemcpyAsync(hread + cur, PD(J[ids[k]].o_sigB), jj[k].n * sizeof(double), hipMemcpyDeviceToHost, stream));
+            cur += jj[k].n;
+        }
+        HIPC(hipMemcpyAsync(hread + cur, d_fro, nr * sizeof(double), hipMemcpyDeviceToHost, stream));
+        sync();
+        std::vector<FactorJob> fj;
+        std::vector<int> kept;
+        std::vector<std::vector<int>> perms(nr);
+        long mxl = 1, mxr = 1;
+        for (int k = 0; k < nr; ++k) {
+            JobInfo& j = J[ids[k]];
+            const int nn = jj[k].n;
+            std::vector<double> sig(hread + hoff[k], hread + hoff[k] + nn);
+            const double fro2 = hread[cur + k];
+            std::vector<int> order(nn);
+            for (int i = 0; i < nn; ++i) order[i] = i;
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sig[a] > sig[b]; });
+            std::vector<double> values(nn);
+            double s2 = 0.0;
+            for (int i = 0; i < nn; ++i) {
+                values[i] = sig[order[i]];
+                s2 += values[i] * values[i];
+            }
+            const bool svd_ok = std::fabs(s2 - fro2) <= SVD_CHECK_TOL * std::max(fro2, 2.2250738585072014e-308) + 1e-28;
+            if (!svd_ok) stats.svd_check_failures++;
+            for (double& s : values)
+                if (s < SINGULAR_CUTOFF * 1e-2) s = 0.0;
+            const size_t c = retained_chi(values, chi, j.rows, j.cols);
+            if (!svd_ok || c == 0 || needs_f64_fallback(values, c)) {
+                j.fallback = true;
+                stats.rsvd_fallbacks++;
+                continue;
+            }
+            stats.rsvd_kept++;
+            bool dropped = false;
+            double drop = 0.0, tot = 0.0;
+            for (int i = 0; i < nn; ++i) {
+                tot += values[i] * values[i];
+                if (i >= (int)c) {
+                    drop += values[i] * values[i];
+                    if (values[i] > SINGULAR_CUTOFF) dropped = true;
+                }
+            }
+            if (dropped) stats.truncated_splits++;
+            const double dw = tot > 0 ? drop / tot : 0.0;
+            stats.max_discarded_weight = std::max(stats.max_discarded_weight, dw);
+            stats.sum_discarded_weight += dw;
+            perms[k].assign(order.begin(), order.begin() + c);
+            j.chi_kept = c;
+            ensure_site_cap(j.p, (size_t)j.dl * 2 * c);
+            ensure_site_cap(j.p + 1, c * 2 * (size_t)j.dr);
+            FactorJob f;
+            f.X = P(j.o_B);
+            f.J = P(j.o_JB);
+            f.left = P(j.o_W);
+            f.right = sites[j.p + 1].ptr;
+            f.perm = upload(perms[k]);
+            f.sigma = PD(j.o_sigB);
+            f.rows = j.rows;
+            f.cols = j.cols;
+            f.chi = (int)c;
+            f.ldx = j.cols;
+            f.ldj = (int)j.ell;
+            f.mode = 2;
+            f.pad0 = f.pad1 = 0;
+            fj.push_back(f);
+            kept.push_back(ids[k]);
+            mxl = std::max(mxl, (long)j.ell * (long)c);
+            mxr = std::max(mxr, (long)c * j.cols);
+        }
+        if (fj.empty()) return;
+        Timer tm(this, 4);
+        wqc::launch_factor(upload(fj), (int)fj.size(), mxl, mxr, stream);
+        check_launch();
+        // left = [W; 0] - V (T (V[0:ell, :]^H W))
+        std::vector<GemmJob> g7, g8, g9;
+        for (int i : kept) {
+            JobInfo& j = J[i];
+            const int r = j.om->r, c = (int)j.chi_kept;
+            double2* W2 = P(j.o_W2);
+            double2* W3 = W2 + (size_t)r * c;
+            g7.push_back(gj(P(j.o_V), r, 1, P(j.o_W), c, 0, W2, c, r, c, (int)j.ell, 1, 0));
+            g8.push_back(gj(P(j.o_T), r, 0, W2, c, 0, W3, c, r, c, r, 1, 0));
+            double2* left = sites[j.p].ptr;
+            HIPC(hipMemcpyAsync(left, P(j.o_W), j.ell * (size_t)c * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+            if ((size_t)j.rows > j.ell)
+                HIPC(hipMemsetAsync(left + j.ell * (size_t)c, 0, ((size_t)j.rows - j.ell) * c * sizeof(double2), stream));
+            g9.push_back(gj(P(j.o_V), r, 0, W3, c, 0, left, c, j.rows, c, r, -1, 1));
+        }
+        gemm(g7);
+        gemm(g8);
+        gemm(g9);
+        for (int i : kept) {
+            JobInfo& j = J[i];
+            sites[j.p].r = (int)j.chi_kept;
+            sites[j.p + 1].l = (int)j.chi_kept;
+        }
+    }
+
+    // ------------------------------------------------------------ scheduling
+    void flush() {
+        if (!pend1.empty()) {
+            std::vector<PendingOne> ops;
+            ops.swap(pend1);
+            run_one_qubit(ops);
+        }
+        if (!pend2.empty()) {
+            std::vector<Unit> units;
+            units.swap(pend2);
+            run_units(units);
+        }
+        std::fill(touched.begin(), touched.end(), 0);
+    }
+
+    void queue_one(int site, const Mat2 u) {
+        if (touched[site]) flush();
+        PendingOne op;
+        op.site = site;
+        for (int k = 0; k < 4; ++k) op.u[k] = u[k];
+        pend1.push_back(op);
+        touched[site] = 1;
+    }
+
+    void queue_unit(int p, const Mat4 u, bool ctrl_is_first) {
+        if (touched[p] || touched[p + 1]) flush();
+        check_cancel();
+        Unit un;
+        un.p = p;
+        resolve_u4(u, ctrl_is_first, un.u4);
+        pend2.push_back(un);
+        touched[p] = touched[p + 1] = 1;
+    }
+
+    // apply_two_qubit_gate (src/tn/mps.rs:353-375): bubble `right` down to left+1 with
+    // SWAP units, apply, bubble back.
+    void queue_two(int ctrl, int tgt, const Mat4 u) {
+        const int left = std::min(ctrl, tgt), right = std::max(ctrl, tgt);
+        const int span = right - left;
+        Mat4 sw;
+        two_qubit_matrix(KIND_SWAP, sw);
+        if (span > 1)
+            for (int pos = right - 1; pos >= left + 1; --pos) queue_unit(pos, sw, true);
+        queue_unit(left, u, ctrl < tgt);
+        if (span > 1)
+            for (int pos = left + 1; pos < right; ++pos) queue_unit(pos, sw, true);
+    }
+
+    void check_gate(const wqc_tn_gate& g) {
+        auto bad = [&](int q) { return q < 0 || q >= n; };
+        const int k = g.kind;
+        if (is_one_qubit(k) || k == WQC_TN_GATE_MEASURE || k == WQC_TN_GATE_RESET) {
+            if (bad(g.q0)) fail(WQC_TN_INVALID_ARG, "qubit index " + std::to_string(g.q0) + " out of bounds (limit: " + std::to_string(n) + ")");
+        } else if (k == WQC_TN_GATE_CNOT || k == WQC_TN_GATE_CZ) {
+            if (bad(g.q0) || bad(g.q1)) fail(WQC_TN_INVALID_ARG, "qubit index out of bounds");
+            if (g.q0 == g.q1) fail(WQC_TN_INVALID_ARG, "two-qubit gate on one qubit");
+        } else if (k == WQC_TN_GATE_CCNOT) {
+            if (bad(g.q0) || bad(g.q1) || bad(g.q2)) fail(WQC_TN_INVALID_ARG, "qubit index out of bounds");
+            if (g.q0 == g.q1 || g.q0 == g.q2 || g.q1 == g.q2) fail(WQC_TN_INVALID_ARG, "CCNOT needs three distinct qubits");
+        } else if (k != WQC_TN_GATE_IF) {
+            fail(WQC_TN_INVALID_ARG, "unknown gate kind " + std::to_string(k));
+        }
+    }
+
+    // apply_gate (src/tn/mps.rs:278-297), queued.
+    void queue_gate(const wqc_tn_gate& g) {
+        check_gate(g);
+        const int k = g.kind;
+        if (is_one_qubit(k)) {
+            Mat2 u;
+            unary_matrix(k, g.angle, u);
+            queue_one(g.q0, u);
+        } else if (k == WQC_TN_GATE_CNOT || k == WQC_TN_GATE_CZ) {
+            Mat4 u;
+            two_qubit_matrix(k, u);
+            queue_two(g.q0, g.q1, u);
+        } else if (k == WQC_TN_GATE_CCNOT) {
+            queue_ccnot(g.q0, g.q1, g.q2);
+        }
+    }
+
+    // apply_ccnot (src/tn/mps.rs:657-677).
+    void queue_ccnot(int c1, int c2, int t) {
+        Mat2 h, tg, tdg;
+        unary_matrix(WQC_TN_GATE_H, 0, h);
+        unary_matrix(WQC_TN_GATE_T, 0, tg);
+        unary_matrix(0, 0, tdg);
+        Mat4 cn;
+        two_qubit_matrix(WQC_TN_GATE_CNOT, cn);
+        queue_one(t, h);
+        queue_two(c2, t, cn);
+        queue_one(t, tdg);
+        queue_two(c1, t, cn);
+        queue_one(t, tg);
+        queue_two(c2, t, cn);
+        queue_one(t, tdg);
+        queue_two(c1, t, cn);
+        queue_one(t, tg);
+        queue_one(t, h);
+        queue_one(c2, tg);
+        queue_two(c1, c2, cn);
+        queue_one(c1, tg);
+        queue_one(c2, tdg);
+        queue_two(c1, c2, cn);
+    }
+
+    // The apply loop of execute_with_trace_using (src/tn/trace.rs:71-161).
+    void queue_circuit(const wqc_tn_gate* gates, size_t count) {
+        size_t i = 0;
+        while (i < count) {
+            const wqc_tn_gate& g = gates[i];
+            check_gate(g);
+            if (g.kind == WQC_TN_GATE_MEASURE || g.kind == WQC_TN_GATE_RESET || g.kind == WQC_TN_GATE_IF) {
+                ++i;
+                continue;
+            }
+            if (g.kind == WQC_TN_GATE_RX || g.kind == WQC_TN_GATE_RY || g.kind == WQC_TN_GATE_RZ) {
+                size_t run = 1;
+                double sum = g.angle;
+                while (i + run < count && gates[i + run].kind == g.kind && gates[i + run].q0 == g.q0) {
+                    sum += gates[i + run].angle;
+                    run++;
+                }
+                const double tau = 6.283185307179586;
+                double red = std::fmod(sum, tau);
+                if (red < 0) red += tau;
+                const bool identity = std::fabs(red) < 1e-9 || std::fabs(tau - red) < 1e-9;
+                if (identity || run == 1) {
+                    for (size_t k = 0; k < run; ++k) queue_gate(gates[i + k]);
+                } else {
+                    wqc_tn_gate net = g;
+                    net.angle = sum;
+                    queue_gate(net);
+                }
+                i += run;
+                continue;
+            }
+            queue_gate(g);
+            ++i;
+        }
+    }
+
+    // ------------------------------------------------------------ readouts
+    // Contract a vector through sites [from, to) (left to right) or (to, from] (right to
+    // left) with physical weight phys (0, 1 or 2 = sum). Returns the device vector.
+    double2* env_pass(int from, int to, int phys, bool right_to_left, int& len, double2* bufA,
+                      double2* bufB) {
+        HIPC(hipMemcpyAsync(bufA, &kOne, sizeof(double2), hipMemcpyHostToDevice, stream));
+        len = 1;
+        double2* cur = bufA;
+        double2* nxt = bufB;
+        std::vector<VecMatJob> jobs;
+        std::vector<int> outl;
+        if (!right_to_left) {
+            int vlen = 1;
+            for (int i = from; i < to; ++i) {
+                VecMatJob v{sites[i].ptr, cur, nxt, sites[i].l, sites[i].r, vlen, phys, 0, 0};
+                jobs.push_back(v);
+                outl.push_back(sites[i].r);
+                vlen = sites[i].r;
+                std::swap(cur, nxt);
+            }
+        } else {
+            int vlen = 1;
+            for (int i = from; i > to; --i) {
+                VecMatJob v{sites[i].ptr, cur, nxt, sites[i].l, sites[i].r, vlen, phys, 1, 0};
+                jobs.push_back(v);
+                outl.push_back(sites[i].l);
+                vlen = sites[i].l;
+                std::swap(cur, nxt);
+            }
+        }
+        if (!jobs.empty()) {
+            VecMatJob* d = upload(jobs);
+            for (size_t k = 0; k < jobs.size(); ++k) wqc::launch_vecmat(d + k, outl[k], stream);
+            check_launch();
+            len = outl.back();
+        }
+        return cur;
+    }
+    static constexpr double2 kOne = {1.0, 0.0};
+
+    void readout_buffers(double2*& a, double2*& b, double2*& c, double2*& d) {
+        size_t mx = 2;
+        for (auto& s : sites) mx = std::max(mx, (size_t)std::max(s.l, s.r) + 2);
+        ws.ensure(4 * mx * sizeof(double2) + 4096);
+        note_peak();
+        double2* base = (double2*)ws.p;
+        a = base;
+        b = base + mx;
+        c = base + 2 * mx;
+        d = base + 3 * mx;
+    }
+
+    std::pair<double, double> amplitude_zero() {
+        flush();
+        sync();
+        tables_reset();
+        Timer tm(this, 5);
+        double2 *a, *b, *c, *d;
+        readout_buffers(a, b, c, d);
+        int len = 0;
+        double2* v = env_pass(0, n, 0, false, len, a, b);
+        double2 out;
+        HIPC(hipMemcpyAsync(&out, v, sizeof(double2), hipMemcpyDeviceToHost, stream));
+        sync();
+        (void)c;
+        (void)d;
+        return {out.x, out.y};
+    }
+
+    // site_amplitudes (src/tn/mps.rs:679-693): left/right environments with the all-ones
+    // physical vector, then a_s = sum_{l,r} le[l] site[l,s,r] re[r].
+    void site_amplitudes(int t, double out[4]) {
+        flush();
+        sync();
+        tables_reset();
+        double2 *a, *b, *c, *d;
+        readout_buffers(a, b, c, d);
+        int llen = 0, rlen = 0;
+        double2* le = env_pass(0, t, 2, false, llen, a, b);
+        double2* re = env_pass(n - 1, t, 2, true, rlen, c, d);
+        std::vector<VecMatJob> jobs;
+        double2* scratch0 = (le == a) ? b : a;
+        // u_s = le . site[:, s, :]  (length site.r), then a_s = u_s . re
+        double2 *u0 = scratch0, *u1 = nullptr;
+        double2* freec = (re == c) ? d : c;
+        u1 = freec;
+        VecMatJob j0{sites[t].ptr, le, u0, sites[t].l, sites[t].r, llen, 0, 0, 0};
+        VecMatJob j1{sites[t].ptr, le, u1, sites[t].l, sites[t].r, llen, 1, 0, 0};
+        jobs.push_back(j0);
+        jobs.push_back(j1);
+        VecMatJob* dj = upload(jobs);
+        wqc::launch_vecmat(dj, sites[t].r, stream);
+        wqc::launch_vecmat(dj + 1, sites[t].r, stream);
+        check_launch();
+        const int rr = sites[t].r;
+        std::vector<double2> hu0(rr), hu1(rr), hre(std::max(rlen, 1));
+        HIPC(hipMemcpyAsync(hu0.data(), u0, rr * sizeof(double2), hipMemcpyDeviceToHost, stream));
+        HIPC(hipMemcpyAsync(hu1.data(), u1, rr * sizeof(double2), hipMemcpyDeviceToHost, stream));
+        HIPC(hipMemcpyAsync(hre.data(), re, rlen * sizeof(double2), hipMemcpyDeviceToHost, stream));
+        sync();
+        double2 a0 = make_double2(0, 0), a1 = make_double2(0, 0);
+        for (int r = 0; r < rr; ++r) {
+            const double2 w = r < rlen ? hre[r] : make_double2(1, 0);
+            a0.x += hu0[r].x * w.x - hu0[r].y * w.y;
+            a0.y += hu0[r].x * w.y + hu0[r].y * w.x;
+            a1.x += hu1[r].x * w.x - hu1[r].y * w.y;
+            a1.y += hu1[r].x * w.y + hu1[r].y * w.x;
+        }
+        out[0] = a0.x;
+        out[1] = a0.y;
+        out[2] = a1.x;
+        out[3] = a1.y;
+    }
+
+    // contract_to_statevector (src/tn/mps.rs:215-260).
+    void statevector(double* out, size_t len_complex) {
+        flush();
+        sync();
+        if (n > 20) fail(WQC_TN_EXEC_FAILED, "full statevector projection requires qubit_count <= 20");
+        const size_t dim = (size_t)1 << n;
+        if (len_complex < dim) fail(WQC_TN_INVALID_ARG, "statevector buffer too small");
+        size_t mx = 2;
+        for (int i = 0; i < n; ++i) mx = std::max(mx, (size_t)sites[i].r * ((size_t)2 << i));
+        for (int i = 0; i < n; ++i) mx = std::max(mx, (size_t)sites[i].l * ((size_t)1 << i));
+        ws.ensure(2 * mx * sizeof(double2));
+        note_peak();
+        double2* a = (double2*)ws.p;
+        double2* b = a + mx;
+        HIPC(hipMemcpyAsync(a, &kOne, sizeof(double2), hipMemcpyHostToDevice, stream));
+        for (int i = 0; i < n; ++i) {
+            const long prefix = 1L << i;
+            if (i > 0 && sites[i - 1].r != sites[i].l)
+                fail(WQC_TN_EXEC_FAILED, "MPS bond layout mismatch during statevector projection");
+            if (i == 0 && sites[0].l != 1)
+                fail(WQC_TN_EXEC_FAILED, "MPS bond layout mismatch during statevector projection");
+            wqc::launch_statevector_step(a, sites[i].ptr, b, sites[i].l, sites[i].r, prefix, stream);
+            check_launch();
+            std::swap(a, b);
+        }
+        if (sites[n - 1].r != 1) fail(WQC_TN_EXEC_FAILED, "projected statevector length mismatch");
+        HIPC(hipMemcpyAsync(out, a, dim * sizeof(double2), hipMemcpyDeviceToHost, stream));
+        sync();
+    }
+};
+
+constexpr double2 wqc_tn_state::kOne;
+
+// =====================================================================================
+// C ABI
+// =====================================================================================
+extern "C" {
+
+const char* wqc_tn_last_error(void) { return g_last_error.c_str(); }
+
+const char* wqc_tn_backend_from_env(void) {
+    const char* v = std::getenv("WQC_TN_BACKEND");
+    if (!v) return "cpu";
+    std::string s(v);
+    for (auto& ch : s) ch = (char)std::tolower((unsigned char)ch);
+    if (s == "hip" || s == "gpu" || s == "rocm") return "hip";
+    return "cpu";
+}
+
+int wqc_tn_engine_status(char* active, size_t cap, uint64_t* vram_budget_bytes) {
+    return guarded([&] {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) fail(WQC_TN_NO_DEVICE, "no HIP device");
+        hipDeviceProp_t prop;
+        HIPC(hipGetDeviceProperties(&prop, 0));
+        std::string s = std::string("hip ") + prop.gcnArchName;
+        if (active && cap) {
+            std::strncpy(active, s.c_str(), cap - 1);
+            active[cap - 1] = 0;
+        }
+        size_t fr = 0, tot = 0;
+        HIPC(hipSetDevice(0));
+        HIPC(hipMemGetInfo(&fr, &tot));
+        if (vram_budget_bytes) *vram_budget_bytes = fr;
+    });
+}
+
+int wqc_tn_device_open(int ordinal, wqc_tn_device** out) {
+    return guarded([&] {
+        if (!out) fail(WQC_TN_INVALID_ARG, "null output");
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) fail(WQC_TN_NO_DEVICE, "no HIP device");
+        if (ordinal < 0 || ordinal >= count) fail(WQC_TN_INVALID_ARG, "device ordinal out of range");
+        auto d = std::make_unique<wqc_tn_device>();
+        d->ordinal = ordinal;
+        HIPC(hipSetDevice(ordinal));
+        HIPC(hipGetDeviceProperties(&d->prop, ordinal));
+        if (std::strncmp(d->prop.gcnArchName, "gfx950", 6) != 0)
+            fail(WQC_TN_NO_DEVICE, std::string("device is ") + d->prop.gcnArchName + ", this build targets gfx950");
+        *out = d.release();
+    });
+}
+
+void wqc_tn_device_close(wqc_tn_device* dev) { delete dev; }
+
+int wqc_tn_state_new(wqc_tn_device* dev, size_t qubit_count, size_t max_bond_dim, int precision,
+                     wqc_tn_state** out) {
+    return guarded([&] {
+        if (!dev || !out) fail(WQC_TN_INVALID_ARG, "null argument");
+        if (qubit_count == 0) fail(WQC_TN_INVALID_ARG, "invalid qubit count: 0");
+        if (max_bond_dim == 0) fail(WQC_TN_EXEC_FAILED, "MPS max bond dimension must be > 0");
+        if (qubit_count > (1u << 20)) fail(WQC_TN_INVALID_ARG, "qubit count too large");
+        if (precision != WQC_TN_PRECISION_FP64 && precision != WQC_TN_PRECISION_MIXED_FP32)
+            fail(WQC_TN_INVALID_ARG, "unknown precision mode");
+        // ensure_mps_workspace_budget analogue (src/tn/mps.rs:26-48): N chi^2 32 bytes
+        // against free device memory.
+        HIPC(hipSetDevice(dev->ordinal));
+        size_t fr = 0, tot = 0;
+        HIPC(hipMemGetInfo(&fr, &tot));
+        unsigned __int128 need = (unsigned __int128)qubit_count * max_bond_dim * max_bond_dim * 32;
+        if (need > fr)
+            fail(WQC_TN_OUT_OF_MEMORY, "insufficient device memory: required " +
+                                           std::to_string((unsigned long long)std::min<unsigned __int128>(need, ~0ull)) +
+                                           " bytes, available " + std::to_string(fr));
+        auto st = std::make_unique<wqc_tn_state>();
+        st->init(dev, qubit_count, max_bond_dim, precision);
+        *out = st.release();
+    });
+}
+
+void wqc_tn_state_free(wqc_tn_state* st) { delete st; }
+
+size_t wqc_tn_qubit_count(const wqc_tn_state* st) { return st ? (size_t)st->n : 0; }
+
+int wqc_tn_set_max_bond_dim(wqc_tn_state* st, size_t max_bond_dim) {
+    return guarded([&] {
+        if (!st || max_bond_dim == 0) fail(WQC_TN_INVALID_ARG, "bad max bond dim");
+        st->flush();
+        st->chi = max_bond_dim;
+    });
+}
+
+int wqc_tn_set_reject_truncation(wqc_tn_state* st, int reject) {
+    return guarded([&] {
+        if (!st) fail(WQC_TN_INVALID_ARG, "null state");
+        st->flush();
+        st->reject = reject != 0;
+    });
+}
+
+int wqc_tn_apply_gate(wqc_tn_state* st, const wqc_tn_gate* gate) {
+    return guarded([&] {
+        if (!st || !gate) fail(WQC_TN_INVALID_ARG, "null argument");
+        HIPC(hipSetDevice(st->dev->ordinal));
+        st->queue_gate(*gate);
+        st->flush();
+    });
+}
+
+int wqc_tn_apply_circuit(wqc_tn_state* st, const wqc_tn_gate* gates, size_t count) {
+    return guarded([&] {
+        if (!st || (!gates && count)) fail(WQC_TN_INVALID_ARG, "null argument");
+        HIPC(hipSetDevice(st->dev->ordinal));
+        try {
+            st->queue_circuit(gates, count);
+            st->flush();
+        } catch (...) {
+            st->pend1.clear();
+            st->pend2.clear();
+            std::fill(st->touched.begin(), st->touched.end(), 0);
+            throw;
+        }
+    });
+}
+
+int wqc_tn_apply_adjacent_disjoint_gates(wqc_tn_state* st, const wqc_tn_gate* gates, size_t count) {
+    return guarded([&] {
+        if (!st || (!gates && count)) fail(WQC_TN_INVALID_ARG, "null argument");
+        HIPC(hipSetDevice(st->dev->ordinal));
+        st->check_cancel();
+        if (count < 2) {
+            for (size_t i = 0; i < count; ++i) st->queue_gate(gates[i]);
+            st->flush();
+            return;
+        }
+        std::vector<char> seen(st->n, 0);
+        for (size_t i = 0; i < count; ++i) {
+            const wqc_tn_gate& g = gates[i];
+            if (g.kind != WQC_TN_GATE_CNOT && g.kind != WQC_TN_GATE_CZ)
+                fail(WQC_TN_EXEC_FAILED, "disjoint layer contains a gate that is not CNOT or CZ");
+            st->check_gate(g);
+            const int p = std::min(g.q0, g.q1), q = std::max(g.q0, g.q1);
+            if (q != p + 1) fail(WQC_TN_INVALID_ARG, "disjoint layer gate is not adjacent");
+            if (seen[p] || seen[q]) fail(WQC_TN_INVALID_ARG, "disjoint layer gates share a qubit");
+            seen[p] = seen[q] = 1;
+        }
+        st->flush();
+        for (size_t i = 0; i < count; ++i) {
+            Mat4 u;
+            two_qubit_matrix(gates[i].kind, u);
+            st->queue_unit(std::min(gates[i].q0, gates[i].q1), u, gates[i].q0 < gates[i].q1);
+        }
+        st->flush();
+    });
+}
+
+int wqc_tn_apply_unary_at(wqc_tn_state* st, size_t site, const wqc_tn_gate* gate) {
+    return guarded([&] {
+        if (!st || !gate || site >= (size_t)st->n || !is_one_qubit(gate->kind))
+            fail(WQC_TN_INVALID_ARG, "bad unary_at arguments");
+        Mat2 u;
+        unary_matrix(gate->kind, gate->angle, u);
+        st->queue_one((int)site, u);
+        st->flush();
+    });
+}
+
+int wqc_tn_apply_tdg_at(wqc_tn_state* st, size_t site) {
+    return guarded([&] {
+        if (!st || site >= (size_t)st->n) fail(WQC_TN_INVALID_ARG, "bad site");
+        Mat2 u;
+        unary_matrix(0, 0, u);
+        st->queue_one((int)site, u);
+        st->flush();
+    });
+}
+
+static int pair_at(wqc_tn_state* st, size_t site, int kind, int ctrl_on_left) {
+    return guarded([&] {
+        if (!st || site + 1 >= (size_t)st->n) fail(WQC_TN_INVALID_ARG, "bad site");
+        Mat4 u;
+        two_qubit_matrix(kind, u);
+        st->queue_unit((int)site, u, ctrl_on_left != 0);
+        st->flush();
+    });
+}
+
+int wqc_tn_apply_swap_at(wqc_tn_state* st, size_t site) { return pair_at(st, site, KIND_SWAP, 1); }
+
+int wqc_tn_apply_pair_at(wqc_tn_state* st, size_t site, const wqc_tn_gate* gate, int ctrl_on_left) {
+    if (!gate || (gate->kind != WQC_TN_GATE_CNOT && gate->kind != WQC_TN_GATE_CZ)) {
+        g_last_error = "apply_pair_at takes CNOT or CZ";
+        return WQC_TN_INVALID_ARG;
+    }
+    return pair_at(st, site, gate->kind, ctrl_on_left);
+}
+
+int wqc_tn_apply_cnot_at(wqc_tn_state* st, size_t site, int ctrl_on_left) {
+    return pair_at(st, site, WQC_TN_GATE_CNOT, ctrl_on_left);
+}
+
+void wqc_tn_request_cancel(wqc_tn_state* st) {
+    if (st) st->cancel.store(1);
+}
+
+void wqc_tn_clear_cancel(wqc_tn_state* st) {
+    if (st) st->cancel.store(0);
+}
+
+int wqc_tn_amplitude_at_compact_zero(wqc_tn_state* st, double out[2]) {
+    return guarded([&] {
+        if (!st || !out) fail(WQC_TN_INVALID_ARG, "null argument");
+        auto a = st->amplitude_zero();
+        out[0] = a.first;
+        out[1] = a.second;
+    });
+}
+
+int wqc_tn_site_amplitudes(wqc_tn_state* st, size_t target, double out[4]) {
+    return guarded([&] {
+        if (!st || !out || target >= (size_t)st->n) fail(WQC_TN_INVALID_ARG, "bad target");
+        st->site_amplitudes((int)target, out);
+    });
+}
+
+int wqc_tn_control_probability(wqc_tn_state* st, size_t control, double* out) {
+    return guarded([&] {
+        if (!st || !out || control >= (size_t)st->n) fail(WQC_TN_INVALID_ARG, "bad control");
+        double a[4];
+        st->site_amplitudes((int)control, a);
+        *out = a[2] * a[2] + a[3] * a[3];
+    });
+}
+
+int wqc_tn_contract_to_statevector(wqc_tn_state* st, double* out, size_t len_complex) {
+    return guarded([&] {
+        if (!st || !out) fail(WQC_TN_INVALID_ARG, "null argument");
+        st->statevector(out, len_complex);
+    });
+}
+
+int wqc_tn_bond_dims(wqc_tn_state* st, size_t* out, size_t len) {
+    return guarded([&] {
+        if (!st || !out) fail(WQC_TN_INVALID_ARG, "null argument");
+        st->flush();
+        for (int i = 0; i < st->n && (size_t)i < len; ++i) out[i] = (size_t)st->sites[i].r;
+    });
+}
+
+int wqc_tn_download_site(wqc_tn_state* st, size_t site, double* out, size_t len_complex, size_t* l,
+                         size_t* r) {
+    return guarded([&] {
+        if (!st || site >= (size_t)st->n) fail(WQC_TN_INVALID_ARG, "bad site");
+        st->flush();
+        st->sync();
+        const Site& s = st->sites[site];
+        if (l) *l = (size_t)s.l;
+        if (r) *r = (size_t)s.r;
+        const size_t cnt = (size_t)s.l * 2 * s.r;
+        if (out && len_complex >= cnt)
+            HIPC(hipMemcpy(out, s.ptr, cnt * sizeof(double2), hipMemcpyDeviceToHost));
+    });
+}
+
+int wqc_tn_upload_site(wqc_tn_state* st, size_t site, const double* data, size_t l, size_t r) {
+    return guarded([&] {
+        if (!st || !data || site >= (size_t)st->n || l == 0 || r == 0) fail(WQC_TN_INVALID_ARG, "bad site upload");
+        if (l > (1u << 16) || r > (1u << 16)) fail(WQC_TN_INVALID_ARG, "bond too large");
+        st->flush();
+        st->sync();
+        st->ensure_site_cap((int)site, l * 2 * r);
+        HIPC(hipMemcpy(st->sites[site].ptr, data, l * 2 * r * sizeof(double2), hipMemcpyHostToDevice));
+        st->sites[site].l = (int)l;
+        st->sites[site].r = (int)r;
+    });
+}
+
+const char* wqc_tn_backend_label(const wqc_tn_state*) { return "hip"; }
+
+uint64_t wqc_tn_peak_vram_bytes(const wqc_tn_state* st) { return st ? st->stats.peak_vram_bytes : 0; }
+
+int wqc_tn_get_stats(const wqc_tn_state* st, wqc_tn_stats* out) {
+    if (!st || !out) {
+        g_last_error = "null argument";
+        return WQC_TN_INVALID_ARG;
+    }
+    *out = st->stats;
+    return WQC_TN_OK;
+}
+
+int wqc_tn_kernel_times(const wqc_tn_state* st, double* ms, size_t len) {
+    if (!st || !ms) return 0;
+    size_t k = std::min<size_t>(len, 6);
+    for (size_t i = 0; i < k; ++i) ms[i] = st->kms[i];
+    return (int)k;
+}
+
+int wqc_tn_enable_timing(wqc_tn_state* st, int enable) {
+    if (!st) return WQC_TN_INVALID_ARG;
+    st->timing = enable != 0;
+    return WQC_TN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,121 @@
+// kernels.h — device job descriptors and launch wrappers of the gfx950 MPS kernels.
+//
+// Complex numbers are double2 (x = re, y = im). Matrices are row-major. Every launch
+// wrapper takes a device-resident job table (one entry per independent job of a
+// layer) and enqueues on the given stream; none of them synchronizes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wqc {
+
+// One-qubit update of one site (l, 2, r) in place: site[a,s',b] = sum_s U[s'][s] site[a,s,b].
+struct OneQubitJob {
+    double2* site;
+    int l, r;
+    double2 u[4];  // U[0][0], U[0][1], U[1][0], U[1][1]
+};
+
+// theta' = U4 . (L . R). L is (2dl x bond), R is (bond x 2dr), both row-major views of
+// sites (dl,2,bond) and (bond,2,dr). U4[(s',t')][(s,t)] is the two-qubit unitary with the
+// control ordering already resolved. out is (2dl x 2dr) row-major, or its conjugate
+// transpose (2dr x 2dl) when transpose != 0.
+struct MergeJob {
+    const double2* L;
+    const double2* R;
+    double2* out;
+    int dl, bond, dr, transpose;
+    double2 u4[16];
+};
+
+// Jacobi working matrix X (n x m, row stride ldx, n <= m): rows are rotated in place
+// until mutually orthogonal; J (n x n, row stride n) accumulates the rotations (X_final =
+// J X_orig) when non-null.
+struct JacobiJob {
+    double2* X;
+    double2* J;
+    int n, m, ldx, pad;
+};
+
+// Batched complex GEMM: C = alpha * op(A) * op(B) + beta * C, op in {N, C} (C = conjugate
+// transpose). M x N result, inner dimension K. Leading dimensions are row strides of the
+// stored matrices.
+struct GemmJob {
+    const double2* A;
+    const double2* B;
+    double2* C;
+    int M, N, K;
+    int lda, ldb, ldc;
+    int opA, opB;  // 0 = N, 1 = C
+    double2 alpha, beta;
+};
+
+// Factor writes of the split.
+//   mode 0 (exact, X = theta', n = rows): left[r,k] = conj(J[p_k, r]) sqrt(s),
+//                                          right[k,c] = X[p_k, c] / sqrt(s)
+//   mode 1 (exact, X = theta'^H, n = cols): left[r,k] = conj(X[p_k, r]) / sqrt(s),
+//                                           right[k,c] = J[p_k, c] sqrt(s)
+//   mode 2 (range finder): right[k,c] = X[p_k, c] / sqrt(s);
+//                          left (as W, ell x chi): W[t,k] = conj(J[p_k, t]) sqrt(s)
+// p_k = perm[k], s = sigma[p_k]; s == 0 writes zeros.
+struct FactorJob {
+    const double2* X;
+    const double2* J;
+    double2* left;
+    double2* right;
+    const int* perm;
+    const double* sigma;
+    int rows, cols, chi, ldx, ldj, mode, pad0, pad1;
+};
+
+// Rank-thresholded Householder QR of the independent columns of Y (range finder).
+// Y (rows x r, row stride r) is overwritten. Column k sits at position pos[k] of the
+// full ell-column matrix; its reflector acts on rows >= pos[k]. V (rows x r) receives
+// unit reflector vectors (zero for skipped columns), T (r x r) the compact-WY factor of
+// H_pos[0] ... H_pos[r-1] = I - V T V^H.
+constexpr int kMaxHouseCols = 1024;
+struct HouseJob {
+    double2* Y;
+    double2* V;
+    double2* T;
+    const int* pos;
+    int rows, r, pad0, pad1;
+    double tau;
+};
+
+// v_next[b] = sum_a v[a] * w(a, b) over one site (l, 2, r), for a < min(l, vlen), where
+// w = site[a,0,b] (phys = 0), site[a,1,b] (phys = 1) or their sum (phys = 2). With
+// right != 0 the contraction runs from the right: v_next[a] = sum_b w(a,b) v[b].
+struct VecMatJob {
+    const double2* site;
+    const double2* vin;
+    double2* vout;
+    int l, r, vlen, phys, right, pad;
+};
+
+void launch_one_qubit(const OneQubitJob* jobs, int njobs, int max_elems, hipStream_t s);
+void launch_merge(const MergeJob* jobs, int njobs, int max_dl, int max_dr, hipStream_t s);
+void launch_sumsq(const double2* const* ptrs, const long* counts, double* partial,
+                  int njobs, int nblk, hipStream_t s);
+void launch_sumsq_final(const double* partial, double* out, int njobs, int nblk, hipStream_t s);
+void launch_dust(double2* const* ptrs, const long* counts, const double* fro2, double rel,
+                 int njobs, long max_count, hipStream_t s);
+void launch_identity(double2* const* ptrs, const int* dims, int njobs, int max_dim,
+                     hipStream_t s);
+void launch_jacobi_round(const JacobiJob* jobs, int njobs, int max_pairs, int round,
+                         double tol, int* rotated, const int* done, hipStream_t s);
+void launch_row_norms(const JacobiJob* jobs, int njobs, int max_n, double* sigma,
+                      const long* sigma_off, hipStream_t s);
+void launch_gemm(const GemmJob* jobs, int njobs, int max_m, int max_n, hipStream_t s);
+void launch_factor(const FactorJob* jobs, int njobs, long max_left, long max_right,
+                   hipStream_t s);
+void launch_householder(const HouseJob* jobs, int njobs, hipStream_t s);
+void launch_vecmat(const VecMatJob* job, int out_len, hipStream_t s);
+void launch_fill_int(int* p, int value, int count, hipStream_t s);
+void launch_conj_transpose(const double2* src, double2* dst, int rows, int cols, hipStream_t s);
+void launch_add_identity(double2* C, int rows, int cols, int ldc, hipStream_t s);
+void launch_statevector_step(const double2* st, const double2* site, double2* next, int l,
+                             int r, long prefix, hipStream_t s);
+
+}  // namespace wqc

@@ -1,0 +1,766 @@
+// kernels.hip — gfx950 kernels of the MPS engine (FP64 complex, wave64).
+//
+// All kernels take a device job table so that every independent job of a brickwork
+// layer (up to ~50 two-site splits at 100 qubits) runs in one launch: grid.y (or grid.z)
+// indexes the job, and jobs smaller than the launch's largest exit early.
+#include "kernels.h"
+
+#include <math.h>
+
+namespace wqc {
+namespace {
+
+__device__ __forceinline__ double2 cz() { return make_double2(0.0, 0.0); }
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double k) { return make_double2(a.x * k, a.y * k); }
+// acc += a * b
+__device__ __forceinline__ void cmac(double2& acc, double2 a, double2 b) {
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(-a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(a.y, b.x, acc.y);
+}
+// acc += a * conj(b)
+__device__ __forceinline__ void cmacc(double2& acc, double2 a, double2 b) {
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(a.y, b.y, acc.x);
+    acc.y = fma(a.y, b.x, acc.y);
+    acc.y = fma(-a.x, b.y, acc.y);
+}
+// acc += conj(a) * b
+__device__ __forceinline__ void ccmac(double2& acc, double2 a, double2 b) {
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(-a.y, b.x, acc.y);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// ------------------------------------------------------------------ one-qubit update
+__global__ __launch_bounds__(256) void k_one_qubit(const OneQubitJob* __restrict__ jobs) {
+    const OneQubitJob jb = jobs[blockIdx.y];
+    const long total = (long)jb.l * jb.r;
+    const double2 u00 = jb.u[0], u01 = jb.u[1], u10 = jb.u[2], u11 = jb.u[3];
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const long a = e / jb.r, b = e - a * jb.r;
+        double2* p0 = jb.site + a * 2 * jb.r + b;
+        double2* p1 = p0 + jb.r;
+        const double2 v0 = *p0, v1 = *p1;
+        double2 o0 = cz(), o1 = cz();
+        cmac(o0, u00, v0);
+        cmac(o0, u01, v1);
+        cmac(o1, u10, v0);
+        cmac(o1, u11, v1);
+        *p0 = o0;
+        *p1 = o1;
+    }
+}
+
+// ------------------------------------------------------------------ merge + U
+// One workgroup owns 16 values of a and 16 values of b, i.e. theta rows {2a, 2a+1} and
+// columns {b, dr + b}: exactly the four entries (s, t) that the two-qubit unitary mixes.
+constexpr int MT = 16;
+constexpr int MK = 16;
+
+__global__ __launch_bounds__(256) void k_merge(const MergeJob* __restrict__ jobs) {
+    const MergeJob jb = jobs[blockIdx.z];
+    const int a0 = blockIdx.y * MT, b0 = blockIdx.x * MT;
+    if (a0 >= jb.dl || b0 >= jb.dr) return;
+    __shared__ double2 Ls[2 * MT][MK + 1];
+    __shared__ double2 Rs[MK][2 * MT + 1];
+    const int tid = threadIdx.x;
+    const int ta = tid >> 4, tb = tid & 15;
+    double2 acc00 = cz(), acc01 = cz(), acc10 = cz(), acc11 = cz();
+    const long ldl = jb.bond, ldr = 2L * jb.dr;
+    for (int k0 = 0; k0 < jb.bond; k0 += MK) {
+        for (int e = tid; e < 2 * MT * MK; e += 256) {
+            const int lr = e / MK, kk = e % MK;
+            const int a = a0 + (lr >> 1), s = lr & 1, k = k0 + kk;
+            Ls[lr][kk] = (a < jb.dl && k < jb.bond) ? jb.L[(long)(2 * a + s) * ldl + k] : cz();
+        }
+        for (int e = tid; e < MK * 2 * MT; e += 256) {
+            const int kk = e / (2 * MT), lc = e % (2 * MT);
+            const int t = lc / MT, b = b0 + (lc % MT), k = k0 + kk;
+            Rs[kk][lc] = (b < jb.dr && k < jb.bond) ? jb.R[(long)k * ldr + (long)t * jb.dr + b] : cz();
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < MK; ++kk) {
+            const double2 l0 = Ls[2 * ta][kk], l1 = Ls[2 * ta + 1][kk];
+            const double2 r0 = Rs[kk][tb], r1 = Rs[kk][MT + tb];
+            cmac(acc00, l0, r0);
+            cmac(acc01, l0, r1);
+            cmac(acc10, l1, r0);
+            cmac(acc11, l1, r1);
+        }
+        __syncthreads();
+    }
+    const int a = a0 + ta, b = b0 + tb;
+    if (a >= jb.dl || b >= jb.dr) return;
+    const double2 th[4] = {acc00, acc01, acc10, acc11};  // index s*2 + t
+#pragma unroll
+    for (int so = 0; so < 2; ++so)
+#pragma unroll
+        for (int to = 0; to < 2; ++to) {
+            double2 v = cz();
+#pragma unroll
+            for (int i = 0; i < 4; ++i) cmac(v, jb.u4[(so * 2 + to) * 4 + i], th[i]);
+            const long row = 2L * a + so, col = (long)to * jb.dr + b;
+            if (!jb.transpose)
+                jb.out[row * ldr + col] = v;
+            else
+                jb.out[col * (2L * jb.dl) + row] = cconj(v);
+        }
+}
+
+// ------------------------------------------------------------------ sums of squares
+__global__ __launch_bounds__(256) void k_sumsq(const double2* const* __restrict__ ptrs,
+                                               const long* __restrict__ counts,
+                                               double* __restrict__ partial, int nblk) {
+    const int job = blockIdx.y;
+    const double2* p = ptrs[job];
+    const long n = counts[job];
+    double acc = 0.0;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)nblk * 256) {
+        const double2 v = p[e];
+        acc = fma(v.x, v.x, acc);
+        acc = fma(v.y, v.y, acc);
+    }
+    acc = wave_sum(acc);
+    __shared__ double ws[4];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[(long)job * nblk + blockIdx.x] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+}
+
+__global__ void k_sumsq_final(const double* __restrict__ partial, double* __restrict__ out,
+                              int njobs, int nblk) {
+    const int job = blockIdx.x * blockDim.x + threadIdx.x;
+    if (job >= njobs) return;
+    double s = 0.0;
+    for (int i = 0; i < nblk; ++i) s += partial[(long)job * nblk + i];
+    out[job] = s;
+}
+
+// checked_svd's dust rule: entries with |z| < rel * ||A||_F become exactly zero.
+__global__ __launch_bounds__(256) void k_dust(double2* const* __restrict__ ptrs,
+                                              const long* __restrict__ counts,
+                                              const double* __restrict__ fro2, double rel) {
+    const int job = blockIdx.y;
+    double2* p = ptrs[job];
+    const long n = counts[job];
+    const double dust = sqrt(fro2[job]) * rel;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+        const double2 v = p[e];
+        if (hypot(v.x, v.y) < dust) p[e] = cz();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_identity(double2* const* __restrict__ ptrs,
+                                                  const int* __restrict__ dims) {
+    const int job = blockIdx.y;
+    double2* p = ptrs[job];
+    const long n = dims[job];
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n * n; e += (long)gridDim.x * 256) {
+        const long i = e / n, j = e - i * n;
+        p[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    }
+}
+
+// ------------------------------------------------------------------ block Jacobi round
+// One workgroup orthogonalizes the 2*JB rows of one block pair of one job: Gram matrix
+// G = P P^H of the pair (FP64), Hermitian eigensolve of G by parallel cyclic Jacobi in
+// LDS (16 disjoint rotations per step), then P <- T P and J_pair <- T J_pair with the
+// accumulated rotation T. A pair whose off-diagonal already satisfies
+// |G_ij| <= tol sqrt(G_ii G_jj) is left untouched; rotated[job] records that a rotation
+// was needed (the sweep is not converged).
+constexpr int JB = 16;
+constexpr int JP = 2 * JB;
+constexpr int JCH = 32;
+
+__global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restrict__ jobs,
+                                                      int round, double tol,
+                                                      int* __restrict__ rotated,
+                                                      const int* __restrict__ done) {
+    const int job = blockIdx.y;
+    if (done[job]) return;
+    const JacobiJob jb = jobs[job];
+    const int nb = (jb.n + JB - 1) / JB;
+    const int nbe = nb + (nb & 1);
+    if (round >= nbe - 1) return;
+    const int k = blockIdx.x;
+    if (k >= nbe / 2) return;
+    int bi, bj;
+    if (k == 0) {
+        bi = nbe - 1;
+        bj = round;
+    } else {
+        bi = (round + k) % (nbe - 1);
+        bj = (round - k + nbe - 1) % (nbe - 1);
+    }
+
+    __shared__ double2 G[JP][JP + 1];
+    __shared__ double2 T[JP][JP + 1];
+    __shared__ double2 P[JP][JCH + 1];
+    __shared__ int s_row[JP];
+    __shared__ int rp[16], rq[16], ract[16];
+    __shared__ double rcs[16], rsn[16], rtg[16], ra[16], rd[16];
+    __shared__ double2 rph[16];
+    __shared__ int s_any;
+
+    const int tid = threadIdx.x;
+    if (tid < JP) {
+        const int b = tid < JB ? bi : bj;
+        const int r = b * JB + (tid & (JB - 1));
+        s_row[tid] = (b < nb && r < jb.n) ? r : -1;
+    }
+    if (tid == 0) s_any = 0;
+    __syncthreads();
+
+    // Gram matrix, 2x2 entries per thread.
+    const int ti = tid >> 4, tj = tid & 15;
+    double2 g00 = cz(), g01 = cz(), g10 = cz(), g11 = cz();
+    for (int c0 = 0; c0 < jb.m; c0 += JCH) {
+        for (int e = tid; e < JP * JCH; e += 256) {
+            const int lr = e / JCH, lc = e % JCH;
+            const int gr = s_row[lr], gc = c0 + lc;
+            P[lr][lc] = (gr >= 0 && gc < jb.m) ? jb.X[(long)gr * jb.ldx + gc] : cz();
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int c = 0; c < JCH; ++c) {
+            const double2 a0 = P[2 * ti][c], a1 = P[2 * ti + 1][c];
+            const double2 b0 = P[2 * tj][c], b1 = P[2 * tj + 1][c];
+            cmacc(g00, a0, b0);
+            cmacc(g01, a0, b1);
+            cmacc(g10, a1, b0);
+            cmacc(g11, a1, b1);
+        }
+        __syncthreads();
+    }
+    G[2 * ti][2 * tj] = g00;
+    G[2 * ti][2 * tj + 1] = g01;
+    G[2 * ti + 1][2 * tj] = g10;
+    G[2 * ti + 1][2 * tj + 1] = g11;
+    __syncthreads();
+    {
+        bool off = false;
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+                const int i = 2 * ti + u, j = 2 * tj + v;
+                if (i < j) {
+                    const double2 c = G[i][j];
+                    const double g = hypot(c.x, c.y);
+                    if (g > 0.0 && g > tol * sqrt(G[i][i].x * G[j][j].x)) off = true;
+                }
+            }
+        if (off) s_any = 1;
+    }
+    __syncthreads();
+    if (!s_any) return;
+    if (tid == 0) rotated[job] = 1;
+
+    for (int e = tid; e < JP * JP; e += 256) {
+        const int i = e / JP, j = e % JP;
+        T[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    }
+    __syncthreads();
+
+    const double tol_in = 1e-15;
+    for (int sweep = 0; sweep < 16; ++sweep) {
+        if (tid == 0) s_any = 0;
+        __syncthreads();
+        for (int step = 0; step < JP - 1; ++step) {
+            if (tid < 16) {
+                int p, q;
+                if (tid == 0) {
+                    p = JP - 1;
+                    q = step;
+                } else {
+                    p = (step + tid) % (JP - 1);
+                    q = (step - tid + JP - 1) % (JP - 1);
+                }
+                const double a = G[p][p].x, d = G[q][q].x;
+                const double2 c = G[p][q];
+                const double g = hypot(c.x, c.y);
+                const int act = (g > 0.0) && (g > tol_in * sqrt(fmax(a, 0.0) * fmax(d, 0.0)));
+                rp[tid] = p;
+                rq[tid] = q;
+                ract[tid] = act;
+                if (act) {
+                    const double zeta = (d - a) / (2.0 * g);
+                    const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                    const double cs = 1.0 / sqrt(1.0 + t * t);
+                    rcs[tid] = cs;
+                    rsn[tid] = cs * t;
+                    rph[tid] = make_double2(c.x / g, c.y / g);
+                    rtg[tid] = t * g;
+                    ra[tid] = a;
+                    rd[tid] = d;
+                    s_any = 1;
+                }
+            }
+            __syncthreads();
+            // rows p, q of G and T: M = [[cs, -ph sn], [sn conj(ph), cs]]
+            for (int e = tid; e < 16 * JP * 2; e += 256) {
+                const int mat = e / (16 * JP), rem = e % (16 * JP);
+                const int kk = rem / JP, j = rem % JP;
+                if (!ract[kk]) continue;
+                const int p = rp[kk], q = rq[kk];
+                const double cs = rcs[kk], sn = rsn[kk];
+                const double2 ph = rph[kk];
+                double2(*M)[JP + 1] = mat ? T : G;
+                const double2 xp = M[p][j], xq = M[q][j];
+                M[p][j] = csub(cscale(xp, cs), cmul(cscale(ph, sn), xq));
+                M[q][j] = cadd(cmul(cscale(cconj(ph), sn), xp), cscale(xq, cs));
+            }
+            __syncthreads();
+            // columns p, q of G: G <- G M^H
+            for (int e = tid; e < 16 * JP; e += 256) {
+                const int kk = e / JP, i = e % JP;
+                if (!ract[kk]) continue;
+                const int p = rp[kk], q = rq[kk];
+                const double cs = rcs[kk], sn = rsn[kk];
+                const double2 ph = rph[kk];
+                const double2 xp = G[i][p], xq = G[i][q];
+                G[i][p] = csub(cscale(xp, cs), cmul(cscale(cconj(ph), sn), xq));
+                G[i][q] = cadd(cmul(cscale(ph, sn), xp), cscale(xq, cs));
+            }
+            __syncthreads();
+            if (tid < 16 && ract[tid]) {
+                const int p = rp[tid], q = rq[tid];
+                G[p][p] = make_double2(ra[tid] - rtg[tid], 0.0);
+                G[q][q] = make_double2(rd[tid] + rtg[tid], 0.0);
+                G[p][q] = cz();
+                G[q][p] = cz();
+            }
+            __syncthreads();
+        }
+        if (!s_any) break;
+    }
+
+    // P <- T P over the columns of X, then the rows of J.
+    for (int pass = 0; pass < 2; ++pass) {
+        double2* M = pass == 0 ? jb.X : jb.J;
+        if (!M) break;
+        const int cols = pass == 0 ? jb.m : jb.n;
+        const long ld = pass == 0 ? jb.ldx : jb.n;
+        for (int c0 = 0; c0 < cols; c0 += JCH) {
+            for (int e = tid; e < JP * JCH; e += 256) {
+                const int lr = e / JCH, lc = e % JCH;
+                const int gr = s_row[lr], gc = c0 + lc;
+                P[lr][lc] = (gr >= 0 && gc < cols) ? M[(long)gr * ld + gc] : cz();
+            }
+            __syncthreads();
+            for (int e = tid; e < JP * JCH; e += 256) {
+                const int lr = e / JCH, lc = e % JCH;
+                const int gr = s_row[lr], gc = c0 + lc;
+                if (gr < 0 || gc >= cols) continue;
+                double2 acc = cz();
+#pragma unroll 8
+                for (int j = 0; j < JP; ++j) cmac(acc, T[lr][j], P[j][lc]);
+                M[(long)gr * ld + gc] = acc;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// sigma[row] = ||X[row, :]||, one wave per row.
+__global__ __launch_bounds__(256) void k_row_norms(const JacobiJob* __restrict__ jobs,
+                                                   double* __restrict__ sigma,
+                                                   const long* __restrict__ off) {
+    const int job = blockIdx.y;
+    const JacobiJob jb = jobs[job];
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= jb.n) return;
+    const int lane = threadIdx.x & 63;
+    double acc = 0.0;
+    const double2* x = jb.X + (long)row * jb.ldx;
+    for (int c = lane; c < jb.m; c += 64) {
+        const double2 v = x[c];
+        acc = fma(v.x, v.x, acc);
+        acc = fma(v.y, v.y, acc);
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) sigma[off[job] + row] = sqrt(acc);
+}
+
+// ------------------------------------------------------------------ batched ZGEMM
+constexpr int GT = 32;
+constexpr int GK = 16;
+
+__global__ __launch_bounds__(256) void k_gemm(const GemmJob* __restrict__ jobs) {
+    const GemmJob jb = jobs[blockIdx.z];
+    const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+    if (m0 >= jb.M || n0 >= jb.N) return;
+    __shared__ double2 As[GT][GK + 1];
+    __shared__ double2 Bs[GK][GT + 1];
+    const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+    double2 acc[2][2] = {{cz(), cz()}, {cz(), cz()}};
+    for (int k0 = 0; k0 < jb.K; k0 += GK) {
+        if (jb.opA == 0) {
+            for (int e = tid; e < GT * GK; e += 256) {
+                const int mm = e / GK, kk = e % GK, m = m0 + mm, k = k0 + kk;
+                As[mm][kk] = (m < jb.M && k < jb.K) ? jb.A[(long)m * jb.lda + k] : cz();
+            }
+        } else {
+            for (int e = tid; e < GT * GK; e += 256) {
+                const int kk = e / GT, mm = e % GT, m = m0 + mm, k = k0 + kk;
+                As[mm][kk] = (m < jb.M && k < jb.K) ? cconj(jb.A[(long)k * jb.lda + m]) : cz();
+            }
+        }
+        if (jb.opB == 0) {
+            for (int e = tid; e < GK * GT; e += 256) {
+                const int kk = e / GT, nn = e % GT, n = n0 + nn, k = k0 + kk;
+                Bs[kk][nn] = (n < jb.N && k < jb.K) ? jb.B[(long)k * jb.ldb + n] : cz();
+            }
+        } else {
+            for (int e = tid; e < GK * GT; e += 256) {
+                const int nn = e / GK, kk = e % GK, n = n0 + nn, k = k0 + kk;
+                Bs[kk][nn] = (n < jb.N && k < jb.K) ? cconj(jb.B[(long)n * jb.ldb + k]) : cz();
+            }
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < GK; ++kk) {
+            const double2 a0 = As[ty][kk], a1 = As[ty + 16][kk];
+            const double2 b0 = Bs[kk][tx], b1 = Bs[kk][tx + 16];
+            cmac(acc[0][0], a0, b0);
+            cmac(acc[0][1], a0, b1);
+            cmac(acc[1][0], a1, b0);
+            cmac(acc[1][1], a1, b1);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int m = m0 + ty + 16 * i, n = n0 + tx + 16 * j;
+            if (m >= jb.M || n >= jb.N) continue;
+            double2* c = jb.C + (long)m * jb.ldc + n;
+            double2 v = cmul(jb.alpha, acc[i][j]);
+            if (jb.beta.x != 0.0 || jb.beta.y != 0.0) cmac(v, jb.beta, *c);
+            *c = v;
+        }
+}
+
+// ------------------------------------------------------------------ factors
+__global__ __launch_bounds__(256) void k_factor(const FactorJob* __restrict__ jobs) {
+    const FactorJob jb = jobs[blockIdx.y];
+    const bool left = blockIdx.z == 0;
+    // mode 2 writes W (ell x chi) into `left`; its row count is the Jacobi dimension.
+    const long nrows = left ? (jb.mode == 2 ? (long)jb.ldj : (long)jb.rows) : (long)jb.chi;
+    const long ncols = left ? (long)jb.chi : (long)jb.cols;
+    const long total = nrows * ncols;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+        const long i = e / ncols, j = e - i * ncols;
+        double2 v = cz();
+        if (left) {
+            const int p = jb.perm[j];
+            const double s = jb.sigma[p];
+            if (s > 0.0) {
+                if (jb.mode == 1)
+                    v = cscale(cconj(jb.X[(long)p * jb.ldx + i]), 1.0 / sqrt(s));
+                else
+                    v = cscale(cconj(jb.J[(long)p * jb.ldj + i]), sqrt(s));
+            }
+            jb.left[e] = v;
+        } else {
+            const int p = jb.perm[i];
+            const double s = jb.sigma[p];
+            if (s > 0.0) {
+                if (jb.mode == 1)
+                    v = cscale(jb.J[(long)p * jb.ldj + j], sqrt(s));
+                else
+                    v = cscale(jb.X[(long)p * jb.ldx + j], 1.0 / sqrt(s));
+            }
+            jb.right[e] = v;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ Householder QR
+// One workgroup per job; see HouseJob. Reflector convention v = x + (x0/|x0|) ||x|| e1,
+// normalized (the reflection axis nalgebra computes), H = I - 2 v v^H.
+__global__ __launch_bounds__(256) void k_householder(const HouseJob* __restrict__ jobs) {
+    const HouseJob jb = jobs[blockIdx.x];
+    const int rows = jb.rows, r = jb.r;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ double red[4];
+    __shared__ double s_cn[kMaxHouseCols];
+    __shared__ double s_tau[kMaxHouseCols];
+    __shared__ double2 s_w[kMaxHouseCols];
+    __shared__ double2 s_x0;
+    __shared__ double s_alpha, s_vn;
+    __shared__ int s_skip;
+    double2* Y = jb.Y;
+    double2* V = jb.V;
+    // Original column norms.
+    for (int k = wave; k < r; k += 4) {
+        double acc = 0.0;
+        for (int i = lane; i < rows; i += 64) {
+            const double2 v = Y[(long)i * r + k];
+            acc = fma(v.x, v.x, fma(v.y, v.y, acc));
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) s_cn[k] = sqrt(acc);
+    }
+    for (long e = tid; e < (long)rows * r; e += 256) V[e] = cz();
+    __syncthreads();
+    for (int k = 0; k < r; ++k) {
+        const int j = jb.pos[k];
+        double acc = 0.0;
+        for (int i = j + tid; i < rows; i += 256) {
+            const double2 v = Y[(long)i * r + k];
+            acc = fma(v.x, v.x, fma(v.y, v.y, acc));
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) red[wave] = acc;
+        __syncthreads();
+        if (tid == 0) {
+            const double a2 = ((red[0] + red[1]) + red[2]) + red[3];
+            const double alpha = sqrt(a2);
+            s_skip = !(alpha > 0.0) || alpha <= jb.tau * s_cn[k] || j >= rows;
+            if (!s_skip) {
+                const double2 x0 = Y[(long)j * r + k];
+                const double x0n = hypot(x0.x, x0.y);
+                const double2 ph = x0n > 0.0 ? make_double2(x0.x / x0n, x0.y / x0n) : make_double2(1.0, 0.0);
+                const double2 nx0 = make_double2(x0.x + ph.x * alpha, x0.y + ph.y * alpha);
+                // ||v||^2 = ||x||^2 - |x0|^2 + |x0 + ph alpha|^2
+                const double vn2 = a2 - x0n * x0n + (nx0.x * nx0.x + nx0.y * nx0.y);
+                s_x0 = nx0;
+                s_vn = sqrt(vn2);
+                s_alpha = alpha;
+            }
+            s_tau[k] = s_skip ? 0.0 : 2.0;
+        }
+        __syncthreads();
+        if (s_skip) continue;
+        const double inv = 1.0 / s_vn;
+        for (int i = j + tid; i < rows; i += 256) {
+            const double2 x = (i == j) ? s_x0 : Y[(long)i * r + k];
+            V[(long)i * r + k] = cscale(x, inv);
+        }
+        __syncthreads();
+        // Trailing columns: Y[:, c] -= 2 v (v^H Y[:, c]), one wave per column.
+        for (int c = k + 1 + wave; c < r; c += 4) {
+            double2 d = cz();
+            for (int i = j + lane; i < rows; i += 64) ccmac(d, V[(long)i * r + k], Y[(long)i * r + c]);
+            d.x = wave_sum(d.x);
+            d.y = wave_sum(d.y);
+            const double2 f = cscale(d, 2.0);
+            for (int i = j + lane; i < rows; i += 64) {
+                double2 y = Y[(long)i * r + c];
+                const double2 v = V[(long)i * r + k];
+                y = csub(y, cmul(v, f));
+                Y[(long)i * r + c] = y;
+            }
+        }
+        __syncthreads();
+    }
+    // T (upper triangular): T[k][k] = tau_k, T[0:k, k] = -tau_k T[0:k, 0:k] (V[:,0:k]^H v_k).
+    double2* T = jb.T;
+    for (long e = tid; e < (long)r * r; e += 256) T[e] = cz();
+    __syncthreads();
+    for (int k = 0; k < r; ++k) {
+        if (s_tau[k] == 0.0) continue;
+        for (int i = wave; i < k; i += 4) {
+            double2 d = cz();
+            for (int row = lane; row < rows; row += 64)
+                ccmac(d, V[(long)row * r + i], V[(long)row * r + k]);
+            d.x = wave_sum(d.x);
+            d.y = wave_sum(d.y);
+            if (lane == 0) s_w[i] = d;
+        }
+        __syncthreads();
+        if (tid < k) {
+            double2 acc = cz();
+            for (int l = tid; l < k; ++l) cmac(acc, T[(long)tid * r + l], s_w[l]);
+            T[(long)tid * r + k] = cscale(acc, -s_tau[k]);
+        }
+        if (tid == 0) T[(long)k * r + k] = make_double2(s_tau[k], 0.0);
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ vector x site
+__global__ __launch_bounds__(256) void k_vecmat(const VecMatJob* __restrict__ jobp, int out_len) {
+    const VecMatJob jb = *jobp;
+    const int o = blockIdx.x * 256 + threadIdx.x;
+    if (o >= out_len) return;
+    double2 acc = cz();
+    const long rs = 2L * jb.r;
+    if (!jb.right) {
+        const int lim = jb.l < jb.vlen ? jb.l : jb.vlen;
+        for (int a = 0; a < lim; ++a) {
+            const double2* s = jb.site + a * rs + o;
+            double2 w = jb.phys == 1 ? s[jb.r] : s[0];
+            if (jb.phys == 2) w = cadd(w, s[jb.r]);
+            cmac(acc, jb.vin[a], w);
+        }
+    } else {
+        const int lim = jb.r < jb.vlen ? jb.r : jb.vlen;
+        const double2* s = jb.site + (long)o * rs;
+        for (int b = 0; b < lim; ++b) {
+            double2 w = jb.phys == 1 ? s[jb.r + b] : s[b];
+            if (jb.phys == 2) w = cadd(w, s[jb.r + b]);
+            cmac(acc, w, jb.vin[b]);
+        }
+    }
+    jb.vout[o] = acc;
+}
+
+// dst (cols x rows) = src (rows x cols)^H
+__global__ __launch_bounds__(256) void k_conj_transpose(const double2* __restrict__ src,
+                                                        double2* __restrict__ dst, int rows,
+                                                        int cols) {
+    __shared__ double2 tile[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int i = ty; i < 32; i += 8) {
+        const int r = r0 + i, c = c0 + tx;
+        tile[i][tx] = (r < rows && c < cols) ? src[(long)r * cols + c] : cz();
+    }
+    __syncthreads();
+    for (int i = ty; i < 32; i += 8) {
+        const int c = c0 + i, r = r0 + tx;
+        if (c < cols && r < rows) dst[(long)c * rows + r] = cconj(tile[tx][i]);
+    }
+}
+
+__global__ void k_fill_int(int* p, int value, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = value;
+}
+
+__global__ void k_add_identity(double2* C, int rows, int cols, int ldc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < rows && i < cols) C[(long)i * ldc + i].x += 1.0;
+}
+
+// next[b*(2p) + ph*p + bp] = sum_a st[a*p + bp] * site[a, ph, b]
+__global__ __launch_bounds__(256) void k_statevector_step(const double2* __restrict__ st,
+                                                          const double2* __restrict__ site,
+                                                          double2* __restrict__ next, int l,
+                                                          int r, long prefix) {
+    const long total = (long)r * 2 * prefix;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+        const long b = e / (2 * prefix), rem = e - b * 2 * prefix;
+        const long ph = rem / prefix, bp = rem - ph * prefix;
+        double2 acc = cz();
+        for (int a = 0; a < l; ++a) cmac(acc, st[a * prefix + bp], site[(long)a * 2 * r + ph * r + b]);
+        next[e] = acc;
+    }
+}
+
+inline int blocks_for(long n, int per = 256, int cap = 4096) {
+    long b = (n + per - 1) / per;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (int)b;
+}
+
+}  // namespace
+
+void launch_one_qubit(const OneQubitJob* jobs, int njobs, int max_elems, hipStream_t s) {
+    dim3 grid(blocks_for(max_elems, 256, 1024), njobs);
+    hipLaunchKernelGGL(k_one_qubit, grid, dim3(256), 0, s, jobs);
+}
+
+void launch_merge(const MergeJob* jobs, int njobs, int max_dl, int max_dr, hipStream_t s) {
+    dim3 grid((max_dr + MT - 1) / MT, (max_dl + MT - 1) / MT, njobs);
+    hipLaunchKernelGGL(k_merge, grid, dim3(256), 0, s, jobs);
+}
+
+void launch_sumsq(const double2* const* ptrs, const long* counts, double* partial, int njobs,
+                  int nblk, hipStream_t s) {
+    hipLaunchKernelGGL(k_sumsq, dim3(nblk, njobs), dim3(256), 0, s, ptrs, counts, partial, nblk);
+}
+
+void launch_sumsq_final(const double* partial, double* out, int njobs, int nblk, hipStream_t s) {
+    hipLaunchKernelGGL(k_sumsq_final, dim3((njobs + 63) / 64), dim3(64), 0, s, partial, out,
+                       njobs, nblk);
+}
+
+void launch_dust(double2* const* ptrs, const long* counts, const double* fro2, double rel,
+                 int njobs, long max_count, hipStream_t s) {
+    hipLaunchKernelGGL(k_dust, dim3(blocks_for(max_count, 256, 2048), njobs), dim3(256), 0, s,
+                       ptrs, counts, fro2, rel);
+}
+
+void launch_identity(double2* const* ptrs, const int* dims, int njobs, int max_dim,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_identity, dim3(blocks_for((long)max_dim * max_dim, 256, 2048), njobs),
+                       dim3(256), 0, s, ptrs, dims);
+}
+
+void launch_jacobi_round(const JacobiJob* jobs, int njobs, int max_pairs, int round, double tol,
+                         int* rotated, const int* done, hipStream_t s) {
+    hipLaunchKernelGGL(k_jacobi_round, dim3(max_pairs, njobs), dim3(256), 0, s, jobs, round, tol,
+                       rotated, done);
+}
+
+void launch_row_norms(const JacobiJob* jobs, int njobs, int max_n, double* sigma,
+                      const long* sigma_off, hipStream_t s) {
+    hipLaunchKernelGGL(k_row_norms, dim3((max_n + 3) / 4, njobs), dim3(256), 0, s, jobs, sigma,
+                       sigma_off);
+}
+
+void launch_gemm(const GemmJob* jobs, int njobs, int max_m, int max_n, hipStream_t s) {
+    dim3 grid((max_n + GT - 1) / GT, (max_m + GT - 1) / GT, njobs);
+    hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, s, jobs);
+}
+
+void launch_factor(const FactorJob* jobs, int njobs, long max_left, long max_right,
+                   hipStream_t s) {
+    long mx = max_left > max_right ? max_left : max_right;
+    hipLaunchKernelGGL(k_factor, dim3(blocks_for(mx, 256, 2048), njobs, 2), dim3(256), 0, s, jobs);
+}
+
+void launch_householder(const HouseJob* jobs, int njobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_householder, dim3(njobs), dim3(256), 0, s, jobs);
+}
+
+void launch_vecmat(const VecMatJob* job, int out_len, hipStream_t s) {
+    hipLaunchKernelGGL(k_vecmat, dim3((out_len + 255) / 256), dim3(256), 0, s, job, out_len);
+}
+
+void launch_conj_transpose(const double2* src, double2* dst, int rows, int cols, hipStream_t s) {
+    dim3 grid((cols + 31) / 32, (rows + 31) / 32);
+    hipLaunchKernelGGL(k_conj_transpose, grid, dim3(256), 0, s, src, dst, rows, cols);
+}
+
+void launch_fill_int(int* p, int value, int count, hipStream_t s) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_fill_int, dim3((count + 255) / 256), dim3(256), 0, s, p, value, count);
+}
+
+void launch_add_identity(double2* C, int rows, int cols, int ldc, hipStream_t s) {
+    int n = rows < cols ? rows : cols;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_add_identity, dim3((n + 255) / 256), dim3(256), 0, s, C, rows, cols, ldc);
+}
+
+void launch_statevector_step(const double2* st, const double2* site, double2* next, int l,
+                             int r, long prefix, hipStream_t s) {
+    long total = (long)r * 2 * prefix;
+    hipLaunchKernelGGL(k_statevector_step, dim3(blocks_for(total, 256, 8192)), dim3(256), 0, s,
+                       st, site, next, l, r, prefix);
+}
+
+}  // namespace wqc

@@ -1,0 +1,276 @@
+"""Python binding of libwqc_tn.so, the MI355X MPS engine, through its C ABI (include/wqc_tn.h).
+
+``MpsState`` mirrors the reference's ``MpsState`` (src/tn/mps.rs:55-781): same method names,
+argument meaning and error behaviour (a non-OK status raises ``WqcTnError`` carrying the
+status code and the engine's message). The engine runs on the GPU only: there is no CPU
+fallback, and loading fails loudly when the library or the device is missing.
+"""
+
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwqc_tn.so")
+
+OK, INVALID_ARG, OUT_OF_MEMORY, EXEC_FAILED, CANCELLED, BOND_EXCEEDED, NO_DEVICE = range(7)
+PRECISION_FP64, PRECISION_MIXED_FP32 = 0, 1
+
+EXPORTED = (
+    "wqc_tn_last_error", "wqc_tn_backend_from_env", "wqc_tn_engine_status", "wqc_tn_device_open",
+    "wqc_tn_device_close", "wqc_tn_state_new", "wqc_tn_state_free", "wqc_tn_qubit_count",
+    "wqc_tn_set_max_bond_dim", "wqc_tn_set_reject_truncation", "wqc_tn_apply_gate",
+    "wqc_tn_apply_circuit", "wqc_tn_apply_adjacent_disjoint_gates", "wqc_tn_apply_unary_at",
+    "wqc_tn_apply_tdg_at", "wqc_tn_apply_swap_at", "wqc_tn_apply_pair_at", "wqc_tn_apply_cnot_at",
+    "wqc_tn_request_cancel", "wqc_tn_clear_cancel", "wqc_tn_amplitude_at_compact_zero",
+    "wqc_tn_site_amplitudes", "wqc_tn_control_probability", "wqc_tn_contract_to_statevector",
+    "wqc_tn_bond_dims", "wqc_tn_download_site", "wqc_tn_upload_site", "wqc_tn_backend_label",
+    "wqc_tn_peak_vram_bytes", "wqc_tn_get_stats", "wqc_tn_kernel_times", "wqc_tn_enable_timing",
+)
+
+
+class Gate(ct.Structure):
+    _fields_ = [("kind", ct.c_int32), ("q0", ct.c_int32), ("q1", ct.c_int32), ("q2", ct.c_int32),
+                ("angle", ct.c_double)]
+
+
+class Stats(ct.Structure):
+    _fields_ = [("two_qubit_units", ct.c_uint64), ("one_qubit_gates", ct.c_uint64),
+                ("exact_splits", ct.c_uint64), ("rsvd_kept", ct.c_uint64),
+                ("rsvd_fallbacks", ct.c_uint64), ("svd_check_failures", ct.c_uint64),
+                ("jacobi_sweeps", ct.c_uint64), ("truncated_splits", ct.c_uint64),
+                ("peak_vram_bytes", ct.c_uint64), ("max_discarded_weight", ct.c_double),
+                ("sum_discarded_weight", ct.c_double)]
+
+
+class WqcTnError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"wqc_tn status {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load libwqc_tn.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it first (__graft_entry__.build())")
+    L = ct.CDLL(LIB_PATH)
+    vp, sz, dp, i32 = ct.c_void_p, ct.c_size_t, ct.POINTER(ct.c_double), ct.c_int
+    gp = ct.POINTER(Gate)
+    L.wqc_tn_last_error.restype = ct.c_char_p
+    L.wqc_tn_backend_from_env.restype = ct.c_char_p
+    L.wqc_tn_engine_status.argtypes = [ct.c_char_p, sz, ct.POINTER(ct.c_uint64)]
+    L.wqc_tn_device_open.argtypes = [i32, ct.POINTER(vp)]
+    L.wqc_tn_device_close.argtypes = [vp]
+    L.wqc_tn_state_new.argtypes = [vp, sz, sz, i32, ct.POINTER(vp)]
+    L.wqc_tn_state_free.argtypes = [vp]
+    L.wqc_tn_qubit_count.restype = sz
+    L.wqc_tn_qubit_count.argtypes = [vp]
+    L.wqc_tn_set_max_bond_dim.argtypes = [vp, sz]
+    L.wqc_tn_set_reject_truncation.argtypes = [vp, i32]
+    L.wqc_tn_apply_gate.argtypes = [vp, gp]
+    L.wqc_tn_apply_circuit.argtypes = [vp, gp, sz]
+    L.wqc_tn_apply_adjacent_disjoint_gates.argtypes = [vp, gp, sz]
+    L.wqc_tn_apply_unary_at.argtypes = [vp, sz, gp]
+    L.wqc_tn_apply_tdg_at.argtypes = [vp, sz]
+    L.wqc_tn_apply_swap_at.argtypes = [vp, sz]
+    L.wqc_tn_apply_pair_at.argtypes = [vp, sz, gp, i32]
+    L.wqc_tn_apply_cnot_at.argtypes = [vp, sz, i32]
+    L.wqc_tn_request_cancel.argtypes = [vp]
+    L.wqc_tn_clear_cancel.argtypes = [vp]
+    L.wqc_tn_amplitude_at_compact_zero.argtypes = [vp, dp]
+    L.wqc_tn_site_amplitudes.argtypes = [vp, sz, dp]
+    L.wqc_tn_control_probability.argtypes = [vp, sz, dp]
+    L.wqc_tn_contract_to_statevector.argtypes = [vp, dp, sz]
+    L.wqc_tn_bond_dims.argtypes = [vp, ct.POINTER(sz), sz]
+    L.wqc_tn_download_site.argtypes = [vp, sz, dp, sz, ct.POINTER(sz), ct.POINTER(sz)]
+    L.wqc_tn_upload_site.argtypes = [vp, sz, dp, sz, sz]
+    L.wqc_tn_backend_label.restype = ct.c_char_p
+    L.wqc_tn_backend_label.argtypes = [vp]
+    L.wqc_tn_peak_vram_bytes.restype = ct.c_uint64
+    L.wqc_tn_peak_vram_bytes.argtypes = [vp]
+    L.wqc_tn_get_stats.argtypes = [vp, ct.POINTER(Stats)]
+    L.wqc_tn_kernel_times.argtypes = [vp, dp, sz]
+    L.wqc_tn_enable_timing.argtypes = [vp, i32]
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != OK:
+        raise WqcTnError(rc, load().wqc_tn_last_error().decode())
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ct.POINTER(ct.c_double))
+
+
+def gate_array(gates):
+    arr = (Gate * max(1, len(gates)))()
+    for i, g in enumerate(gates):
+        arr[i].kind, arr[i].q0, arr[i].q1, arr[i].q2, arr[i].angle = (
+            int(g[0]), int(g[1]), int(g[2]), int(g[3]), float(g[4]))
+    return arr
+
+
+class Device:
+    """An opened gfx950 device (GpuMpsDevice::try_new, src/tn/gpu/device.rs:68)."""
+
+    def __init__(self, ordinal: int = 0):
+        L = load()
+        h = ct.c_void_p()
+        _check(L.wqc_tn_device_open(ordinal, ct.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().wqc_tn_device_close(self._h)
+            self._h = None
+
+
+_default_device = None
+
+
+def default_device() -> Device:
+    global _default_device
+    if _default_device is None:
+        _default_device = Device(0)
+    return _default_device
+
+
+class MpsState:
+    """Device-resident MPS (src/tn/mps.rs:55-781)."""
+
+    def __init__(self, qubit_count: int, max_bond_dim: int, precision: int = PRECISION_FP64,
+                 device: Device | None = None):
+        L = load()
+        dev = device or default_device()
+        h = ct.c_void_p()
+        _check(L.wqc_tn_state_new(dev._h, qubit_count, max_bond_dim, precision, ct.byref(h)))
+        self._h = h
+        self._dev = dev
+        self.qubit_count = qubit_count
+        self.max_bond_dim = max_bond_dim
+
+    try_new_with_bond = classmethod(lambda cls, n, chi, **kw: cls(n, chi, **kw))
+
+    def free(self):
+        if getattr(self, "_h", None):
+            load().wqc_tn_state_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def set_max_bond_dim(self, chi: int):
+        _check(load().wqc_tn_set_max_bond_dim(self._h, chi))
+        self.max_bond_dim = chi
+
+    def set_reject_truncation(self, on: bool):
+        _check(load().wqc_tn_set_reject_truncation(self._h, 1 if on else 0))
+
+    def apply_gate(self, g):
+        _check(load().wqc_tn_apply_gate(self._h, gate_array([g])))
+
+    def apply_gates(self, gates):
+        for g in gates:
+            self.apply_gate(g)
+
+    def apply_circuit(self, gates):
+        _check(load().wqc_tn_apply_circuit(self._h, gate_array(gates), len(gates)))
+
+    def apply_adjacent_disjoint_gates(self, gates):
+        _check(load().wqc_tn_apply_adjacent_disjoint_gates(self._h, gate_array(gates), len(gates)))
+
+    def apply_unary_at(self, site, g):
+        _check(load().wqc_tn_apply_unary_at(self._h, site, gate_array([g])))
+
+    def apply_tdg_at(self, site):
+        _check(load().wqc_tn_apply_tdg_at(self._h, site))
+
+    def apply_swap_at(self, site):
+        _check(load().wqc_tn_apply_swap_at(self._h, site))
+
+    def apply_pair_at(self, site, g, ctrl_on_left=True):
+        _check(load().wqc_tn_apply_pair_at(self._h, site, gate_array([g]), 1 if ctrl_on_left else 0))
+
+    def apply_cnot_at(self, site, ctrl_on_left=True):
+        _check(load().wqc_tn_apply_cnot_at(self._h, site, 1 if ctrl_on_left else 0))
+
+    def request_cancel(self):
+        load().wqc_tn_request_cancel(self._h)
+
+    def clear_cancel(self):
+        load().wqc_tn_clear_cancel(self._h)
+
+    def amplitude_at_compact_zero(self) -> complex:
+        out = np.zeros(2)
+        _check(load().wqc_tn_amplitude_at_compact_zero(self._h, _dp(out)))
+        return complex(out[0], out[1])
+
+    def site_amplitudes(self, target: int):
+        out = np.zeros(4)
+        _check(load().wqc_tn_site_amplitudes(self._h, target, _dp(out)))
+        return complex(out[0], out[1]), complex(out[2], out[3])
+
+    def control_probability(self, control: int) -> float:
+        out = np.zeros(1)
+        _check(load().wqc_tn_control_probability(self._h, control, _dp(out)))
+        return float(out[0])
+
+    def contract_to_statevector(self):
+        n = self.qubit_count
+        out = np.zeros(2 ** (n + 1)) if n <= 20 else np.zeros(2)
+        _check(load().wqc_tn_contract_to_statevector(self._h, _dp(out), out.size // 2))
+        return out.view(np.complex128)
+
+    def bond_dims(self):
+        out = (ct.c_size_t * self.qubit_count)()
+        _check(load().wqc_tn_bond_dims(self._h, out, self.qubit_count))
+        return [int(x) for x in out]
+
+    def site(self, i: int) -> np.ndarray:
+        l, r = ct.c_size_t(), ct.c_size_t()
+        _check(load().wqc_tn_download_site(self._h, i, None, 0, ct.byref(l), ct.byref(r)))
+        out = np.zeros(l.value * 2 * r.value * 2)
+        _check(load().wqc_tn_download_site(self._h, i, _dp(out), out.size // 2, ct.byref(l), ct.byref(r)))
+        return out.view(np.complex128).reshape(l.value, 2, r.value)
+
+    def set_site(self, i: int, arr):
+        a = np.ascontiguousarray(arr, dtype=np.complex128)
+        _check(load().wqc_tn_upload_site(self._h, i, _dp(a.view(np.float64)), a.shape[0], a.shape[2]))
+
+    def backend_label(self) -> str:
+        return load().wqc_tn_backend_label(self._h).decode()
+
+    def peak_vram_bytes(self) -> int:
+        return int(load().wqc_tn_peak_vram_bytes(self._h))
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(load().wqc_tn_get_stats(self._h, ct.byref(s)))
+        return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+    def enable_timing(self, on=True):
+        _check(load().wqc_tn_enable_timing(self._h, 1 if on else 0))
+
+    def kernel_times(self):
+        out = np.zeros(6)
+        k = load().wqc_tn_kernel_times(self._h, _dp(out), 6)
+        names = ("one_qubit", "merge", "range_finder", "jacobi", "factors", "readout")
+        return dict(zip(names[:k], out[:k].tolist()))
+
+
+def backend_from_env() -> str:
+    return load().wqc_tn_backend_from_env().decode()
