@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Benchmark: two-qubit gate-merges per second at chi = 512 on one MI355X.
+
+Workload: 100-qubit brickwork, depth 30, RX/RY/RZ (uniform axis, angle U[0, 2pi)) on every
+qubit then CZ on alternating even/odd pairs, chi = 512 (1485 two-qubit units). One step is
+one pass of the whole circuit from |0...0> through the C ABI (wqc_tn_apply_circuit) plus
+the final <0...0|psi>; inputs (the gate list) are on the host, every site tensor lives in
+HBM for the whole step.
+
+Printed JSON line: value = merges/s of the headline form; the other form, the relative
+amplitude error between the forms, a roofline record for the dominant kernel and a CPU
+baseline ride along. `--gpus N` with N > 1 exits: this build runs on one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "wqc-core_amd")
+for p in (ROOT, PKG):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import circuits as C  # noqa: E402
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (matrix and vector)
+HBM_PEAK_GBS = 8000.0
+
+
+def unit_flops(rows, cols, bond, path, ell, chi):
+    """Algorithmic flops of one two-qubit unit (complex MAC = 8 flops)."""
+    dl, dr = rows // 2, cols // 2
+    f = 8.0 * rows * bond * cols + 8.0 * 16 * dl * dr          # merge + U
+    if path == 1:                                              # range finder kept
+        m, n = max(ell, cols), min(ell, cols)
+        f += 8.0 * (4.0 * rows * cols * ell + rows * ell * chi)
+        f += 4.0 * (2.0 * rows * ell * ell - (2.0 / 3.0) * ell ** 3)
+        f += 4.0 * (4.0 * m * m * n + 8.0 * m * n * n + 9.0 * n ** 3)
+    else:                                                      # exact split (GVL R-SVD)
+        m, n = max(rows, cols), min(rows, cols)
+        f += 4.0 * (4.0 * m * m * n + 8.0 * m * n * n + 9.0 * n ** 3)
+    return f
+
+
+def svd_flops(rows, cols, path, ell):
+    if path == 1:
+        m, n = max(ell, cols), min(ell, cols)
+    else:
+        m, n = max(rows, cols), min(rows, cols)
+    return 4.0 * (4.0 * m * m * n + 8.0 * m * n * n + 9.0 * n ** 3)
+
+
+def run_gpu(wt, n, chi, gates, precision, steps, warmup, timed_pass=True):
+    amps = []
+    for _ in range(warmup):
+        st = wt.MpsState(n, chi, precision)
+        st.apply_circuit(gates)
+        amps.append(st.amplitude_at_compact_zero())
+        st.free()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = wt.MpsState(n, chi, precision)
+        st.apply_circuit(gates)
+        amps.append(st.amplitude_at_compact_zero())
+        stats = st.stats()
+        st.free()
+    dt = time.perf_counter() - t0
+    info = {"seconds": dt, "amplitude": amps[-1], "stats": stats}
+    if timed_pass:
+        # A further pass with HIP-event timing of every kernel family and the split log.
+        st = wt.MpsState(n, chi, precision)
+        st.enable_timing(True)
+        st.enable_split_log(True)
+        t1 = time.perf_counter()
+        st.apply_circuit(gates)
+        amp = st.amplitude_at_compact_zero()
+        info["timed_pass_seconds"] = time.perf_counter() - t1
+        info["kernel_ms"] = st.kernel_times()
+        info["launches"] = st.kernel_launches()
+        info["split_log"] = st.split_log()
+        info["timed_amplitude"] = amp
+        st.free()
+    return info
+
+
+def cpu_baseline(sample_units: int = 1):
+    """The FP64 oracle on the host cores: saturated chi=512 units (bonds 512 on both sides,
+    theta 1024 x 1024, range finder with ell = 528) on synthetic random sites."""
+    import numpy as np
+
+    from oracle import oracle_mps as om
+
+    om.set_threads(1)
+    rng = np.random.default_rng(2024)
+    b = 512
+    sites = []
+    for l, r in [(1, b), (b, b), (b, b), (b, 1)]:
+        t = rng.standard_normal((l, 2, r)) + 1j * rng.standard_normal((l, 2, r))
+        t *= np.exp(-0.004 * np.arange(r))[None, None, :]
+        t /= np.linalg.norm(t) / math.sqrt(r)
+        sites.append(t)
+    t0 = time.perf_counter()
+    for _ in range(sample_units):
+        o = om.OracleMps(4, 512)
+        for i, s in enumerate(sites):
+            o.set_site(i, s)
+        o.apply_gate(C.two(C.CZ, 1, 2))
+    dt = time.perf_counter() - t0
+    return {"value": sample_units / dt, "unit": "merges/s", "cores": 1, "kind": "port",
+            "sample": f"{sample_units} saturated chi=512 unit(s): theta 1024x1024, range finder "
+                      f"ell=528, synthetic sites; FP64 oracle, 1 thread ({platform.processor() or 'x86_64'})",
+            "seconds": dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--depth", type=int, default=30)
+    ap.add_argument("--qubits", type=int, default=100)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-units", type=int, default=1)
+    ap.add_argument("--verbose", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding design is documented, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import wqc_tn as wt
+
+    wt.load()
+    n, chi, gates = C.config3(n=args.qubits, depth=args.depth)
+    units = C.two_qubit_units(gates, n)
+    print(f"# workload: {n}q depth {args.depth} chi {chi}: {units} two-qubit units", file=sys.stderr,
+          flush=True)
+    fp64 = run_gpu(wt, n, chi, gates, wt.PRECISION_FP64, args.steps, args.warmup)
+    ms_step = 1e3 * fp64["seconds"] / args.steps
+    value = units * args.steps / fp64["seconds"]
+    log = fp64["split_log"]
+    alg = sum(unit_flops(*e) for e in log)
+    svd = sum(svd_flops(e[0], e[1], e[3], e[4]) for e in log)
+    kms = fp64["kernel_ms"]
+    launches = fp64["launches"]
+    jac_ms = kms.get("jacobi", 0.0)
+    dom = max(kms, key=lambda k: kms[k]) if kms else "jacobi"
+    achieved = svd / (jac_ms * 1e-3) / 1e12 if jac_ms > 0 else None
+    roofline = {
+        "bound": "mfma", "kernel": "k_jacobi_round", "achieved": achieved,
+        "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None, "traffic": None,
+        "launches": launches.get("jacobi"),
+        "avg_launch_us": 1e3 * jac_ms / max(1, launches.get("jacobi", 1)),
+        "dominant_family": dom,
+    }
+    out = {
+        "metric": "2q gate-merges/s at chi=512 (mixed vs FP64 plain) + rel. amp error vs FP64",
+        "value": value, "unit": "merges/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"rotation+CZ brickwork {n}q depth {args.depth} chi={chi}",
+                   "qubits": n, "depth": args.depth, "chi": chi, "units": units,
+                   "form": "plain FP64"},
+        "fp64": {"merges_per_s": value, "amplitude": [fp64["amplitude"].real, fp64["amplitude"].imag],
+                 "alg_tflops_whole_step": alg / fp64["timed_pass_seconds"] / 1e12,
+                 "kernel_ms": kms, "launches": launches, "stats": {k: v for k, v in fp64["stats"].items()}},
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_units)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

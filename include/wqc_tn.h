@@ -208,6 +208,15 @@ int wqc_tn_get_stats(const wqc_tn_state *st, wqc_tn_stats *out);
  * 4 = factors, 5 = readouts. Returns the number of entries written. */
 int wqc_tn_kernel_times(const wqc_tn_state *st, double *ms, size_t len);
 int wqc_tn_enable_timing(wqc_tn_state *st, int enable);
+/* Launch counts per kernel family (same indexing as wqc_tn_kernel_times). */
+int wqc_tn_kernel_launches(const wqc_tn_state *st, uint64_t *count, size_t len);
+
+/* Split log for measurement: when enabled, every two-qubit unit appends six int32
+ * values {rows, cols, bond, path, ell, chi_kept} (path 0 exact, 1 range finder kept,
+ * 2 range finder fell back to exact). wqc_tn_split_log copies up to cap entries and
+ * returns the number recorded. */
+int wqc_tn_enable_split_log(wqc_tn_state *st, int enable);
+size_t wqc_tn_split_log(const wqc_tn_state *st, int32_t *out, size_t cap_entries);
 
 #ifdef __cplusplus
 }

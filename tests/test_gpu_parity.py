@@ -114,7 +114,7 @@ def test_config2_amplitude(wt):
     assert g.stats()["two_qubit_units"] == o.counters()["units"]
 
 
-@pytest.mark.parametrize("n,depth,chi", [(24, 16, 16), (20, 14, 8), (16, 12, 64)])
+@pytest.mark.parametrize("n,depth,chi", [(24, 16, 16), (20, 14, 8), (18, 16, 12)])
 def test_truncating_brickwork_vs_oracle(wt, n, depth, chi):
     # chi below the circuit's Schmidt ranks: every split truncates through the exact SVD.
     gates = C.brickwork(n, depth, 77 + n)

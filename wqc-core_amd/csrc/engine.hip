@@ -25,6 +25,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -383,6 +384,15 @@ struct wqc_tn_state {
     uint64_t site_bytes = 0;
     bool timing = false;
     double kms[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t klaunch[6] = {0, 0, 0, 0, 0, 0};
+    struct TimedSpan {
+        hipEvent_t a, b;
+        int fam;
+    };
+    std::vector<TimedSpan> spans;
+    std::vector<hipEvent_t> event_pool;
+    bool split_log_on = false;
+    std::vector<int32_t> split_log;
     std::vector<PendingOne> pend1;
     std::vector<Unit> pend2;
     std::vector<char> touched;
@@ -394,6 +404,11 @@ struct wqc_tn_state {
             if (s.ptr) (void)hipFree(s.ptr);
         ws.release();
         tab.release();
+        for (auto& sp : spans) {
+            (void)hipEventDestroy(sp.a);
+            (void)hipEventDestroy(sp.b);
+        }
+        for (auto e : event_pool) (void)hipEventDestroy(e);
         if (htab) (void)hipHostFree(htab);
         if (hread) (void)hipHostFree(hread);
         if (stream) (void)hipStreamDestroy(stream);
@@ -489,48 +504,56 @@ struct wqc_tn_state {
             max_elems = std::max(max_elems, s.l * s.r);
         }
         OneQubitJob* d = upload(jobs);
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (timing) {
-            HIPC(hipEventCreate(&e0));
-            HIPC(hipEventCreate(&e1));
-            HIPC(hipEventRecord(e0, stream));
-        }
-        wqc::launch_one_qubit(d, (int)jobs.size(), max_elems, stream);
-        check_launch();
-        if (timing) {
-            HIPC(hipEventRecord(e1, stream));
-            HIPC(hipEventSynchronize(e1));
-            float ms = 0;
-            HIPC(hipEventElapsedTime(&ms, e0, e1));
-            kms[0] += ms;
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
+        {
+            Timer tm(this, 0);
+            wqc::launch_one_qubit(d, (int)jobs.size(), max_elems, stream);
+            check_launch();
+            klaunch[0]++;
         }
         stats.one_qubit_gates += ops.size();
         sync();
+        resolve_timings();
     }
 
     // ------------------------------------------------------------ timing helper
+    // Brackets a group of launches on the state's stream with two events; the elapsed
+    // time is read at the next synchronization (resolve_timings), so timing adds no
+    // host waits of its own.
+    hipEvent_t take_event() {
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        HIPC(hipEventCreate(&e));
+        return e;
+    }
     struct Timer {
         wqc_tn_state* st;
         int fam;
-        hipEvent_t a = nullptr, b = nullptr;
+        hipEvent_t a = nullptr;
         Timer(wqc_tn_state* s, int f) : st(s), fam(f) {
             if (!st->timing) return;
-            HIPC(hipEventCreate(&a));
-            HIPC(hipEventCreate(&b));
+            a = st->take_event();
             HIPC(hipEventRecord(a, st->stream));
         }
         ~Timer() {
-            if (!st->timing || !a) return;
-            if (hipEventRecord(b, st->stream) == hipSuccess && hipEventSynchronize(b) == hipSuccess) {
-                float ms = 0;
-                if (hipEventElapsedTime(&ms, a, b) == hipSuccess) st->kms[fam] += ms;
-            }
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
+            if (!a) return;
+            hipEvent_t b = st->take_event();
+            if (hipEventRecord(b, st->stream) == hipSuccess) st->spans.push_back({a, b, fam});
         }
     };
+    void resolve_timings() {
+        for (auto& sp : spans) {
+            float ms = 0;
+            if (hipEventSynchronize(sp.b) == hipSuccess && hipEventElapsedTime(&ms, sp.a, sp.b) == hipSuccess)
+                kms[sp.fam] += ms;
+            event_pool.push_back(sp.a);
+            event_pool.push_back(sp.b);
+        }
+        spans.clear();
+    }
 
     // ------------------------------------------------------------ Jacobi driver
     // Rotates the rows of every job's X until no block pair needs a rotation, then
@@ -539,7 +562,6 @@ struct wqc_tn_state {
                 int* d_flags) {
         const int nj = (int)jobs.size();
         if (!nj) return;
-        Timer tm(this, 3);
         JacobiJob* dj = upload(jobs);
         int max_n = 1, max_m = 1;
         for (auto& j : jobs) {
@@ -558,9 +580,13 @@ struct wqc_tn_state {
         for (int sweep = 0; sweep < MAX_SWEEPS && active > 0; ++sweep) {
             check_cancel();
             wqc::launch_fill_int(d_rot, 0, nj, stream);
-            for (int r = 0; r < rounds; ++r)
-                wqc::launch_jacobi_round(dj, nj, pairs, r, tol, d_rot, d_done, stream);
-            check_launch();
+            {
+                Timer tm(this, 3);
+                for (int r = 0; r < rounds; ++r)
+                    wqc::launch_jacobi_round(dj, nj, pairs, r, tol, d_rot, d_done, stream);
+                check_launch();
+                klaunch[3] += rounds;
+            }
             stats.jacobi_sweeps += active;
             HIPC(hipMemcpyAsync(hread, d_rot, nj * sizeof(int), hipMemcpyDeviceToHost, stream));
             sync();
@@ -695,6 +721,7 @@ struct wqc_tn_state {
             }
             wqc::launch_merge(upload(mj), nu, mdl, mdr, stream);
             check_launch();
+            klaunch[1]++;
         }
         stats.two_qubit_units += nu;
 
@@ -705,6 +732,14 @@ struct wqc_tn_state {
             if (J[i].fallback) exact_ids.push_back(i);
         if (!exact_ids.empty()) run_exact(J, exact_ids, base, d_flags, d_fro + nu, d_part + (size_t)nu * nblk, nblk);
         sync();
+        resolve_timings();
+        if (split_log_on)
+            for (int i = 0; i < nu; ++i) {
+                const JobInfo& j = J[i];
+                const int path = !j.rsvd ? 0 : (j.fallback ? 2 : 1);
+                const int32_t e[6] = {j.rows, j.cols, j.bond, path, (int32_t)j.ell, (int32_t)j.chi_kept};
+                split_log.insert(split_log.end(), e, e + 6);
+            }
     }
 
     // Exact split (truncated_svd_split) for the listed jobs.
@@ -779,8 +814,11 @@ struct wqc_tn_state {
             }
             if (!(std::fabs(s2 - fro2) <= SVD_CHECK_TOL * std::max(fro2, 2.2250738585072014e-308) + 1e-28)) {
                 stats.svd_check_failures++;
+                char buf[256];
+                std::snprintf(buf, sizeof buf, " (sum s^2 = %.17g, |A|_F^2 = %.17g, job %d of %d, n=%d m=%d)", s2, fro2, k,
+                              ne, nn, jj[k].m);
                 fail(WQC_TN_EXEC_FAILED, "SVD of a " + std::to_string(j.rows) + "x" + std::to_string(j.cols) +
-                                             " matrix failed its check");
+                                             " matrix failed its check" + buf);
             }
             const size_t c = retained_chi(sorted, chi, j.rows, j.cols);
             bool dropped = false;
@@ -819,6 +857,7 @@ struct wqc_tn_state {
             Timer tm(this, 4);
             wqc::launch_factor(upload(fj), ne, mxl, mxr, stream);
             check_launch();
+            klaunch[4]++;
         }
         for (int k = 0; k < ne; ++k) {
             JobInfo& j = J[ids[k]];
@@ -836,6 +875,7 @@ struct wqc_tn_state {
         }
         wqc::launch_gemm(upload(g), (int)g.size(), mm, mn, stream);
         check_launch();
+        klaunch[2]++;
     }
 
     static GemmJob gj(const double2* A, int lda, int opA, const double2* B, int ldb, int opB, double2* C,
@@ -895,6 +935,7 @@ struct wqc_tn_state {
             }
             wqc::launch_householder(upload(hj), (int)hj.size(), stream);
             check_launch();
+            klaunch[2]++;
             // B = A[0:ell, :] - V[0:ell, :] (T^H (V^H A))
             std::vector<GemmJob> g4, g5, g6;
             for (int i : ids) {
@@ -1012,6 +1053,7 @@ struct wqc_tn_state {
         Timer tm(this, 4);
         wqc::launch_factor(upload(fj), (int)fj.size(), mxl, mxr, stream);
         check_launch();
+        klaunch[4]++;
         // left = [W; 0] - V (T (V[0:ell, :]^H W))
         std::vector<GemmJob> g7, g8, g9;
         for (int i : kept) {
@@ -1235,6 +1277,7 @@ struct wqc_tn_state {
         flush();
         sync();
         tables_reset();
+        klaunch[5] += n;
         Timer tm(this, 5);
         double2 *a, *b, *c, *d;
         readout_buffers(a, b, c, d);
@@ -1246,6 +1289,11 @@ struct wqc_tn_state {
         (void)c;
         (void)d;
         return {out.x, out.y};
+    }
+    std::pair<double, double> amplitude_zero_timed() {
+        auto r = amplitude_zero();
+        resolve_timings();
+        return r;
     }
 
     // site_amplitudes (src/tn/mps.rs:679-693): left/right environments with the all-ones
@@ -1535,7 +1583,7 @@ void wqc_tn_clear_cancel(wqc_tn_state* st) {
 int wqc_tn_amplitude_at_compact_zero(wqc_tn_state* st, double out[2]) {
     return guarded([&] {
         if (!st || !out) fail(WQC_TN_INVALID_ARG, "null argument");
-        auto a = st->amplitude_zero();
+        auto a = st->amplitude_zero_timed();
         out[0] = a.first;
         out[1] = a.second;
     });
@@ -1624,6 +1672,30 @@ int wqc_tn_enable_timing(wqc_tn_state* st, int enable) {
     if (!st) return WQC_TN_INVALID_ARG;
     st->timing = enable != 0;
     return WQC_TN_OK;
+}
+
+int wqc_tn_kernel_launches(const wqc_tn_state* st, uint64_t* count, size_t len) {
+    if (!st || !count) return 0;
+    size_t k = std::min<size_t>(len, 6);
+    for (size_t i = 0; i < k; ++i) count[i] = st->klaunch[i];
+    return (int)k;
+}
+
+int wqc_tn_enable_split_log(wqc_tn_state* st, int enable) {
+    if (!st) return WQC_TN_INVALID_ARG;
+    st->split_log_on = enable != 0;
+    if (!enable) st->split_log.clear();
+    return WQC_TN_OK;
+}
+
+size_t wqc_tn_split_log(const wqc_tn_state* st, int32_t* out, size_t cap_entries) {
+    if (!st) return 0;
+    const size_t m = st->split_log.size() / 6;
+    if (out) {
+        const size_t k = std::min(m, cap_entries);
+        std::memcpy(out, st->split_log.data(), k * 6 * sizeof(int32_t));
+    }
+    return m;
 }
 
 }  // extern "C"

@@ -280,9 +280,9 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
     __syncthreads();
 
     const double tol_in = 1e-15;
+    if (tid == 0) s_any = 0;
+    __syncthreads();
     for (int sweep = 0; sweep < 16; ++sweep) {
-        if (tid == 0) s_any = 0;
-        __syncthreads();
         for (int step = 0; step < JP - 1; ++step) {
             if (tid < 16) {
                 int p, q;
@@ -349,7 +349,12 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
             }
             __syncthreads();
         }
-        if (!s_any) break;
+        // Every wave reads the flag before thread 0 clears it for the next sweep.
+        const int again = s_any;
+        __syncthreads();
+        if (!again) break;
+        if (tid == 0) s_any = 0;
+        __syncthreads();
     }
 
     // P <- T P over the columns of X, then the rows of J.

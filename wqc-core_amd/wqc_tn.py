@@ -29,6 +29,7 @@ EXPORTED = (
     "wqc_tn_site_amplitudes", "wqc_tn_control_probability", "wqc_tn_contract_to_statevector",
     "wqc_tn_bond_dims", "wqc_tn_download_site", "wqc_tn_upload_site", "wqc_tn_backend_label",
     "wqc_tn_peak_vram_bytes", "wqc_tn_get_stats", "wqc_tn_kernel_times", "wqc_tn_enable_timing",
+    "wqc_tn_kernel_launches", "wqc_tn_enable_split_log", "wqc_tn_split_log",
 )
 
 
@@ -100,6 +101,10 @@ def load():
     L.wqc_tn_get_stats.argtypes = [vp, ct.POINTER(Stats)]
     L.wqc_tn_kernel_times.argtypes = [vp, dp, sz]
     L.wqc_tn_enable_timing.argtypes = [vp, i32]
+    L.wqc_tn_kernel_launches.argtypes = [vp, ct.POINTER(ct.c_uint64), sz]
+    L.wqc_tn_enable_split_log.argtypes = [vp, i32]
+    L.wqc_tn_split_log.restype = sz
+    L.wqc_tn_split_log.argtypes = [vp, ct.POINTER(ct.c_int32), sz]
     _lib = L
     return L
 
@@ -265,11 +270,29 @@ class MpsState:
     def enable_timing(self, on=True):
         _check(load().wqc_tn_enable_timing(self._h, 1 if on else 0))
 
+    FAMILIES = ("one_qubit", "merge", "range_finder", "jacobi", "factors", "readout")
+
     def kernel_times(self):
+        """Device milliseconds per kernel family (timing must be enabled)."""
         out = np.zeros(6)
         k = load().wqc_tn_kernel_times(self._h, _dp(out), 6)
-        names = ("one_qubit", "merge", "range_finder", "jacobi", "factors", "readout")
-        return dict(zip(names[:k], out[:k].tolist()))
+        return dict(zip(self.FAMILIES[:k], out[:k].tolist()))
+
+    def kernel_launches(self):
+        out = (ct.c_uint64 * 6)()
+        k = load().wqc_tn_kernel_launches(self._h, out, 6)
+        return dict(zip(self.FAMILIES[:k], [int(x) for x in out[:k]]))
+
+    def enable_split_log(self, on=True):
+        _check(load().wqc_tn_enable_split_log(self._h, 1 if on else 0))
+
+    def split_log(self):
+        """[(rows, cols, bond, path, ell, chi_kept)] per two-qubit unit since enabling."""
+        m = load().wqc_tn_split_log(self._h, None, 0)
+        buf = (ct.c_int32 * max(6, 6 * m))()
+        load().wqc_tn_split_log(self._h, buf, m)
+        a = np.frombuffer(buf, dtype=np.int32)[: 6 * m].reshape(m, 6)
+        return [tuple(int(x) for x in row) for row in a]
 
 
 def backend_from_env() -> str:
