@@ -95,6 +95,8 @@ typedef struct wqc_tn_stats {
     uint64_t peak_vram_bytes;     /* vram_peak_bytes (src/tn/mps.rs:157-166)              */
     double max_discarded_weight;  /* max over splits of sum_{k>=chi} s_k^2 / sum s_k^2     */
     double sum_discarded_weight;
+    uint64_t jacobi_inner_sweeps; /* sweeps of the per-block-pair eigensolves               */
+    uint64_t jacobi_pair_solves;  /* block-pair eigensolves run                             */
 } wqc_tn_stats;
 
 /* ---- errors ------------------------------------------------------------------ */

@@ -416,3 +416,62 @@ def test_pauli_convention_kats():
     assert abs(_expect(2, [C.one(C.H, 0), C.two(C.CNOT, 0, 1)], "YY") + 1.0) < 1e-9
     assert abs(_expect(1, [C.one(C.H, 0)], "Z")) < 1e-12
     assert abs(_expect(2, [C.one(C.H, 0), C.two(C.CNOT, 0, 1)], "ZZ") - 1.0) < 1e-12
+
+
+def test_range_finder_amplifies_rounding():
+    """The reference's range finder (src/tn/mps.rs:983-1053) is ill-conditioned in its
+    input: Y = A A^H A Omega is not re-orthonormalized and Omega has rank 48, so the
+    Householder completion of Q follows the direction of nearly collinear columns. A 1e-14
+    relative perturbation of the state before the first range-finder layer changes
+    <0|psi> by far more than 1e-10 three range-finder layers later. Amplitudes of
+    workloads with range-finder splits are therefore reproducible only to this
+    amplification, by any implementation; per-split parity on identical inputs is not
+    affected."""
+    from test_gpu_parity import layers_of
+
+    om.set_qr_rank_tol(1e-10)
+    try:
+        n, chi = 40, 128
+        L = layers_of(C.rotation_cz_brickwork(40, 26, 0x5EED), n)
+        o = om.OracleMps(n, chi)
+        for lay in L[:20]:
+            o.apply_circuit(lay)
+        sites = [o.site(i) for i in range(n)]
+        amps = []
+        for eps in (0.0, 1e-14):
+            rng = np.random.default_rng(7)
+            p = om.OracleMps(n, chi)
+            for i, s in enumerate(sites):
+                d = (rng.standard_normal(s.shape) + 1j * rng.standard_normal(s.shape)) * eps * np.abs(s).max()
+                p.set_site(i, s + d)
+            for lay in L[20:23]:
+                p.apply_circuit(lay)
+            amps.append(p.amplitude_at_compact_zero())
+        assert abs(amps[1] - amps[0]) / abs(amps[0]) > 1e-7
+    finally:
+        om.set_qr_rank_tol(0.0)
+
+
+def test_reference_omega_has_rank_48():
+    """rademacher (src/tn/mps.rs:1103-1114) reads bits 8 and 9 of a 2^64 LCG, which
+    depend only on the low 10 state bits: entries repeat every 1024 draws, so Omega has at
+    most 64 distinct rows and numerical rank 48 at the saturated shapes."""
+    def fast_rademacher(cols, ell, salt):
+        # bits 8 and 9 of the state depend only on its low 10 bits (period 1024)
+        a, x = 6364136223846793005 % 1024, (salt | 1) % 1024
+        low = []
+        for _ in range(1024):
+            x = (a * x + 1) % 1024
+            low.append(x)
+        low = np.array(low)[np.arange(cols * ell) % 1024]
+        sc = 1.0 / math.sqrt(cols)
+        re = np.where(low & 0x100, -sc, sc)
+        im = np.where(low & 0x200, -sc, sc)
+        return (re + 1j * im).reshape(cols, ell)
+
+    O = fast_rademacher(256, 144, om.mix_seed(256, 256, 144))
+    assert np.array_equal(O, py_rademacher(256, 144, om.mix_seed(256, 256, 144)))
+    for rows, cols, ell in [(256, 256, 144), (512, 512, 272), (1024, 1024, 528)]:
+        O = fast_rademacher(cols, ell, om.mix_seed(rows, cols, ell))
+        s = np.linalg.svd(O, compute_uv=False)
+        assert int((s > 1e-10 * s[0]).sum()) == 48

@@ -404,6 +404,7 @@ struct wqc_tn_state {
             if (s.ptr) (void)hipFree(s.ptr);
         ws.release();
         tab.release();
+        jstats_buf.release();
         for (auto& sp : spans) {
             (void)hipEventDestroy(sp.a);
             (void)hipEventDestroy(sp.b);
@@ -574,6 +575,7 @@ struct wqc_tn_state {
         const double tol = 1e-15 * std::sqrt((double)max_m);
         int* d_rot = d_flags;
         int* d_done = d_flags + nj;
+        unsigned long long* d_jstats = jstats_dev();
         wqc::launch_fill_int(d_done, 0, nj, stream);
         std::vector<int> done(nj, 0);
         int active = nj;
@@ -583,7 +585,7 @@ struct wqc_tn_state {
             {
                 Timer tm(this, 3);
                 for (int r = 0; r < rounds; ++r)
-                    wqc::launch_jacobi_round(dj, nj, pairs, r, tol, d_rot, d_done, stream);
+                    wqc::launch_jacobi_round(dj, nj, pairs, r, tol, d_rot, d_done, d_jstats, stream);
                 check_launch();
                 klaunch[3] += rounds;
             }
@@ -608,6 +610,24 @@ struct wqc_tn_state {
     }
 
     int* upload_ints(const std::vector<int>& v) { return upload(v); }
+
+    // Device counters of the Jacobi rounds: [0] inner sweeps, [1] pair eigensolves.
+    DevBuf jstats_buf;
+    unsigned long long* jstats_dev() {
+        if (!jstats_buf.p) {
+            jstats_buf.ensure(64);
+            HIPC(hipMemsetAsync(jstats_buf.p, 0, 64, stream));
+        }
+        return reinterpret_cast<unsigned long long*>(jstats_buf.p);
+    }
+    void read_jstats() {
+        if (!jstats_buf.p) return;
+        unsigned long long v[2] = {0, 0};
+        HIPC(hipMemcpyAsync(v, jstats_buf.p, sizeof v, hipMemcpyDeviceToHost, stream));
+        sync();
+        stats.jacobi_inner_sweeps = v[0];
+        stats.jacobi_pair_solves = v[1];
+    }
 
     // Frobenius^2 (deterministic two-stage sum) then checked_svd's dust rule, per matrix.
     void dust(const std::vector<double2*>& mats, const std::vector<long>& counts, double* d_fro2,
@@ -1657,8 +1677,10 @@ int wqc_tn_get_stats(const wqc_tn_state* st, wqc_tn_stats* out) {
         g_last_error = "null argument";
         return WQC_TN_INVALID_ARG;
     }
-    *out = st->stats;
-    return WQC_TN_OK;
+    return guarded([&] {
+        const_cast<wqc_tn_state*>(st)->read_jstats();
+        *out = st->stats;
+    });
 }
 
 int wqc_tn_kernel_times(const wqc_tn_state* st, double* ms, size_t len) {

@@ -104,7 +104,8 @@ void launch_dust(double2* const* ptrs, const long* counts, const double* fro2, d
 void launch_identity(double2* const* ptrs, const int* dims, int njobs, int max_dim,
                      hipStream_t s);
 void launch_jacobi_round(const JacobiJob* jobs, int njobs, int max_pairs, int round,
-                         double tol, int* rotated, const int* done, hipStream_t s);
+                         double tol, int* rotated, const int* done, unsigned long long* stats,
+                         hipStream_t s);
 void launch_row_norms(const JacobiJob* jobs, int njobs, int max_n, double* sigma,
                       const long* sigma_off, hipStream_t s);
 void launch_gemm(const GemmJob* jobs, int njobs, int max_m, int max_n, hipStream_t s);

@@ -179,47 +179,62 @@ __global__ __launch_bounds__(256) void k_identity(double2* const* __restrict__ p
 }
 
 // ------------------------------------------------------------------ block Jacobi round
-// One workgroup orthogonalizes the 2*JB rows of one block pair of one job: Gram matrix
-// G = P P^H of the pair (FP64), Hermitian eigensolve of G by parallel cyclic Jacobi in
-// LDS (16 disjoint rotations per step), then P <- T P and J_pair <- T J_pair with the
-// accumulated rotation T. A pair whose off-diagonal already satisfies
-// |G_ij| <= tol sqrt(G_ii G_jj) is left untouched; rotated[job] records that a rotation
-// was needed (the sweep is not converged).
+// One workgroup (4 waves) orthogonalizes the 2*JB rows of one block pair of one job:
+//  1. Gram matrix G = P P^H of the pair on FP64 MFMA (v_mfma_f64_16x16x4f64), operands
+//     loaded from HBM straight into registers, the columns split over the 4 waves;
+//  2. Hermitian eigensolve of G by parallel cyclic Jacobi in LDS: per step 16 disjoint
+//     rotations, each thread owning one 2x2 block of G and of the accumulated rotation T
+//     (G <- M G M^H and T <- M T in one phase);
+//  3. P <- T P over the columns of X and of J, again on FP64 MFMA with T in registers.
+// A pair (i, j) needs a rotation when |G_ij| > tol sqrt(G_ii G_jj) (relative, as the
+// reference's Jacobi, src/tn/svd_guard.rs:178): small singular values come out accurate
+// relative to themselves, so the rank count above 1e-12 and the dust rule see the same
+// values an exact SVD gives. The in-block eigensolve runs at most JINNER sweeps; the
+// next outer round recomputes the pair's Gram matrix from the rows themselves, so an
+// unfinished inner solve only costs another round. rotated[job] records that the sweep
+// is not converged; stats[0..1] count inner sweeps and pair solves.
 constexpr int JB = 16;
 constexpr int JP = 2 * JB;
-constexpr int JCH = 32;
+constexpr int JINNER = 6;
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4v mfma_f64(double a, double b, d4v c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
 
 __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restrict__ jobs,
                                                       int round, double tol,
                                                       int* __restrict__ rotated,
-                                                      const int* __restrict__ done) {
+                                                      const int* __restrict__ done,
+                                                      unsigned long long* __restrict__ stats) {
     const int job = blockIdx.y;
     if (done[job]) return;
     const JacobiJob jb = jobs[job];
     const int nb = (jb.n + JB - 1) / JB;
     const int nbe = nb + (nb & 1);
     if (round >= nbe - 1) return;
-    const int k = blockIdx.x;
-    if (k >= nbe / 2) return;
+    const int kpair = blockIdx.x;
+    if (kpair >= nbe / 2) return;
     int bi, bj;
-    if (k == 0) {
+    if (kpair == 0) {
         bi = nbe - 1;
         bj = round;
     } else {
-        bi = (round + k) % (nbe - 1);
-        bj = (round - k + nbe - 1) % (nbe - 1);
+        bi = (round + kpair) % (nbe - 1);
+        bj = (round - kpair + nbe - 1) % (nbe - 1);
     }
 
     __shared__ double2 G[JP][JP + 1];
     __shared__ double2 T[JP][JP + 1];
-    __shared__ double2 P[JP][JCH + 1];
+    __shared__ double Gp[2][6][4][64];  // partial Gram tiles of two waves
     __shared__ int s_row[JP];
     __shared__ int rp[16], rq[16], ract[16];
-    __shared__ double rcs[16], rsn[16], rtg[16], ra[16], rd[16];
-    __shared__ double2 rph[16];
+    __shared__ double rcs[16], rtg[16], ra[16], rd[16];
+    __shared__ double2 rm01[16], rm10[16];
     __shared__ int s_any;
 
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid < JP) {
         const int b = tid < JB ? bi : bj;
         const int r = b * JB + (tid & (JB - 1));
@@ -228,61 +243,111 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
     if (tid == 0) s_any = 0;
     __syncthreads();
 
-    // Gram matrix, 2x2 entries per thread.
-    const int ti = tid >> 4, tj = tid & 15;
-    double2 g00 = cz(), g01 = cz(), g10 = cz(), g11 = cz();
-    for (int c0 = 0; c0 < jb.m; c0 += JCH) {
-        for (int e = tid; e < JP * JCH; e += 256) {
-            const int lr = e / JCH, lc = e % JCH;
-            const int gr = s_row[lr], gc = c0 + lc;
-            P[lr][lc] = (gr >= 0 && gc < jb.m) ? jb.X[(long)gr * jb.ldx + gc] : cz();
+    // ---- 1. Gram on MFMA. Lane (i = lane%16, g = lane/16) holds columns c0+4g..c0+4g+3 of
+    // local rows i and 16+i; MFMA step s uses column c0 + 4g + s for both operands.
+    {
+        const int i = lane & 15, g = lane >> 4;
+        const int r0 = s_row[i], r1 = s_row[16 + i];
+        const double2* x0 = r0 >= 0 ? jb.X + (long)r0 * jb.ldx : nullptr;
+        const double2* x1 = r1 >= 0 ? jb.X + (long)r1 * jb.ldx : nullptr;
+        d4v a00r = {0, 0, 0, 0}, a00i = a00r, a01r = a00r, a01i = a00r, a11r = a00r, a11i = a00r;
+        for (int c0 = wave * 16; c0 < jb.m; c0 += 64) {
+            double2 u[4], v[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int c = c0 + 4 * g + s;
+                u[s] = (x0 && c < jb.m) ? x0[c] : cz();
+                v[s] = (x1 && c < jb.m) ? x1[c] : cz();
+            }
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                // G_IJ += P_I P_J^H: re = ar*br + ai*bi, im = ai*br - ar*bi
+                a00r = mfma_f64(u[s].x, u[s].x, a00r);
+                a00r = mfma_f64(u[s].y, u[s].y, a00r);
+                a00i = mfma_f64(u[s].y, u[s].x, a00i);
+                a00i = mfma_f64(-u[s].x, u[s].y, a00i);
+                a01r = mfma_f64(u[s].x, v[s].x, a01r);
+                a01r = mfma_f64(u[s].y, v[s].y, a01r);
+                a01i = mfma_f64(u[s].y, v[s].x, a01i);
+                a01i = mfma_f64(-u[s].x, v[s].y, a01i);
+                a11r = mfma_f64(v[s].x, v[s].x, a11r);
+                a11r = mfma_f64(v[s].y, v[s].y, a11r);
+                a11i = mfma_f64(v[s].y, v[s].x, a11i);
+                a11i = mfma_f64(-v[s].x, v[s].y, a11i);
+            }
+        }
+        // Deterministic reduction over the waves: (w0 + w2) + (w1 + w3).
+#define WQC_GP_ALL(OP) OP(0, a00r) OP(1, a00i) OP(2, a01r) OP(3, a01i) OP(4, a11r) OP(5, a11i)
+#define WQC_GP_STORE(t, v) for (int j = 0; j < 4; ++j) Gp[slot][t][j][lane] = v[j];
+#define WQC_GP_ADD(t, v) for (int j = 0; j < 4; ++j) v[j] += Gp[slot][t][j][lane];
+        if (wave >= 2) {
+            const int slot = wave - 2;
+            WQC_GP_ALL(WQC_GP_STORE)
         }
         __syncthreads();
-#pragma unroll 4
-        for (int c = 0; c < JCH; ++c) {
-            const double2 a0 = P[2 * ti][c], a1 = P[2 * ti + 1][c];
-            const double2 b0 = P[2 * tj][c], b1 = P[2 * tj + 1][c];
-            cmacc(g00, a0, b0);
-            cmacc(g01, a0, b1);
-            cmacc(g10, a1, b0);
-            cmacc(g11, a1, b1);
+        if (wave < 2) {
+            const int slot = wave;
+            WQC_GP_ALL(WQC_GP_ADD)
         }
         __syncthreads();
+        if (wave == 1) {
+            const int slot = 0;
+            WQC_GP_ALL(WQC_GP_STORE)
+        }
+        __syncthreads();
+#undef WQC_GP_ALL
+#undef WQC_GP_STORE
+#undef WQC_GP_ADD
+        if (wave == 0) {
+            // MFMA f64 output: element j of lane l is row 4j + l/16, column l%16.
+            const int r = lane >> 4, c = lane & 15;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const double g00r = a00r[j] + Gp[0][0][j][lane], g00i = a00i[j] + Gp[0][1][j][lane];
+                const double g01r = a01r[j] + Gp[0][2][j][lane], g01i = a01i[j] + Gp[0][3][j][lane];
+                const double g11r = a11r[j] + Gp[0][4][j][lane], g11i = a11i[j] + Gp[0][5][j][lane];
+                const int rr = 4 * j + r;
+                G[rr][c] = make_double2(g00r, g00i);
+                G[rr][16 + c] = make_double2(g01r, g01i);
+                G[16 + c][rr] = make_double2(g01r, -g01i);
+                G[16 + rr][16 + c] = make_double2(g11r, g11i);
+            }
+        }
     }
-    G[2 * ti][2 * tj] = g00;
-    G[2 * ti][2 * tj + 1] = g01;
-    G[2 * ti + 1][2 * tj] = g10;
-    G[2 * ti + 1][2 * tj + 1] = g11;
     __syncthreads();
     {
         bool off = false;
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-#pragma unroll
-            for (int v = 0; v < 2; ++v) {
-                const int i = 2 * ti + u, j = 2 * tj + v;
-                if (i < j) {
-                    const double2 c = G[i][j];
-                    const double g = hypot(c.x, c.y);
-                    if (g > 0.0 && g > tol * sqrt(G[i][i].x * G[j][j].x)) off = true;
-                }
+        for (int e = tid; e < JP * JP; e += 256) {
+            const int i = e / JP, j = e % JP;
+            if (i < j) {
+                const double2 c = G[i][j];
+                const double gg = hypot(c.x, c.y);
+                if (gg > 0.0 && gg > tol * sqrt(fmax(G[i][i].x, 0.0) * fmax(G[j][j].x, 0.0))) off = true;
             }
+        }
         if (off) s_any = 1;
     }
     __syncthreads();
-    if (!s_any) return;
-    if (tid == 0) rotated[job] = 1;
-
+    // Every wave reads the flag before thread 0 clears it.
+    const int need = s_any;
+    __syncthreads();
+    if (!need) return;
+    if (tid == 0) {
+        rotated[job] = 1;
+        s_any = 0;
+    }
     for (int e = tid; e < JP * JP; e += 256) {
         const int i = e / JP, j = e % JP;
         T[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
     }
     __syncthreads();
 
-    const double tol_in = 1e-15;
-    if (tid == 0) s_any = 0;
-    __syncthreads();
-    for (int sweep = 0; sweep < 16; ++sweep) {
+    // ---- 2. eigensolve of G (T accumulates the row rotations).
+    const double tol_in = tol;
+    int sweeps_done = 0;
+    const int bk = tid >> 4, bk2 = tid & 15;
+    for (int sweep = 0; sweep < JINNER; ++sweep) {
+        ++sweeps_done;
         for (int step = 0; step < JP - 1; ++step) {
             if (tid < 16) {
                 int p, q;
@@ -295,57 +360,88 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
                 }
                 const double a = G[p][p].x, d = G[q][q].x;
                 const double2 c = G[p][q];
-                const double g = hypot(c.x, c.y);
-                const int act = (g > 0.0) && (g > tol_in * sqrt(fmax(a, 0.0) * fmax(d, 0.0)));
+                const double gg = hypot(c.x, c.y);
+                const int act = gg > 0.0 && gg > tol_in * sqrt(fmax(a, 0.0) * fmax(d, 0.0));
                 rp[tid] = p;
                 rq[tid] = q;
                 ract[tid] = act;
                 if (act) {
-                    const double zeta = (d - a) / (2.0 * g);
+                    const double zeta = (d - a) / (2.0 * gg);
                     const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                     const double cs = 1.0 / sqrt(1.0 + t * t);
+                    const double sn = cs * t;
+                    const double2 ph = make_double2(c.x / gg, c.y / gg);
                     rcs[tid] = cs;
-                    rsn[tid] = cs * t;
-                    rph[tid] = make_double2(c.x / g, c.y / g);
-                    rtg[tid] = t * g;
+                    rm01[tid] = make_double2(-ph.x * sn, -ph.y * sn);  // M = [[cs, -ph sn],
+                    rm10[tid] = make_double2(ph.x * sn, -ph.y * sn);   //      [sn conj(ph), cs]]
+                    rtg[tid] = t * gg;
                     ra[tid] = a;
                     rd[tid] = d;
                     s_any = 1;
                 }
             }
             __syncthreads();
-            // rows p, q of G and T: M = [[cs, -ph sn], [sn conj(ph), cs]]
-            for (int e = tid; e < 16 * JP * 2; e += 256) {
-                const int mat = e / (16 * JP), rem = e % (16 * JP);
-                const int kk = rem / JP, j = rem % JP;
-                if (!ract[kk]) continue;
-                const int p = rp[kk], q = rq[kk];
-                const double cs = rcs[kk], sn = rsn[kk];
-                const double2 ph = rph[kk];
-                double2(*M)[JP + 1] = mat ? T : G;
-                const double2 xp = M[p][j], xq = M[q][j];
-                M[p][j] = csub(cscale(xp, cs), cmul(cscale(ph, sn), xq));
-                M[q][j] = cadd(cmul(cscale(cconj(ph), sn), xp), cscale(xq, cs));
+            const int act1 = ract[bk], act2 = ract[bk2];
+            const int p1 = rp[bk], q1 = rq[bk], p2 = rp[bk2], q2 = rq[bk2];
+            double2 n00, n01, n10, n11, t00, t01, t10, t11;
+            if (act1 | act2) {
+                double2 h00 = G[p1][p2], h01 = G[p1][q2], h10 = G[q1][p2], h11 = G[q1][q2];
+                t00 = T[p1][p2];
+                t01 = T[p1][q2];
+                t10 = T[q1][p2];
+                t11 = T[q1][q2];
+                if (act1) {
+                    const double cs = rcs[bk];
+                    const double2 m01 = rm01[bk], m10 = rm10[bk];
+                    double2 x00 = cadd(cscale(h00, cs), cmul(m01, h10));
+                    double2 x01 = cadd(cscale(h01, cs), cmul(m01, h11));
+                    double2 x10 = cadd(cmul(m10, h00), cscale(h10, cs));
+                    double2 x11 = cadd(cmul(m10, h01), cscale(h11, cs));
+                    h00 = x00;
+                    h01 = x01;
+                    h10 = x10;
+                    h11 = x11;
+                    double2 y00 = cadd(cscale(t00, cs), cmul(m01, t10));
+                    double2 y01 = cadd(cscale(t01, cs), cmul(m01, t11));
+                    double2 y10 = cadd(cmul(m10, t00), cscale(t10, cs));
+                    double2 y11 = cadd(cmul(m10, t01), cscale(t11, cs));
+                    t00 = y00;
+                    t01 = y01;
+                    t10 = y10;
+                    t11 = y11;
+                }
+                if (act2) {
+                    const double cs = rcs[bk2];
+                    const double2 c01 = cconj(rm01[bk2]), c10 = cconj(rm10[bk2]);
+                    n00 = cadd(cscale(h00, cs), cmul(h01, c01));
+                    n01 = cadd(cmul(h00, c10), cscale(h01, cs));
+                    n10 = cadd(cscale(h10, cs), cmul(h11, c01));
+                    n11 = cadd(cmul(h10, c10), cscale(h11, cs));
+                } else {
+                    n00 = h00;
+                    n01 = h01;
+                    n10 = h10;
+                    n11 = h11;
+                }
+                if (bk == bk2 && act1) {
+                    n00 = make_double2(ra[bk] - rtg[bk], 0.0);
+                    n11 = make_double2(rd[bk] + rtg[bk], 0.0);
+                    n01 = cz();
+                    n10 = cz();
+                }
             }
             __syncthreads();
-            // columns p, q of G: G <- G M^H
-            for (int e = tid; e < 16 * JP; e += 256) {
-                const int kk = e / JP, i = e % JP;
-                if (!ract[kk]) continue;
-                const int p = rp[kk], q = rq[kk];
-                const double cs = rcs[kk], sn = rsn[kk];
-                const double2 ph = rph[kk];
-                const double2 xp = G[i][p], xq = G[i][q];
-                G[i][p] = csub(cscale(xp, cs), cmul(cscale(cconj(ph), sn), xq));
-                G[i][q] = cadd(cmul(cscale(ph, sn), xp), cscale(xq, cs));
-            }
-            __syncthreads();
-            if (tid < 16 && ract[tid]) {
-                const int p = rp[tid], q = rq[tid];
-                G[p][p] = make_double2(ra[tid] - rtg[tid], 0.0);
-                G[q][q] = make_double2(rd[tid] + rtg[tid], 0.0);
-                G[p][q] = cz();
-                G[q][p] = cz();
+            if (act1 | act2) {
+                G[p1][p2] = n00;
+                G[p1][q2] = n01;
+                G[q1][p2] = n10;
+                G[q1][q2] = n11;
+                if (act1) {
+                    T[p1][p2] = t00;
+                    T[p1][q2] = t01;
+                    T[q1][p2] = t10;
+                    T[q1][q2] = t11;
+                }
             }
             __syncthreads();
         }
@@ -356,30 +452,64 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
         if (tid == 0) s_any = 0;
         __syncthreads();
     }
+    if (tid == 0 && stats) {
+        atomicAdd(&stats[0], (unsigned long long)sweeps_done);
+        atomicAdd(&stats[1], 1ull);
+    }
 
-    // P <- T P over the columns of X, then the rows of J.
-    for (int pass = 0; pass < 2; ++pass) {
-        double2* M = pass == 0 ? jb.X : jb.J;
-        if (!M) break;
-        const int cols = pass == 0 ? jb.m : jb.n;
-        const long ld = pass == 0 ? jb.ldx : jb.n;
-        for (int c0 = 0; c0 < cols; c0 += JCH) {
-            for (int e = tid; e < JP * JCH; e += 256) {
-                const int lr = e / JCH, lc = e % JCH;
-                const int gr = s_row[lr], gc = c0 + lc;
-                P[lr][lc] = (gr >= 0 && gc < cols) ? M[(long)gr * ld + gc] : cz();
+    // ---- 3. P <- T P on MFMA. Lane l supplies T[I*16 + l%16][4kk + l/16] (registers) and
+    // P[4kk + l/16][c0 + l%16]; output element j of lane l is local row I*16 + 4j + l/16.
+    {
+        const int i = lane & 15, g = lane >> 4;
+        double tr[2][8], ti[2][8], tni[2][8];
+#pragma unroll
+        for (int I = 0; I < 2; ++I)
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) {
+                const double2 t = T[I * 16 + i][4 * kk + g];
+                tr[I][kk] = t.x;
+                ti[I][kk] = t.y;
+                tni[I][kk] = -t.y;
             }
-            __syncthreads();
-            for (int e = tid; e < JP * JCH; e += 256) {
-                const int lr = e / JCH, lc = e % JCH;
-                const int gr = s_row[lr], gc = c0 + lc;
-                if (gr < 0 || gc >= cols) continue;
-                double2 acc = cz();
-#pragma unroll 8
-                for (int j = 0; j < JP; ++j) cmac(acc, T[lr][j], P[j][lc]);
-                M[(long)gr * ld + gc] = acc;
+        int prow[8];
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) prow[kk] = s_row[4 * kk + g];
+        int orow[2][4];
+#pragma unroll
+        for (int I = 0; I < 2; ++I)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) orow[I][j] = s_row[I * 16 + 4 * j + g];
+        for (int pass = 0; pass < 2; ++pass) {
+            double2* M = pass == 0 ? jb.X : jb.J;
+            if (!M) break;
+            const int cols = pass == 0 ? jb.m : jb.n;
+            const long ld = pass == 0 ? jb.ldx : jb.n;
+            for (int c0 = wave * 16; c0 < cols; c0 += 64) {
+                const int c = c0 + i;
+                const bool cok = c < cols;
+                double2 b[8];
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk)
+                    b[kk] = (cok && prow[kk] >= 0) ? M[(long)prow[kk] * ld + c] : cz();
+#pragma unroll
+                for (int I = 0; I < 2; ++I) {
+                    d4v dr = {0, 0, 0, 0}, di = dr;
+#pragma unroll
+                    for (int kk = 0; kk < 8; ++kk) {
+                        dr = mfma_f64(tr[I][kk], b[kk].x, dr);
+                        dr = mfma_f64(tni[I][kk], b[kk].y, dr);
+                        di = mfma_f64(tr[I][kk], b[kk].y, di);
+                        di = mfma_f64(ti[I][kk], b[kk].x, di);
+                    }
+                    if (cok) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int gr = orow[I][j];
+                            if (gr >= 0) M[(long)gr * ld + c] = make_double2(dr[j], di[j]);
+                        }
+                    }
+                }
             }
-            __syncthreads();
         }
     }
 }
@@ -715,9 +845,9 @@ void launch_identity(double2* const* ptrs, const int* dims, int njobs, int max_d
 }
 
 void launch_jacobi_round(const JacobiJob* jobs, int njobs, int max_pairs, int round, double tol,
-                         int* rotated, const int* done, hipStream_t s) {
+                         int* rotated, const int* done, unsigned long long* stats, hipStream_t s) {
     hipLaunchKernelGGL(k_jacobi_round, dim3(max_pairs, njobs), dim3(256), 0, s, jobs, round, tol,
-                       rotated, done);
+                       rotated, done, stats);
 }
 
 void launch_row_norms(const JacobiJob* jobs, int njobs, int max_n, double* sigma,

@@ -44,7 +44,8 @@ class Stats(ct.Structure):
                 ("rsvd_fallbacks", ct.c_uint64), ("svd_check_failures", ct.c_uint64),
                 ("jacobi_sweeps", ct.c_uint64), ("truncated_splits", ct.c_uint64),
                 ("peak_vram_bytes", ct.c_uint64), ("max_discarded_weight", ct.c_double),
-                ("sum_discarded_weight", ct.c_double)]
+                ("sum_discarded_weight", ct.c_double), ("jacobi_inner_sweeps", ct.c_uint64),
+                ("jacobi_pair_solves", ct.c_uint64)]
 
 
 class WqcTnError(RuntimeError):
