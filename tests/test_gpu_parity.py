@@ -353,3 +353,26 @@ def test_range_finder_per_split_on_oracle_states(wt):
         assert rsvd_seen >= 10
     finally:
         om.set_qr_rank_tol(0.0)
+
+
+def test_golden_config3_chi512(wt):
+    # 100 qubits, depth 30, chi = 512. Layers 1-24 use exact splits only (bonds up to 512):
+    # 1e-10 against the oracle. Range-finder splits start at layer 25; from there bonds and
+    # split counters must match (amplitudes are amplification-limited, see above).
+    want = load_golden("config3")
+    n, chi, gates = C.config3()
+    g = wt.MpsState(n, chi)
+    for L, lay in enumerate(layers_of(gates, n)):
+        g.apply_circuit(lay)
+        cp = want["checkpoints"].get(str(L + 1))
+        if cp and cp["counters"]["rsvd_kept"] == 0:
+            check_against_golden(g, cp, f"config3 layer {L + 1}")
+        elif cp:
+            st = g.stats()
+            assert g.bond_dims() == cp["bonds"], L + 1
+            assert (st["rsvd_kept"], st["rsvd_fallbacks"]) == (cp["counters"]["rsvd_kept"], cp["counters"]["rsvd_fallbacks"])
+    st = g.stats()
+    assert g.bond_dims() == want["bonds"]
+    assert (st["two_qubit_units"], st["rsvd_kept"], st["rsvd_fallbacks"]) == (
+        want["counters"]["units"], want["counters"]["rsvd_kept"], want["counters"]["rsvd_fallbacks"])
+    say("config3 final amp", g.amplitude_at_compact_zero(), complex(*want["amplitude"]))
