@@ -7,9 +7,10 @@ one pass of the whole circuit from |0...0> through the C ABI (wqc_tn_apply_circu
 the final <0...0|psi>; inputs (the gate list) are on the host, every site tensor lives in
 HBM for the whole step.
 
-Printed JSON line: value = merges/s of the headline form; the other form, the relative
-amplitude error between the forms, a roofline record for the dominant kernel and a CPU
-baseline ride along. `--gpus N` with N > 1 exits: this build runs on one GPU.
+Printed JSON line: value = merges/s of the mixed form (FP32 early Jacobi sweeps, FP64
+finish); the plain FP64 form's rate, their ratio, the relative amplitude error between the
+forms (after the range-finder-free first 24 layers and at the end), a roofline record for
+the dominant kernel and a CPU baseline ride along. `--gpus N` with N > 1 exits: this build runs on one GPU.
 """
 
 from __future__ import annotations
@@ -57,7 +58,18 @@ def svd_flops(rows, cols, path, ell):
     return 4.0 * (4.0 * m * m * n + 8.0 * m * n * n + 9.0 * n ** 3)
 
 
-def run_gpu(wt, n, chi, gates, precision, steps, warmup, timed_pass=True):
+def layers_of(gates, n):
+    out, i = [], 0
+    while i < len(gates):
+        j = i + n
+        while j < len(gates) and gates[j][0] in (C.CZ, C.CNOT):
+            j += 1
+        out.append(gates[i:j])
+        i = j
+    return out
+
+
+def run_gpu(wt, n, chi, gates, precision, steps, warmup, prefix_layers=24):
     amps = []
     for _ in range(warmup):
         st = wt.MpsState(n, chi, precision)
@@ -73,21 +85,29 @@ def run_gpu(wt, n, chi, gates, precision, steps, warmup, timed_pass=True):
         st.free()
     dt = time.perf_counter() - t0
     info = {"seconds": dt, "amplitude": amps[-1], "stats": stats}
-    if timed_pass:
-        # A further pass with HIP-event timing of every kernel family and the split log.
-        st = wt.MpsState(n, chi, precision)
-        st.enable_timing(True)
-        st.enable_split_log(True)
-        t1 = time.perf_counter()
-        st.apply_circuit(gates)
-        amp = st.amplitude_at_compact_zero()
-        info["timed_pass_seconds"] = time.perf_counter() - t1
-        info["kernel_ms"] = st.kernel_times()
-        info["launches"] = st.kernel_launches()
-        info["split_log"] = st.split_log()
-        info["timed_amplitude"] = amp
-        st.free()
+    # A further, untimed pass with HIP-event timing of every kernel family, the split log,
+    # and the amplitude after the range-finder-free prefix of the circuit.
+    st = wt.MpsState(n, chi, precision)
+    st.enable_timing(True)
+    st.enable_split_log(True)
+    t1 = time.perf_counter()
+    layers = layers_of(gates, n)
+    for L, lay in enumerate(layers):
+        st.apply_circuit(lay)
+        if L + 1 == prefix_layers:
+            info["prefix_amplitude"] = st.amplitude_at_compact_zero()
+    amp = st.amplitude_at_compact_zero()
+    info["timed_pass_seconds"] = time.perf_counter() - t1
+    info["kernel_ms"] = st.kernel_times()
+    info["launches"] = st.kernel_launches()
+    info["split_log"] = st.split_log()
+    info["timed_amplitude"] = amp
+    st.free()
     return info
+
+
+def rel(a, b):
+    return abs(a - b) / abs(b) if b != 0 else float("inf")
 
 
 def cpu_baseline(sample_units: int = 1):
@@ -128,7 +148,7 @@ def main():
     ap.add_argument("--qubits", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-units", type=int, default=1)
-    ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--form", default="both", choices=["both", "fp64", "mixed"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -142,38 +162,59 @@ def main():
     units = C.two_qubit_units(gates, n)
     print(f"# workload: {n}q depth {args.depth} chi {chi}: {units} two-qubit units", file=sys.stderr,
           flush=True)
-    fp64 = run_gpu(wt, n, chi, gates, wt.PRECISION_FP64, args.steps, args.warmup)
-    ms_step = 1e3 * fp64["seconds"] / args.steps
-    value = units * args.steps / fp64["seconds"]
-    log = fp64["split_log"]
-    alg = sum(unit_flops(*e) for e in log)
+    forms = {}
+    for name, prec in (("fp64", wt.PRECISION_FP64), ("mixed", wt.PRECISION_MIXED_FP32)):
+        if args.form not in ("both", name):
+            continue
+        print(f"# form {name}", file=sys.stderr, flush=True)
+        forms[name] = run_gpu(wt, n, chi, gates, prec, args.steps, args.warmup)
+    head = "mixed" if "mixed" in forms else "fp64"
+    main_run = forms[head]
+    ms_step = 1e3 * main_run["seconds"] / args.steps
+    value = units * args.steps / main_run["seconds"]
+    log = main_run["split_log"]
     svd = sum(svd_flops(e[0], e[1], e[3], e[4]) for e in log)
-    kms = fp64["kernel_ms"]
-    launches = fp64["launches"]
+    alg = sum(unit_flops(*e[:6]) for e in log)
+    kms = main_run["kernel_ms"]
+    launches = main_run["launches"]
     jac_ms = kms.get("jacobi", 0.0)
     dom = max(kms, key=lambda k: kms[k]) if kms else "jacobi"
     achieved = svd / (jac_ms * 1e-3) / 1e12 if jac_ms > 0 else None
     roofline = {
-        "bound": "mfma", "kernel": "k_jacobi_round", "achieved": achieved,
-        "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "bound": "mfma", "kernel": "k_jacobi_round" + ("_lo + k_jacobi_round" if head == "mixed" else ""),
+        "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
         "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None, "traffic": None,
         "launches": launches.get("jacobi"),
         "avg_launch_us": 1e3 * jac_ms / max(1, launches.get("jacobi", 1)),
         "dominant_family": dom,
     }
+    form_report = {}
+    for name, r in forms.items():
+        form_report[name] = {
+            "merges_per_s": units * args.steps / r["seconds"],
+            "amplitude": [r["amplitude"].real, r["amplitude"].imag],
+            "prefix24_amplitude": [r["prefix_amplitude"].real, r["prefix_amplitude"].imag],
+            "alg_tflops_whole_step": sum(unit_flops(*e[:6]) for e in r["split_log"]) / r["timed_pass_seconds"] / 1e12,
+            "kernel_ms": r["kernel_ms"], "launches": r["launches"], "stats": r["stats"]}
     out = {
         "metric": "2q gate-merges/s at chi=512 (mixed vs FP64 plain) + rel. amp error vs FP64",
         "value": value, "unit": "merges/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if head == "fp64" else "f32+f64",
+        "data": "synthetic",
         "config": {"workload": f"rotation+CZ brickwork {n}q depth {args.depth} chi={chi}",
                    "qubits": n, "depth": args.depth, "chi": chi, "units": units,
-                   "form": "plain FP64"},
-        "fp64": {"merges_per_s": value, "amplitude": [fp64["amplitude"].real, fp64["amplitude"].imag],
-                 "alg_tflops_whole_step": alg / fp64["timed_pass_seconds"] / 1e12,
-                 "kernel_ms": kms, "launches": launches, "stats": {k: v for k, v in fp64["stats"].items()}},
+                   "form": "mixed (FP32 early Jacobi sweeps, FP64 finish)" if head == "mixed" else "plain FP64"},
+        "forms": form_report,
         "roofline": roofline,
     }
+    if "mixed" in forms and "fp64" in forms:
+        f, m = forms["fp64"], forms["mixed"]
+        out["mixed_over_fp64"] = (units / m["seconds"]) / (units / f["seconds"])
+        out["rel_amp_error_vs_fp64"] = {
+            "after_layer_24_exact_splits_only": rel(m["prefix_amplitude"], f["prefix_amplitude"]),
+            "final_after_range_finder_layers": rel(m["amplitude"], f["amplitude"]),
+        }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_units)
     print(json.dumps(out), flush=True)

@@ -71,9 +71,10 @@ typedef struct wqc_tn_gate {
 
 /* Arithmetic of the two-site split.
  *  FP64       : plain complex128 everywhere (the drop-in form).
- *  MIXED_FP32 : complex64 GEMMs (merge, range finder, factors) and complex64 early
- *               Jacobi sweeps; the last Jacobi sweeps, the singular values and the
- *               truncation decision in FP64. */
+ *  MIXED_FP32 : the Jacobi SVD runs its early sweeps on complex64 rows (FP32 MFMA Gram and
+ *               rotation, FP32 eigensolve) while the accumulated rotation stays unitary in
+ *               FP64; the rows are then recomputed in FP64 and the last sweeps, the
+ *               singular values, the truncation decision and the factors are FP64. */
 typedef enum wqc_tn_precision {
     WQC_TN_PRECISION_FP64 = 0,
     WQC_TN_PRECISION_MIXED_FP32 = 1
@@ -97,6 +98,7 @@ typedef struct wqc_tn_stats {
     double sum_discarded_weight;
     uint64_t jacobi_inner_sweeps; /* sweeps of the per-block-pair eigensolves               */
     uint64_t jacobi_pair_solves;  /* block-pair eigensolves run                             */
+    uint64_t jacobi_sweeps_lo;    /* low-precision sweeps of the mixed form                 */
 } wqc_tn_stats;
 
 /* ---- errors ------------------------------------------------------------------ */
@@ -213,9 +215,9 @@ int wqc_tn_enable_timing(wqc_tn_state *st, int enable);
 /* Launch counts per kernel family (same indexing as wqc_tn_kernel_times). */
 int wqc_tn_kernel_launches(const wqc_tn_state *st, uint64_t *count, size_t len);
 
-/* Split log for measurement: when enabled, every two-qubit unit appends six int32
- * values {rows, cols, bond, path, ell, chi_kept} (path 0 exact, 1 range finder kept,
- * 2 range finder fell back to exact). wqc_tn_split_log copies up to cap entries and
+/* Split log for measurement: when enabled, every two-qubit unit appends eight int32
+ * values {rows, cols, bond, path, ell, chi_kept, fp64_sweeps, low_sweeps} (path 0 exact,
+ * 1 range finder kept, 2 range finder fell back to exact; sweeps of the unit's Jacobi). wqc_tn_split_log copies up to cap entries and
  * returns the number recorded. */
 int wqc_tn_enable_split_log(wqc_tn_state *st, int enable);
 size_t wqc_tn_split_log(const wqc_tn_state *st, int32_t *out, size_t cap_entries);

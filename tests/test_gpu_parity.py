@@ -376,3 +376,29 @@ def test_golden_config3_chi512(wt):
     assert (st["two_qubit_units"], st["rsvd_kept"], st["rsvd_fallbacks"]) == (
         want["counters"]["units"], want["counters"]["rsvd_kept"], want["counters"]["rsvd_fallbacks"])
     say("config3 final amp", g.amplitude_at_compact_zero(), complex(*want["amplitude"]))
+
+
+
+# ---------------------------------------------------------------- mixed form
+@pytest.mark.parametrize("name", ["config1", "config2"])
+def test_mixed_form_matches_golden(wt, name):
+    # FP32 early sweeps, FP64 finish: same bonds and counters, amplitude to 1e-10.
+    n, chi, gates = getattr(C, name)()
+    g = wt.MpsState(n, chi, wt.PRECISION_MIXED_FP32)
+    g.apply_circuit(gates)
+    check_against_golden(g, load_golden(name), "mixed " + name)
+    assert g.stats()["jacobi_sweeps_lo"] > 0
+
+
+def test_mixed_form_truncating_and_range_finder(wt):
+    want = load_golden("rsvd_brickwork40_chi128")
+    gates = C.rotation_cz_brickwork(40, 26, 0x5EED)
+    g = wt.MpsState(40, 128, wt.PRECISION_MIXED_FP32)
+    for L, lay in enumerate(layers_of(gates, 40)):
+        g.apply_circuit(lay)
+        cp = want["checkpoints"].get(str(L + 1))
+        if cp and cp["counters"]["rsvd_kept"] == 0:
+            check_against_golden(g, cp, f"mixed layer {L + 1}")
+    st = g.stats()
+    assert g.bond_dims() == want["bonds"]
+    assert st["rsvd_kept"] == want["counters"]["rsvd_kept"]

@@ -94,6 +94,14 @@ const size_t RSVD_OVERSAMPLE = 16;          // src/tn/mps.rs:795
 // fraction of its norm is dependent (identity reflector), as in exact arithmetic.
 const double QR_RANK_TOL = 1e-10;
 const int MAX_SWEEPS = 60;
+// Rows whose squared norm is below NULL_ROWS_SQ / n are not rotated: together they carry at
+// most 1e-13 of singular value, an order below SINGULAR_CUTOFF.
+const double NULL_ROWS_SQ = 1e-26;
+// Relative orthogonality tolerances of the Jacobi sweeps, times sqrt(row length): an order
+// above the rounding of an FP64 (FP32) Gram entry over that many terms.
+const double JACOBI_TOL = 1e-14;
+const double JACOBI_TOL_LO = 1e-6;
+const int MAX_SWEEPS_LO = 30;
 
 size_t retained_chi(const std::vector<double>& v, size_t max_bond, size_t rows, size_t cols) {
     if (v.empty() || max_bond == 0 || rows == 0 || cols == 0) return 0;
@@ -374,6 +382,7 @@ struct wqc_tn_state {
     std::vector<Site> sites;
     hipStream_t stream = nullptr;
     DevBuf ws;        // split workspace
+    DevBuf ws32;      // mixed-precision rows and the J X product
     DevBuf tab;       // device job tables
     char* htab = nullptr;  // pinned staging for tables
     size_t htab_cap = 0, tab_off = 0;
@@ -403,6 +412,7 @@ struct wqc_tn_state {
         for (auto& s : sites)
             if (s.ptr) (void)hipFree(s.ptr);
         ws.release();
+        ws32.release();
         tab.release();
         jstats_buf.release();
         for (auto& sp : spans) {
@@ -417,7 +427,7 @@ struct wqc_tn_state {
 
     // ------------------------------------------------------------ helpers
     void note_peak() {
-        uint64_t now = site_bytes + ws.bytes + tab.bytes;
+        uint64_t now = site_bytes + ws.bytes + ws32.bytes + tab.bytes;
         if (now > stats.peak_vram_bytes) stats.peak_vram_bytes = now;
     }
 
@@ -466,8 +476,15 @@ struct wqc_tn_state {
     }
 
     // ------------------------------------------------------------ lifetime
+    // Sweeps of the in-block eigensolve per round (WQC_TN_JACOBI_INNER overrides, 1..32).
+    int inner_cap = 2;
+
     void init(wqc_tn_device* d, size_t qn, size_t mb, int prec) {
         dev = d;
+        if (const char* e = std::getenv("WQC_TN_JACOBI_INNER")) {
+            const int v = std::atoi(e);
+            if (v >= 1 && v <= 32) inner_cap = v;
+        }
         n = (int)qn;
         chi = mb;
         precision = prec;
@@ -559,9 +576,12 @@ struct wqc_tn_state {
     // ------------------------------------------------------------ Jacobi driver
     // Rotates the rows of every job's X until no block pair needs a rotation, then
     // writes the row norms into sigma (device, at sig_off[j]). Returns sweeps used.
+    std::vector<int> last_sweeps, last_sweeps_lo;  // per job of the latest jacobi call
+
     void jacobi(std::vector<JacobiJob>& jobs, double* d_sigma, const std::vector<long>& sig_off,
                 int* d_flags) {
         const int nj = (int)jobs.size();
+        last_sweeps.assign(nj, 0);
         if (!nj) return;
         JacobiJob* dj = upload(jobs);
         int max_n = 1, max_m = 1;
@@ -572,7 +592,8 @@ struct wqc_tn_state {
         const int nb = (max_n + 15) / 16;
         const int nbe = nb + (nb & 1);
         const int rounds = nbe - 1, pairs = nbe / 2;
-        const double tol = 1e-15 * std::sqrt((double)max_m);
+        const double tol = JACOBI_TOL * std::sqrt((double)max_m);
+        const double nu2 = NULL_ROWS_SQ / max_n;
         int* d_rot = d_flags;
         int* d_done = d_flags + nj;
         unsigned long long* d_jstats = jstats_dev();
@@ -585,7 +606,7 @@ struct wqc_tn_state {
             {
                 Timer tm(this, 3);
                 for (int r = 0; r < rounds; ++r)
-                    wqc::launch_jacobi_round(dj, nj, pairs, r, tol, d_rot, d_done, d_jstats, stream);
+                    wqc::launch_jacobi_round(dj, nj, pairs, r, tol, nu2, inner_cap, d_rot, d_done, d_jstats, stream);
                 check_launch();
                 klaunch[3] += rounds;
             }
@@ -594,6 +615,8 @@ struct wqc_tn_state {
             sync();
             const int* rot = reinterpret_cast<const int*>(hread);
             bool changed = false;
+            for (int j = 0; j < nj; ++j)
+                if (!done[j]) last_sweeps[j]++;
             for (int j = 0; j < nj; ++j)
                 if (!done[j] && !rot[j]) {
                     done[j] = 1;
@@ -607,6 +630,111 @@ struct wqc_tn_state {
         long* doff = upload(offs);
         wqc::launch_row_norms(dj, nj, max_n, d_sigma, doff, stream);
         check_launch();
+    }
+
+    // Mixed form of jacobi(): low-precision sweeps on complex64 rows with the rotation J
+    // kept unitary in FP64 (k_jacobi_round_lo), then X <- J X_orig recomputed in FP64 and
+    // polished by the FP64 rounds, which also produce the singular values.
+    void jacobi_mixed(std::vector<JacobiJob>& jobs, double* d_sigma, const std::vector<long>& sig_off,
+                      int* d_flags) {
+        const int nj = (int)jobs.size();
+        if (!nj) return;
+        std::vector<int> lo_count(nj, 0);
+        size_t off = 0;
+        std::vector<size_t> o32(nj), o64(nj);
+        for (int k = 0; k < nj; ++k) {
+            const size_t cnt = (size_t)jobs[k].n * jobs[k].m;
+            o32[k] = off;
+            off = (off + cnt * sizeof(float2) + 255) & ~(size_t)255;
+            o64[k] = off;
+            off = (off + cnt * sizeof(double2) + 255) & ~(size_t)255;
+        }
+        ws32.ensure(off);
+        note_peak();
+        char* b32 = (char*)ws32.p;
+        std::vector<const double2*> src(nj);
+        std::vector<float2*> dst(nj);
+        std::vector<long> counts(nj);
+        std::vector<wqc::JacobiLoJob> lj(nj);
+        long mx = 1;
+        int max_n = 1, max_m = 1;
+        for (int k = 0; k < nj; ++k) {
+            src[k] = jobs[k].X;
+            dst[k] = reinterpret_cast<float2*>(b32 + o32[k]);
+            counts[k] = (long)jobs[k].n * jobs[k].m;
+            mx = std::max(mx, counts[k]);
+            lj[k].X = dst[k];
+            lj[k].J = jobs[k].J;
+            lj[k].n = jobs[k].n;
+            lj[k].m = jobs[k].m;
+            lj[k].ldx = jobs[k].m;
+            lj[k].pad = 0;
+            max_n = std::max(max_n, jobs[k].n);
+            max_m = std::max(max_m, jobs[k].m);
+        }
+        wqc::launch_to_f32(upload(src), upload(dst), upload(counts), nj, mx, stream);
+        check_launch();
+        wqc::JacobiLoJob* dj = upload(lj);
+        const int nb = (max_n + 15) / 16;
+        const int nbe = nb + (nb & 1);
+        const int rounds = nbe - 1, pairs = nbe / 2;
+        const float tol = (float)(JACOBI_TOL_LO * std::sqrt((double)max_m));
+        const double nu2 = NULL_ROWS_SQ / max_n;
+        int* d_rot = d_flags;
+        int* d_done = d_flags + nj;
+        unsigned long long* d_jstats = jstats_dev();
+        wqc::launch_fill_int(d_done, 0, nj, stream);
+        std::vector<int> done(nj, 0);
+        int active = nj;
+        for (int sweep = 0; sweep < MAX_SWEEPS_LO && active > 0; ++sweep) {
+            check_cancel();
+            wqc::launch_fill_int(d_rot, 0, nj, stream);
+            {
+                Timer tm(this, 3);
+                for (int r = 0; r < rounds; ++r)
+                    wqc::launch_jacobi_round_lo(dj, nj, pairs, r, tol, nu2, inner_cap, d_rot, d_done, d_jstats, stream);
+                check_launch();
+                klaunch[3] += rounds;
+            }
+            stats.jacobi_sweeps_lo += active;
+            HIPC(hipMemcpyAsync(hread, d_rot, nj * sizeof(int), hipMemcpyDeviceToHost, stream));
+            sync();
+            const int* rot = reinterpret_cast<const int*>(hread);
+            bool changed = false;
+            for (int j = 0; j < nj; ++j)
+                if (!done[j]) lo_count[j]++;
+            for (int j = 0; j < nj; ++j)
+                if (!done[j] && !rot[j]) {
+                    done[j] = 1;
+                    active--;
+                    changed = true;
+                }
+            if (changed && active > 0)
+                HIPC(hipMemcpyAsync(d_done, upload_ints(done), nj * sizeof(int), hipMemcpyDeviceToDevice, stream));
+        }
+        // X <- J X in FP64 (through a scratch copy), then the FP64 rounds.
+        {
+            Timer tm(this, 2);
+            std::vector<GemmJob> g;
+            for (int k = 0; k < nj; ++k) {
+                double2* tmp = reinterpret_cast<double2*>(b32 + o64[k]);
+                HIPC(hipMemcpyAsync(tmp, jobs[k].X, (size_t)counts[k] * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+                g.push_back(gj(jobs[k].J, jobs[k].n, 0, tmp, jobs[k].m, 0, jobs[k].X, jobs[k].ldx, jobs[k].n,
+                               jobs[k].m, jobs[k].n, 1, 0));
+            }
+            gemm(g);
+        }
+        jacobi(jobs, d_sigma, sig_off, d_flags);
+        last_sweeps_lo = lo_count;
+    }
+
+    void jacobi_any(std::vector<JacobiJob>& jobs, double* d_sigma, const std::vector<long>& sig_off,
+                    int* d_flags) {
+        last_sweeps_lo.assign(jobs.size(), 0);
+        if (precision == WQC_TN_PRECISION_MIXED_FP32)
+            jacobi_mixed(jobs, d_sigma, sig_off, d_flags);
+        else
+            jacobi(jobs, d_sigma, sig_off, d_flags);
     }
 
     int* upload_ints(const std::vector<int>& v) { return upload(v); }
@@ -655,6 +783,7 @@ struct wqc_tn_state {
         OmegaEntry* om;
         size_t chi_kept;
         bool fallback;
+        int sweeps = 0, sweeps_lo = 0;
     };
 
     void run_units(std::vector<Unit>& units) {
@@ -757,8 +886,9 @@ struct wqc_tn_state {
             for (int i = 0; i < nu; ++i) {
                 const JobInfo& j = J[i];
                 const int path = !j.rsvd ? 0 : (j.fallback ? 2 : 1);
-                const int32_t e[6] = {j.rows, j.cols, j.bond, path, (int32_t)j.ell, (int32_t)j.chi_kept};
-                split_log.insert(split_log.end(), e, e + 6);
+                const int32_t e[8] = {j.rows, j.cols, j.bond, path, (int32_t)j.ell, (int32_t)j.chi_kept,
+                                      j.sweeps, j.sweeps_lo};
+                split_log.insert(split_log.end(), e, e + 8);
             }
     }
 
@@ -801,7 +931,11 @@ struct wqc_tn_state {
         wqc::launch_identity(upload(Jm), upload(dims), ne, max_n, stream);
         check_launch();
         // sigma offsets are relative to the workspace base viewed as doubles
-        jacobi(jj, PD(0), offs, d_flags);
+        jacobi_any(jj, PD(0), offs, d_flags);
+        for (int k = 0; k < ne; ++k) {
+            J[ids[k]].sweeps += last_sweeps[k];
+            J[ids[k]].sweeps_lo += last_sweeps_lo[k];
+        }
         // read back sigma and |theta|_F^2
         size_t tot = 0;
         for (int k = 0; k < ne; ++k) tot += jj[k].n;
@@ -994,7 +1128,11 @@ struct wqc_tn_state {
         dust(mats, counts, d_fro, d_part, nblk);
         wqc::launch_identity(upload(Jm), upload(dims), nr, max_n, stream);
         check_launch();
-        jacobi(jj, PD(0), offs, d_flags);
+        jacobi_any(jj, PD(0), offs, d_flags);
+        for (int k = 0; k < nr; ++k) {
+            J[ids[k]].sweeps += last_sweeps[k];
+            J[ids[k]].sweeps_lo += last_sweeps_lo[k];
+        }
         size_t cur = 0;
         std::vector<size_t> hoff(nr);
         for (int k = 0; k < nr; ++k) {
@@ -1712,10 +1850,10 @@ int wqc_tn_enable_split_log(wqc_tn_state* st, int enable) {
 
 size_t wqc_tn_split_log(const wqc_tn_state* st, int32_t* out, size_t cap_entries) {
     if (!st) return 0;
-    const size_t m = st->split_log.size() / 6;
+    const size_t m = st->split_log.size() / 8;
     if (out) {
         const size_t k = std::min(m, cap_entries);
-        std::memcpy(out, st->split_log.data(), k * 6 * sizeof(int32_t));
+        std::memcpy(out, st->split_log.data(), k * 8 * sizeof(int32_t));
     }
     return m;
 }

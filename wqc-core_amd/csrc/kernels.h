@@ -38,6 +38,14 @@ struct JacobiJob {
     int n, m, ldx, pad;
 };
 
+// Mixed-precision Jacobi job: rows X in complex64 (n x m, row stride ldx), rotation J
+// (n x n, complex128) accumulated so that J stays unitary to FP64 precision.
+struct JacobiLoJob {
+    float2* X;
+    double2* J;
+    int n, m, ldx, pad;
+};
+
 // Batched complex GEMM: C = alpha * op(A) * op(B) + beta * C, op in {N, C} (C = conjugate
 // transpose). M x N result, inner dimension K. Leading dimensions are row strides of the
 // stored matrices.
@@ -104,8 +112,13 @@ void launch_dust(double2* const* ptrs, const long* counts, const double* fro2, d
 void launch_identity(double2* const* ptrs, const int* dims, int njobs, int max_dim,
                      hipStream_t s);
 void launch_jacobi_round(const JacobiJob* jobs, int njobs, int max_pairs, int round,
-                         double tol, int* rotated, const int* done, unsigned long long* stats,
-                         hipStream_t s);
+                         double tol, double nu2, int inner_cap, int* rotated, const int* done,
+                         unsigned long long* stats, hipStream_t s);
+void launch_jacobi_round_lo(const JacobiLoJob* jobs, int njobs, int max_pairs, int round,
+                            float tol, double nu2, int inner_cap, int* rotated, const int* done,
+                            unsigned long long* stats, hipStream_t s);
+void launch_to_f32(const double2* const* src, float2* const* dst, const long* counts, int njobs,
+                   long max_count, hipStream_t s);
 void launch_row_norms(const JacobiJob* jobs, int njobs, int max_n, double* sigma,
                       const long* sigma_off, hipStream_t s);
 void launch_gemm(const GemmJob* jobs, int njobs, int max_m, int max_n, hipStream_t s);

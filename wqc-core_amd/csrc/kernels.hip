@@ -189,13 +189,49 @@ __global__ __launch_bounds__(256) void k_identity(double2* const* __restrict__ p
 // A pair (i, j) needs a rotation when |G_ij| > tol sqrt(G_ii G_jj) (relative, as the
 // reference's Jacobi, src/tn/svd_guard.rs:178): small singular values come out accurate
 // relative to themselves, so the rank count above 1e-12 and the dust rule see the same
-// values an exact SVD gives. The in-block eigensolve runs at most JINNER sweeps; the
+// values an exact SVD gives. Rows of squared norm <= nu2 (the host passes 1e-26/n, so
+// that all such rows together hold at most 1e-13 of singular value, an order below the
+// 1e-12 cutoff) take part in no rotation: they cannot move a kept value or a decision,
+// and orthogonalizing rounding noise against itself would never settle. The in-block eigensolve runs at most JINNER sweeps; the
 // next outer round recomputes the pair's Gram matrix from the rows themselves, so an
 // unfinished inner solve only costs another round. rotated[job] records that the sweep
 // is not converged; stats[0..1] count inner sweeps and pair solves.
 constexpr int JB = 16;
 constexpr int JP = 2 * JB;
-constexpr int JINNER = 6;
+
+// De Rijk-style ordering of a pair step: after the in-block eigensolve, the rotated rows are
+// ranked by their Gram diagonal (descending; ties by row index) and the k-th largest is
+// written to the k-th lowest row index of the pair. Every pair step then agrees on one
+// global order (large rows first), which removes the slow first phase of block Jacobi with
+// unordered eigenvectors (about half the sweeps on decaying spectra). Slots beyond the
+// matrix (s_row < 0) keep their place. Threads 0..JP-1 write dst[k], the destination slot
+// of local row k.
+template <typename Key>
+__device__ __forceinline__ void pair_order(int tid, const int* s_row, const Key* diag, int stride,
+                                           int* dst) {
+    if (tid >= JP) return;
+    const int rk = s_row[tid];
+    int out = tid;
+    if (rk >= 0) {
+        const Key key = diag[tid * stride];
+        int rank = 0;
+        for (int j = 0; j < JP; ++j) {
+            const int rj = s_row[j];
+            if (j == tid || rj < 0) continue;
+            const Key kj = diag[j * stride];
+            if (kj > key || (kj == key && rj < rk)) ++rank;
+        }
+        for (int j = 0; j < JP; ++j) {
+            const int rj = s_row[j];
+            if (rj < 0) continue;
+            int pos = 0;
+            for (int i = 0; i < JP; ++i)
+                if (s_row[i] >= 0 && s_row[i] < rj) ++pos;
+            if (pos == rank) out = j;
+        }
+    }
+    dst[tid] = out;
+}
 
 typedef double d4v __attribute__((ext_vector_type(4)));
 
@@ -204,8 +240,8 @@ __device__ __forceinline__ d4v mfma_f64(double a, double b, d4v c) {
 }
 
 __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restrict__ jobs,
-                                                      int round, double tol,
-                                                      int* __restrict__ rotated,
+                                                      int round, double tol, double nu2,
+                                                      int inner_cap, int* __restrict__ rotated,
                                                       const int* __restrict__ done,
                                                       unsigned long long* __restrict__ stats) {
     const int job = blockIdx.y;
@@ -233,6 +269,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
     __shared__ double rcs[16], rtg[16], ra[16], rd[16];
     __shared__ double2 rm01[16], rm10[16];
     __shared__ int s_any;
+    __shared__ int s_dst[JP];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid < JP) {
@@ -322,7 +359,8 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
             if (i < j) {
                 const double2 c = G[i][j];
                 const double gg = hypot(c.x, c.y);
-                if (gg > 0.0 && gg > tol * sqrt(fmax(G[i][i].x, 0.0) * fmax(G[j][j].x, 0.0))) off = true;
+                const double di = G[i][i].x, dj = G[j][j].x;
+                if (gg > 0.0 && fmin(di, dj) > nu2 && gg > tol * sqrt(fmax(di, 0.0) * fmax(dj, 0.0))) off = true;
             }
         }
         if (off) s_any = 1;
@@ -346,7 +384,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
     const double tol_in = tol;
     int sweeps_done = 0;
     const int bk = tid >> 4, bk2 = tid & 15;
-    for (int sweep = 0; sweep < JINNER; ++sweep) {
+    for (int sweep = 0; sweep < inner_cap; ++sweep) {
         ++sweeps_done;
         for (int step = 0; step < JP - 1; ++step) {
             if (tid < 16) {
@@ -361,7 +399,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
                 const double a = G[p][p].x, d = G[q][q].x;
                 const double2 c = G[p][q];
                 const double gg = hypot(c.x, c.y);
-                const int act = gg > 0.0 && gg > tol_in * sqrt(fmax(a, 0.0) * fmax(d, 0.0));
+                const int act = gg > 0.0 && fmin(a, d) > nu2 && gg > tol_in * sqrt(fmax(a, 0.0) * fmax(d, 0.0));
                 rp[tid] = p;
                 rq[tid] = q;
                 ract[tid] = act;
@@ -457,6 +495,15 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
         atomicAdd(&stats[1], 1ull);
     }
 
+    // Ordered rotation: row k of T moves to row dst[k]; G is free and holds the result.
+    pair_order(tid, s_row, &G[0][0].x, 2 * (JP + 2), s_dst);
+    __syncthreads();
+    for (int e = tid; e < JP * JP; e += 256) {
+        const int i = e / JP, j = e % JP;
+        G[s_dst[i]][j] = T[i][j];
+    }
+    __syncthreads();
+
     // ---- 3. P <- T P on MFMA. Lane l supplies T[I*16 + l%16][4kk + l/16] (registers) and
     // P[4kk + l/16][c0 + l%16]; output element j of lane l is local row I*16 + 4j + l/16.
     {
@@ -466,7 +513,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
         for (int I = 0; I < 2; ++I)
 #pragma unroll
             for (int kk = 0; kk < 8; ++kk) {
-                const double2 t = T[I * 16 + i][4 * kk + g];
+                const double2 t = G[I * 16 + i][4 * kk + g];
                 tr[I][kk] = t.x;
                 ti[I][kk] = t.y;
                 tni[I][kk] = -t.y;
@@ -511,6 +558,429 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
                 }
             }
         }
+    }
+}
+
+// ------------------------------------------------------------------ mixed-precision round
+// Low-precision phase of the mixed split: the rows live in complex64 (X32), the
+// accumulated rotation J in complex128. Per block pair: Gram on FP32 MFMA
+// (v_mfma_f32_16x16x4f32), eigensolve of the 32x32 Gram matrix in FP32, then the rotation
+// T is lifted to FP64 and made unitary to FP64 precision by two Newton-Schulz steps
+// T <- T (3I - T^H T) / 2, applied to X32 in FP32 and to J in FP64. J is therefore unitary
+// to FP64 precision whatever the FP32 rounding, and X32 tracks J X to FP32 precision; the
+// FP64 finish recomputes J X exactly and polishes it with the FP64 rounds.
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f4v mfma_f32(float a, float b, f4v c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+__global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __restrict__ jobs,
+                                                         int round, float tol, double nu2,
+                                                         int inner_cap, int* __restrict__ rotated,
+                                                         const int* __restrict__ done,
+                                                         unsigned long long* __restrict__ stats) {
+    const int job = blockIdx.y;
+    if (done[job]) return;
+    const JacobiLoJob jb = jobs[job];
+    const int nb = (jb.n + JB - 1) / JB;
+    const int nbe = nb + (nb & 1);
+    if (round >= nbe - 1) return;
+    const int kpair = blockIdx.x;
+    if (kpair >= nbe / 2) return;
+    int bi, bj;
+    if (kpair == 0) {
+        bi = nbe - 1;
+        bj = round;
+    } else {
+        bi = (round + kpair) % (nbe - 1);
+        bj = (round - kpair + nbe - 1) % (nbe - 1);
+    }
+
+    __shared__ float2 G[JP][JP + 1];
+    __shared__ float2 T[JP][JP + 1];
+    __shared__ double2 T64[JP][JP + 1];
+    __shared__ double2 W64[JP][JP + 1];
+    __shared__ float Gp[2][6][4][64];
+    __shared__ int s_row[JP];
+    __shared__ int rp[16], rq[16], ract[16];
+    __shared__ float rcs[16], rtg[16], ra[16], rd[16];
+    __shared__ float2 rm01[16], rm10[16];
+    __shared__ int s_any;
+    __shared__ float s_low;
+    __shared__ int s_dst[JP];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid < JP) {
+        const int b = tid < JB ? bi : bj;
+        const int r = b * JB + (tid & (JB - 1));
+        s_row[tid] = (b < nb && r < jb.n) ? r : -1;
+    }
+    if (tid == 0) s_any = 0;
+    __syncthreads();
+
+    {
+        const int i = lane & 15, g = lane >> 4;
+        const int r0 = s_row[i], r1 = s_row[16 + i];
+        const float2* x0 = r0 >= 0 ? jb.X + (long)r0 * jb.ldx : nullptr;
+        const float2* x1 = r1 >= 0 ? jb.X + (long)r1 * jb.ldx : nullptr;
+        f4v a00r = {0, 0, 0, 0}, a00i = a00r, a01r = a00r, a01i = a00r, a11r = a00r, a11i = a00r;
+        const float2 zf = make_float2(0.f, 0.f);
+        for (int c0 = wave * 16; c0 < jb.m; c0 += 64) {
+            float2 u[4], v[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int c = c0 + 4 * g + s;
+                u[s] = (x0 && c < jb.m) ? x0[c] : zf;
+                v[s] = (x1 && c < jb.m) ? x1[c] : zf;
+            }
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                a00r = mfma_f32(u[s].x, u[s].x, a00r);
+                a00r = mfma_f32(u[s].y, u[s].y, a00r);
+                a00i = mfma_f32(u[s].y, u[s].x, a00i);
+                a00i = mfma_f32(-u[s].x, u[s].y, a00i);
+                a01r = mfma_f32(u[s].x, v[s].x, a01r);
+                a01r = mfma_f32(u[s].y, v[s].y, a01r);
+                a01i = mfma_f32(u[s].y, v[s].x, a01i);
+                a01i = mfma_f32(-u[s].x, v[s].y, a01i);
+                a11r = mfma_f32(v[s].x, v[s].x, a11r);
+                a11r = mfma_f32(v[s].y, v[s].y, a11r);
+                a11i = mfma_f32(v[s].y, v[s].x, a11i);
+                a11i = mfma_f32(-v[s].x, v[s].y, a11i);
+            }
+        }
+#define WQC_GP_ALL(OP) OP(0, a00r) OP(1, a00i) OP(2, a01r) OP(3, a01i) OP(4, a11r) OP(5, a11i)
+#define WQC_GP_STORE(t, v) for (int j = 0; j < 4; ++j) Gp[slot][t][j][lane] = v[j];
+#define WQC_GP_ADD(t, v) for (int j = 0; j < 4; ++j) v[j] += Gp[slot][t][j][lane];
+        if (wave >= 2) {
+            const int slot = wave - 2;
+            WQC_GP_ALL(WQC_GP_STORE)
+        }
+        __syncthreads();
+        if (wave < 2) {
+            const int slot = wave;
+            WQC_GP_ALL(WQC_GP_ADD)
+        }
+        __syncthreads();
+        if (wave == 1) {
+            const int slot = 0;
+            WQC_GP_ALL(WQC_GP_STORE)
+        }
+        __syncthreads();
+#undef WQC_GP_ALL
+#undef WQC_GP_STORE
+#undef WQC_GP_ADD
+        if (wave == 0) {
+            // MFMA f32 output: element j of lane l is row 4*(l/16) + j, column l%16.
+            const int c = lane & 15;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int rr = 4 * (lane >> 4) + j;
+                const float g00r = a00r[j] + Gp[0][0][j][lane], g00i = a00i[j] + Gp[0][1][j][lane];
+                const float g01r = a01r[j] + Gp[0][2][j][lane], g01i = a01i[j] + Gp[0][3][j][lane];
+                const float g11r = a11r[j] + Gp[0][4][j][lane], g11i = a11i[j] + Gp[0][5][j][lane];
+                G[rr][c] = make_float2(g00r, g00i);
+                G[rr][16 + c] = make_float2(g01r, g01i);
+                G[16 + c][rr] = make_float2(g01r, -g01i);
+                G[16 + rr][16 + c] = make_float2(g11r, g11i);
+            }
+        }
+    }
+    __syncthreads();
+    // Rows below FP32 resolution of the pair's largest row are left to the FP64 finish.
+    if (tid == 0) {
+        float mx = 0.f;
+        for (int i = 0; i < JP; ++i) mx = fmaxf(mx, G[i][i].x);
+        s_low = fmaxf((float)nu2, mx * 1e-12f);
+    }
+    __syncthreads();
+    const float low = s_low;
+    {
+        bool off = false;
+        for (int e = tid; e < JP * JP; e += 256) {
+            const int i = e / JP, j = e % JP;
+            if (i < j) {
+                const float2 c = G[i][j];
+                const float gg = hypotf(c.x, c.y);
+                const float di = G[i][i].x, dj = G[j][j].x;
+                if (gg > 0.f && fminf(di, dj) > low && gg > tol * sqrtf(fmaxf(di, 0.f) * fmaxf(dj, 0.f))) off = true;
+            }
+        }
+        if (off) s_any = 1;
+    }
+    __syncthreads();
+    const int need = s_any;
+    __syncthreads();
+    if (!need) return;
+    if (tid == 0) {
+        rotated[job] = 1;
+        s_any = 0;
+    }
+    for (int e = tid; e < JP * JP; e += 256) {
+        const int i = e / JP, j = e % JP;
+        T[i][j] = make_float2(i == j ? 1.f : 0.f, 0.f);
+    }
+    __syncthreads();
+
+    int sweeps_done = 0;
+    const int bk = tid >> 4, bk2 = tid & 15;
+    for (int sweep = 0; sweep < inner_cap; ++sweep) {
+        ++sweeps_done;
+        for (int step = 0; step < JP - 1; ++step) {
+            if (tid < 16) {
+                int p, q;
+                if (tid == 0) {
+                    p = JP - 1;
+                    q = step;
+                } else {
+                    p = (step + tid) % (JP - 1);
+                    q = (step - tid + JP - 1) % (JP - 1);
+                }
+                const float a = G[p][p].x, d = G[q][q].x;
+                const float2 c = G[p][q];
+                const float gg = hypotf(c.x, c.y);
+                const int act = gg > 0.f && fminf(a, d) > low && gg > tol * sqrtf(fmaxf(a, 0.f) * fmaxf(d, 0.f));
+                rp[tid] = p;
+                rq[tid] = q;
+                ract[tid] = act;
+                if (act) {
+                    const float zeta = (d - a) / (2.f * gg);
+                    const float t = copysignf(1.f, zeta) / (fabsf(zeta) + sqrtf(1.f + zeta * zeta));
+                    const float cs = 1.f / sqrtf(1.f + t * t);
+                    const float sn = cs * t;
+                    const float2 ph = make_float2(c.x / gg, c.y / gg);
+                    rcs[tid] = cs;
+                    rm01[tid] = make_float2(-ph.x * sn, -ph.y * sn);
+                    rm10[tid] = make_float2(ph.x * sn, -ph.y * sn);
+                    rtg[tid] = t * gg;
+                    ra[tid] = a;
+                    rd[tid] = d;
+                    s_any = 1;
+                }
+            }
+            __syncthreads();
+            const int act1 = ract[bk], act2 = ract[bk2];
+            const int p1 = rp[bk], q1 = rq[bk], p2 = rp[bk2], q2 = rq[bk2];
+            float2 n00, n01, n10, n11, t00, t01, t10, t11;
+            if (act1 | act2) {
+                float2 h00 = G[p1][p2], h01 = G[p1][q2], h10 = G[q1][p2], h11 = G[q1][q2];
+                t00 = T[p1][p2];
+                t01 = T[p1][q2];
+                t10 = T[q1][p2];
+                t11 = T[q1][q2];
+                auto fm = [](float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
+                auto fa = [](float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); };
+                auto fs = [](float2 a, float k) { return make_float2(a.x * k, a.y * k); };
+                auto fc = [](float2 a) { return make_float2(a.x, -a.y); };
+                if (act1) {
+                    const float cs = rcs[bk];
+                    const float2 m01 = rm01[bk], m10 = rm10[bk];
+                    float2 x00 = fa(fs(h00, cs), fm(m01, h10)), x01 = fa(fs(h01, cs), fm(m01, h11));
+                    float2 x10 = fa(fm(m10, h00), fs(h10, cs)), x11 = fa(fm(m10, h01), fs(h11, cs));
+                    h00 = x00;
+                    h01 = x01;
+                    h10 = x10;
+                    h11 = x11;
+                    float2 y00 = fa(fs(t00, cs), fm(m01, t10)), y01 = fa(fs(t01, cs), fm(m01, t11));
+                    float2 y10 = fa(fm(m10, t00), fs(t10, cs)), y11 = fa(fm(m10, t01), fs(t11, cs));
+                    t00 = y00;
+                    t01 = y01;
+                    t10 = y10;
+                    t11 = y11;
+                }
+                if (act2) {
+                    const float cs = rcs[bk2];
+                    const float2 c01 = fc(rm01[bk2]), c10 = fc(rm10[bk2]);
+                    n00 = fa(fs(h00, cs), fm(h01, c01));
+                    n01 = fa(fm(h00, c10), fs(h01, cs));
+                    n10 = fa(fs(h10, cs), fm(h11, c01));
+                    n11 = fa(fm(h10, c10), fs(h11, cs));
+                } else {
+                    n00 = h00;
+                    n01 = h01;
+                    n10 = h10;
+                    n11 = h11;
+                }
+                if (bk == bk2 && act1) {
+                    n00 = make_float2(ra[bk] - rtg[bk], 0.f);
+                    n11 = make_float2(rd[bk] + rtg[bk], 0.f);
+                    n01 = make_float2(0.f, 0.f);
+                    n10 = n01;
+                }
+            }
+            __syncthreads();
+            if (act1 | act2) {
+                G[p1][p2] = n00;
+                G[p1][q2] = n01;
+                G[q1][p2] = n10;
+                G[q1][q2] = n11;
+                if (act1) {
+                    T[p1][p2] = t00;
+                    T[p1][q2] = t01;
+                    T[q1][p2] = t10;
+                    T[q1][q2] = t11;
+                }
+            }
+            __syncthreads();
+        }
+        const int again = s_any;
+        __syncthreads();
+        if (!again) break;
+        if (tid == 0) s_any = 0;
+        __syncthreads();
+    }
+    if (tid == 0 && stats) {
+        atomicAdd(&stats[0], (unsigned long long)sweeps_done);
+        atomicAdd(&stats[1], 1ull);
+    }
+
+    // T -> FP64, two Newton-Schulz steps T <- T (3I - T^H T) / 2 (unitary to FP64 precision).
+    for (int e = tid; e < JP * JP; e += 256) {
+        const int i = e / JP, j = e % JP;
+        T64[i][j] = make_double2((double)T[i][j].x, (double)T[i][j].y);
+    }
+    __syncthreads();
+    for (int it = 0; it < 2; ++it) {
+        for (int e = tid; e < JP * JP; e += 256) {
+            const int i = e / JP, j = e % JP;
+            double2 acc = cz();
+            for (int k = 0; k < JP; ++k) ccmac(acc, T64[k][i], T64[k][j]);
+            W64[i][j] = make_double2((i == j ? 1.5 : 0.0) - 0.5 * acc.x, -0.5 * acc.y);
+        }
+        __syncthreads();
+        double2 out[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u, i = e / JP, j = e % JP;
+            double2 acc = cz();
+            for (int k = 0; k < JP; ++k) cmac(acc, T64[i][k], W64[k][j]);
+            out[u] = acc;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u, i = e / JP, j = e % JP;
+            T64[i][j] = out[u];
+        }
+        __syncthreads();
+    }
+
+    // Ordered rotation (as in the FP64 round): row k of T moves to row dst[k], into W64.
+    pair_order(tid, s_row, &G[0][0].x, 2 * (JP + 2), s_dst);
+    __syncthreads();
+    for (int e = tid; e < JP * JP; e += 256) {
+        const int i = e / JP, j = e % JP;
+        W64[s_dst[i]][j] = T64[i][j];
+    }
+    __syncthreads();
+
+    // X32 <- T X32 (FP32 MFMA), J <- T J (FP64 MFMA), T in W64.
+    {
+        const int i = lane & 15, g = lane >> 4;
+        int prow[8];
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) prow[kk] = s_row[4 * kk + g];
+        {
+            float tr[2][8], ti[2][8], tni[2][8];
+#pragma unroll
+            for (int I = 0; I < 2; ++I)
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk) {
+                    const double2 t = W64[I * 16 + i][4 * kk + g];
+                    tr[I][kk] = (float)t.x;
+                    ti[I][kk] = (float)t.y;
+                    tni[I][kk] = -(float)t.y;
+                }
+            int orow[2][4];
+#pragma unroll
+            for (int I = 0; I < 2; ++I)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) orow[I][j] = s_row[I * 16 + 4 * g + j];
+            for (int c0 = wave * 16; c0 < jb.m; c0 += 64) {
+                const int c = c0 + i;
+                const bool cok = c < jb.m;
+                float2 b[8];
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk)
+                    b[kk] = (cok && prow[kk] >= 0) ? jb.X[(long)prow[kk] * jb.ldx + c] : make_float2(0.f, 0.f);
+#pragma unroll
+                for (int I = 0; I < 2; ++I) {
+                    f4v dr = {0, 0, 0, 0}, di = dr;
+#pragma unroll
+                    for (int kk = 0; kk < 8; ++kk) {
+                        dr = mfma_f32(tr[I][kk], b[kk].x, dr);
+                        dr = mfma_f32(tni[I][kk], b[kk].y, dr);
+                        di = mfma_f32(tr[I][kk], b[kk].y, di);
+                        di = mfma_f32(ti[I][kk], b[kk].x, di);
+                    }
+                    if (cok) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int gr = orow[I][j];
+                            if (gr >= 0) jb.X[(long)gr * jb.ldx + c] = make_float2(dr[j], di[j]);
+                        }
+                    }
+                }
+            }
+        }
+        {
+            double tr[2][8], ti[2][8], tni[2][8];
+#pragma unroll
+            for (int I = 0; I < 2; ++I)
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk) {
+                    const double2 t = W64[I * 16 + i][4 * kk + g];
+                    tr[I][kk] = t.x;
+                    ti[I][kk] = t.y;
+                    tni[I][kk] = -t.y;
+                }
+            int orow[2][4];
+#pragma unroll
+            for (int I = 0; I < 2; ++I)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) orow[I][j] = s_row[I * 16 + 4 * j + g];
+            for (int c0 = wave * 16; c0 < jb.n; c0 += 64) {
+                const int c = c0 + i;
+                const bool cok = c < jb.n;
+                double2 b[8];
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk)
+                    b[kk] = (cok && prow[kk] >= 0) ? jb.J[(long)prow[kk] * jb.n + c] : cz();
+#pragma unroll
+                for (int I = 0; I < 2; ++I) {
+                    d4v dr = {0, 0, 0, 0}, di = dr;
+#pragma unroll
+                    for (int kk = 0; kk < 8; ++kk) {
+                        dr = mfma_f64(tr[I][kk], b[kk].x, dr);
+                        dr = mfma_f64(tni[I][kk], b[kk].y, dr);
+                        di = mfma_f64(tr[I][kk], b[kk].y, di);
+                        di = mfma_f64(ti[I][kk], b[kk].x, di);
+                    }
+                    if (cok) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int gr = orow[I][j];
+                            if (gr >= 0) jb.J[(long)gr * jb.n + c] = make_double2(dr[j], di[j]);
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+// X32 = (complex64) X
+__global__ __launch_bounds__(256) void k_to_f32(const double2* const* __restrict__ src,
+                                                float2* const* __restrict__ dst,
+                                                const long* __restrict__ counts) {
+    const int job = blockIdx.y;
+    const double2* s = src[job];
+    float2* d = dst[job];
+    const long n = counts[job];
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+        const double2 v = s[e];
+        d[e] = make_float2((float)v.x, (float)v.y);
     }
 }
 
@@ -845,9 +1315,23 @@ void launch_identity(double2* const* ptrs, const int* dims, int njobs, int max_d
 }
 
 void launch_jacobi_round(const JacobiJob* jobs, int njobs, int max_pairs, int round, double tol,
-                         int* rotated, const int* done, unsigned long long* stats, hipStream_t s) {
+                         double nu2, int inner_cap, int* rotated, const int* done, unsigned long long* stats,
+                         hipStream_t s) {
     hipLaunchKernelGGL(k_jacobi_round, dim3(max_pairs, njobs), dim3(256), 0, s, jobs, round, tol,
-                       rotated, done, stats);
+                       nu2, inner_cap, rotated, done, stats);
+}
+
+void launch_jacobi_round_lo(const JacobiLoJob* jobs, int njobs, int max_pairs, int round, float tol,
+                            double nu2, int inner_cap, int* rotated, const int* done, unsigned long long* stats,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_jacobi_round_lo, dim3(max_pairs, njobs), dim3(256), 0, s, jobs, round, tol,
+                       nu2, inner_cap, rotated, done, stats);
+}
+
+void launch_to_f32(const double2* const* src, float2* const* dst, const long* counts, int njobs,
+                   long max_count, hipStream_t s) {
+    hipLaunchKernelGGL(k_to_f32, dim3(blocks_for(max_count, 256, 2048), njobs), dim3(256), 0, s, src,
+                       dst, counts);
 }
 
 void launch_row_norms(const JacobiJob* jobs, int njobs, int max_n, double* sigma,

@@ -45,7 +45,7 @@ class Stats(ct.Structure):
                 ("jacobi_sweeps", ct.c_uint64), ("truncated_splits", ct.c_uint64),
                 ("peak_vram_bytes", ct.c_uint64), ("max_discarded_weight", ct.c_double),
                 ("sum_discarded_weight", ct.c_double), ("jacobi_inner_sweeps", ct.c_uint64),
-                ("jacobi_pair_solves", ct.c_uint64)]
+                ("jacobi_pair_solves", ct.c_uint64), ("jacobi_sweeps_lo", ct.c_uint64)]
 
 
 class WqcTnError(RuntimeError):
@@ -288,11 +288,12 @@ class MpsState:
         _check(load().wqc_tn_enable_split_log(self._h, 1 if on else 0))
 
     def split_log(self):
-        """[(rows, cols, bond, path, ell, chi_kept)] per two-qubit unit since enabling."""
+        """[(rows, cols, bond, path, ell, chi_kept, fp64_sweeps, low_sweeps)] per two-qubit
+        unit since enabling."""
         m = load().wqc_tn_split_log(self._h, None, 0)
-        buf = (ct.c_int32 * max(6, 6 * m))()
+        buf = (ct.c_int32 * max(8, 8 * m))()
         load().wqc_tn_split_log(self._h, buf, m)
-        a = np.frombuffer(buf, dtype=np.int32)[: 6 * m].reshape(m, 6)
+        a = np.frombuffer(buf, dtype=np.int32)[: 8 * m].reshape(m, 8)
         return [tuple(int(x) for x in row) for row in a]
 
 
