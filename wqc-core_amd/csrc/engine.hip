@@ -415,6 +415,7 @@ struct wqc_tn_state {
         ws32.release();
         tab.release();
         jstats_buf.release();
+        trace_buf.release();
         for (auto& sp : spans) {
             (void)hipEventDestroy(sp.a);
             (void)hipEventDestroy(sp.b);
@@ -478,6 +479,8 @@ struct wqc_tn_state {
     // ------------------------------------------------------------ lifetime
     // Sweeps of the in-block eigensolve per round (WQC_TN_JACOBI_INNER overrides, 1..32).
     int inner_cap = 2;
+    bool trace_jacobi = false;
+    DevBuf trace_buf;
 
     void init(wqc_tn_device* d, size_t qn, size_t mb, int prec) {
         dev = d;
@@ -485,6 +488,7 @@ struct wqc_tn_state {
             const int v = std::atoi(e);
             if (v >= 1 && v <= 32) inner_cap = v;
         }
+        if (const char* e = std::getenv("WQC_TN_JACOBI_TRACE")) trace_jacobi = std::atoi(e) != 0;
         n = (int)qn;
         chi = mb;
         precision = prec;
@@ -600,17 +604,41 @@ struct wqc_tn_state {
         wqc::launch_fill_int(d_done, 0, nj, stream);
         std::vector<int> done(nj, 0);
         int active = nj;
+        // WQC_TN_JACOBI_TRACE=1 prints, per sweep, the largest relative off-diagonal
+        // |G_ij| / sqrt(G_ii G_jj) seen before rotation in each of the widest jobs.
+        unsigned long long* d_off = nullptr;
+        if (trace_jacobi) {
+            trace_buf.ensure((size_t)nj * sizeof(unsigned long long));
+            d_off = reinterpret_cast<unsigned long long*>(trace_buf.p);
+        }
         for (int sweep = 0; sweep < MAX_SWEEPS && active > 0; ++sweep) {
             check_cancel();
             wqc::launch_fill_int(d_rot, 0, nj, stream);
+            if (d_off) wqc::launch_fill_int(reinterpret_cast<int*>(d_off), 0, 2 * nj, stream);
             {
                 Timer tm(this, 3);
                 for (int r = 0; r < rounds; ++r)
-                    wqc::launch_jacobi_round(dj, nj, pairs, r, tol, nu2, inner_cap, d_rot, d_done, d_jstats, stream);
+                    wqc::launch_jacobi_round(dj, nj, pairs, r, tol, nu2, inner_cap, d_rot, d_done, d_jstats,
+                                             d_off, stream);
                 check_launch();
                 klaunch[3] += rounds;
             }
             stats.jacobi_sweeps += active;
+            if (d_off) {
+                std::vector<unsigned long long> off(nj);
+                HIPC(hipMemcpyAsync(off.data(), d_off, nj * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+                sync();
+                std::fprintf(stderr, "jacobi sweep %d active %d:", sweep + 1, active);
+                int shown = 0;
+                for (int j = 0; j < nj && shown < 4; ++j) {
+                    if (done[j] || jobs[j].n < max_n) continue;
+                    double v;
+                    std::memcpy(&v, &off[j], sizeof v);
+                    std::fprintf(stderr, " [%dx%d] %.2e", jobs[j].n, jobs[j].m, v);
+                    ++shown;
+                }
+                std::fprintf(stderr, "\n");
+            }
             HIPC(hipMemcpyAsync(hread, d_rot, nj * sizeof(int), hipMemcpyDeviceToHost, stream));
             sync();
             const int* rot = reinterpret_cast<const int*>(hread);

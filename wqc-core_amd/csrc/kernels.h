@@ -113,7 +113,7 @@ void launch_identity(double2* const* ptrs, const int* dims, int njobs, int max_d
                      hipStream_t s);
 void launch_jacobi_round(const JacobiJob* jobs, int njobs, int max_pairs, int round,
                          double tol, double nu2, int inner_cap, int* rotated, const int* done,
-                         unsigned long long* stats, hipStream_t s);
+                         unsigned long long* stats, unsigned long long* offmax, hipStream_t s);
 void launch_jacobi_round_lo(const JacobiLoJob* jobs, int njobs, int max_pairs, int round,
                             float tol, double nu2, int inner_cap, int* rotated, const int* done,
                             unsigned long long* stats, hipStream_t s);

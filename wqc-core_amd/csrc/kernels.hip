@@ -235,6 +235,71 @@ __device__ __forceinline__ void pair_order(int tid, const int* s_row, const Key*
 
 typedef double d4v __attribute__((ext_vector_type(4)));
 
+// Reciprocal and reciprocal square root from the hardware estimates plus two Newton steps
+// (full FP64 precision), a third of the dependent instructions of IEEE division and sqrt.
+__device__ __forceinline__ double rcp_nr(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
+__device__ __forceinline__ double rsq_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    double e = fma(-h * y, y, 0.5);
+    y = fma(y, e, y);
+    e = fma(-h * y, y, 0.5);
+    return fma(y, e, y);
+}
+
+// Rotation pair k of step `step` of the circle method on JP indices.
+__device__ __forceinline__ void circle_pair(int step, int k, int& p, int& q) {
+    if (k == 0) {
+        p = JP - 1;
+        q = step;
+    } else {
+        p = step + k;
+        if (p >= JP - 1) p -= JP - 1;
+        q = step - k;
+        if (q < 0) q += JP - 1;
+    }
+}
+
+// Complex 2x2 Jacobi rotation M = [[cs, m01], [m10, cs]] that zeroes G_pq of the Hermitian
+// block [[a, c], [conj(c), d]]: M G M^H = diag(a - tg, d + tg). act = 0 when the pair is
+// orthogonal to tol (relative) or one of its rows is below nu2.
+struct Rot64 {
+    int act;
+    double cs, tg;
+    double2 m01, m10;
+};
+
+__device__ __forceinline__ Rot64 rot_params64(double a, double d, double2 c, double tol, double nu2) {
+    Rot64 r;
+    r.act = 0;
+    r.cs = 1.0;
+    r.tg = 0.0;
+    r.m01 = cz();
+    r.m10 = cz();
+    const double g2 = fma(c.x, c.x, c.y * c.y);
+    if (!(g2 > 0.0) || !(fmin(a, d) > nu2) || !(g2 > tol * tol * (a * d))) return r;
+    r.act = 1;
+    const double rg = rsq_nr(g2);
+    const double zeta = 0.5 * (d - a) * rg;
+    const double w = fma(zeta, zeta, 1.0);
+    const double t = copysign(rcp_nr(fabs(zeta) + w * rsq_nr(w)), zeta);
+    const double cs = rsq_nr(fma(t, t, 1.0));
+    const double sn = cs * t;
+    const double phx = c.x * rg, phy = c.y * rg;
+    r.cs = cs;
+    r.m01 = make_double2(-phx * sn, -phy * sn);
+    r.m10 = make_double2(phx * sn, -phy * sn);
+    r.tg = t * (g2 * rg);
+    return r;
+}
+
 __device__ __forceinline__ d4v mfma_f64(double a, double b, d4v c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
@@ -243,7 +308,8 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
                                                       int round, double tol, double nu2,
                                                       int inner_cap, int* __restrict__ rotated,
                                                       const int* __restrict__ done,
-                                                      unsigned long long* __restrict__ stats) {
+                                                      unsigned long long* __restrict__ stats,
+                                                      unsigned long long* __restrict__ offmax) {
     const int job = blockIdx.y;
     if (done[job]) return;
     const JacobiJob jb = jobs[job];
@@ -263,13 +329,15 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
 
     __shared__ double2 G[JP][JP + 1];
     __shared__ double2 T[JP][JP + 1];
-    __shared__ double Gp[2][6][4][64];  // partial Gram tiles of two waves
+    // Partial Gram tiles of two waves; afterwards the second G and T buffers of the eigensolve.
+    __shared__ __align__(16) double s_aux[4 * JP * (JP + 1)];
     __shared__ int s_row[JP];
-    __shared__ int rp[16], rq[16], ract[16];
-    __shared__ double rcs[16], rtg[16], ra[16], rd[16];
-    __shared__ double2 rm01[16], rm10[16];
     __shared__ int s_any;
     __shared__ int s_dst[JP];
+    __shared__ unsigned long long s_off;
+    double (*Gp)[6][4][64] = reinterpret_cast<double (*)[6][4][64]>(s_aux);
+    double2 (*G2)[JP + 1] = reinterpret_cast<double2 (*)[JP + 1]>(s_aux);
+    double2 (*T2)[JP + 1] = reinterpret_cast<double2 (*)[JP + 1]>(s_aux + 2 * JP * (JP + 1));
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid < JP) {
@@ -277,7 +345,10 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
         const int r = b * JB + (tid & (JB - 1));
         s_row[tid] = (b < nb && r < jb.n) ? r : -1;
     }
-    if (tid == 0) s_any = 0;
+    if (tid == 0) {
+        s_any = 0;
+        s_off = 0;
+    }
     __syncthreads();
 
     // ---- 1. Gram on MFMA. Lane (i = lane%16, g = lane/16) holds columns c0+4g..c0+4g+3 of
@@ -354,18 +425,25 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
     __syncthreads();
     {
         bool off = false;
+        double worst = 0.0;
         for (int e = tid; e < JP * JP; e += 256) {
             const int i = e / JP, j = e % JP;
             if (i < j) {
                 const double2 c = G[i][j];
-                const double gg = hypot(c.x, c.y);
+                const double g2 = fma(c.x, c.x, c.y * c.y);
                 const double di = G[i][i].x, dj = G[j][j].x;
-                if (gg > 0.0 && fmin(di, dj) > nu2 && gg > tol * sqrt(fmax(di, 0.0) * fmax(dj, 0.0))) off = true;
+                if (g2 > 0.0 && fmin(di, dj) > nu2) {
+                    if (g2 > tol * tol * (di * dj)) off = true;
+                    if (offmax) worst = fmax(worst, sqrt(g2 / (di * dj)));
+                }
             }
         }
         if (off) s_any = 1;
+        // Non-negative doubles order as their bit patterns.
+        if (offmax && worst > 0.0) atomicMax(&s_off, (unsigned long long)__double_as_longlong(worst));
     }
     __syncthreads();
+    if (offmax && tid == 0 && s_off) atomicMax(&offmax[job], s_off);
     // Every wave reads the flag before thread 0 clears it.
     const int need = s_any;
     __syncthreads();
@@ -380,108 +458,80 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
     }
     __syncthreads();
 
-    // ---- 2. eigensolve of G (T accumulates the row rotations).
-    const double tol_in = tol;
+    // ---- 2. eigensolve of G (T accumulates the row rotations). Every thread derives the
+    // parameters of its own two rotations from the current buffers, so a step is one
+    // read-update-write pass with G and T ping-ponged between two LDS buffers and a single
+    // barrier.
+    double2 (*Gc)[JP + 1] = G;
+    double2 (*Tc)[JP + 1] = T;
+    double2 (*Gn)[JP + 1] = G2;
+    double2 (*Tn)[JP + 1] = T2;
     int sweeps_done = 0;
     const int bk = tid >> 4, bk2 = tid & 15;
     for (int sweep = 0; sweep < inner_cap; ++sweep) {
         ++sweeps_done;
         for (int step = 0; step < JP - 1; ++step) {
-            if (tid < 16) {
-                int p, q;
-                if (tid == 0) {
-                    p = JP - 1;
-                    q = step;
-                } else {
-                    p = (step + tid) % (JP - 1);
-                    q = (step - tid + JP - 1) % (JP - 1);
-                }
-                const double a = G[p][p].x, d = G[q][q].x;
-                const double2 c = G[p][q];
-                const double gg = hypot(c.x, c.y);
-                const int act = gg > 0.0 && fmin(a, d) > nu2 && gg > tol_in * sqrt(fmax(a, 0.0) * fmax(d, 0.0));
-                rp[tid] = p;
-                rq[tid] = q;
-                ract[tid] = act;
-                if (act) {
-                    const double zeta = (d - a) / (2.0 * gg);
-                    const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                    const double cs = 1.0 / sqrt(1.0 + t * t);
-                    const double sn = cs * t;
-                    const double2 ph = make_double2(c.x / gg, c.y / gg);
-                    rcs[tid] = cs;
-                    rm01[tid] = make_double2(-ph.x * sn, -ph.y * sn);  // M = [[cs, -ph sn],
-                    rm10[tid] = make_double2(ph.x * sn, -ph.y * sn);   //      [sn conj(ph), cs]]
-                    rtg[tid] = t * gg;
-                    ra[tid] = a;
-                    rd[tid] = d;
-                    s_any = 1;
-                }
+            int p1, q1, p2, q2;
+            circle_pair(step, bk, p1, q1);
+            circle_pair(step, bk2, p2, q2);
+            const Rot64 r1 = rot_params64(Gc[p1][p1].x, Gc[q1][q1].x, Gc[p1][q1], tol, nu2);
+            const Rot64 r2 = rot_params64(Gc[p2][p2].x, Gc[q2][q2].x, Gc[p2][q2], tol, nu2);
+            double2 h00 = Gc[p1][p2], h01 = Gc[p1][q2], h10 = Gc[q1][p2], h11 = Gc[q1][q2];
+            double2 t00 = Tc[p1][p2], t01 = Tc[p1][q2], t10 = Tc[q1][p2], t11 = Tc[q1][q2];
+            if (r1.act) {
+                const double cs = r1.cs;
+                const double2 m01 = r1.m01, m10 = r1.m10;
+                const double2 x00 = cadd(cscale(h00, cs), cmul(m01, h10));
+                const double2 x01 = cadd(cscale(h01, cs), cmul(m01, h11));
+                const double2 x10 = cadd(cmul(m10, h00), cscale(h10, cs));
+                const double2 x11 = cadd(cmul(m10, h01), cscale(h11, cs));
+                h00 = x00;
+                h01 = x01;
+                h10 = x10;
+                h11 = x11;
+                const double2 y00 = cadd(cscale(t00, cs), cmul(m01, t10));
+                const double2 y01 = cadd(cscale(t01, cs), cmul(m01, t11));
+                const double2 y10 = cadd(cmul(m10, t00), cscale(t10, cs));
+                const double2 y11 = cadd(cmul(m10, t01), cscale(t11, cs));
+                t00 = y00;
+                t01 = y01;
+                t10 = y10;
+                t11 = y11;
             }
-            __syncthreads();
-            const int act1 = ract[bk], act2 = ract[bk2];
-            const int p1 = rp[bk], q1 = rq[bk], p2 = rp[bk2], q2 = rq[bk2];
-            double2 n00, n01, n10, n11, t00, t01, t10, t11;
-            if (act1 | act2) {
-                double2 h00 = G[p1][p2], h01 = G[p1][q2], h10 = G[q1][p2], h11 = G[q1][q2];
-                t00 = T[p1][p2];
-                t01 = T[p1][q2];
-                t10 = T[q1][p2];
-                t11 = T[q1][q2];
-                if (act1) {
-                    const double cs = rcs[bk];
-                    const double2 m01 = rm01[bk], m10 = rm10[bk];
-                    double2 x00 = cadd(cscale(h00, cs), cmul(m01, h10));
-                    double2 x01 = cadd(cscale(h01, cs), cmul(m01, h11));
-                    double2 x10 = cadd(cmul(m10, h00), cscale(h10, cs));
-                    double2 x11 = cadd(cmul(m10, h01), cscale(h11, cs));
-                    h00 = x00;
-                    h01 = x01;
-                    h10 = x10;
-                    h11 = x11;
-                    double2 y00 = cadd(cscale(t00, cs), cmul(m01, t10));
-                    double2 y01 = cadd(cscale(t01, cs), cmul(m01, t11));
-                    double2 y10 = cadd(cmul(m10, t00), cscale(t10, cs));
-                    double2 y11 = cadd(cmul(m10, t01), cscale(t11, cs));
-                    t00 = y00;
-                    t01 = y01;
-                    t10 = y10;
-                    t11 = y11;
-                }
-                if (act2) {
-                    const double cs = rcs[bk2];
-                    const double2 c01 = cconj(rm01[bk2]), c10 = cconj(rm10[bk2]);
-                    n00 = cadd(cscale(h00, cs), cmul(h01, c01));
-                    n01 = cadd(cmul(h00, c10), cscale(h01, cs));
-                    n10 = cadd(cscale(h10, cs), cmul(h11, c01));
-                    n11 = cadd(cmul(h10, c10), cscale(h11, cs));
-                } else {
-                    n00 = h00;
-                    n01 = h01;
-                    n10 = h10;
-                    n11 = h11;
-                }
-                if (bk == bk2 && act1) {
-                    n00 = make_double2(ra[bk] - rtg[bk], 0.0);
-                    n11 = make_double2(rd[bk] + rtg[bk], 0.0);
-                    n01 = cz();
-                    n10 = cz();
-                }
+            if (r2.act) {
+                const double cs = r2.cs;
+                const double2 c01 = cconj(r2.m01), c10 = cconj(r2.m10);
+                const double2 n00 = cadd(cscale(h00, cs), cmul(h01, c01));
+                const double2 n01 = cadd(cmul(h00, c10), cscale(h01, cs));
+                const double2 n10 = cadd(cscale(h10, cs), cmul(h11, c01));
+                const double2 n11 = cadd(cmul(h10, c10), cscale(h11, cs));
+                h00 = n00;
+                h01 = n01;
+                h10 = n10;
+                h11 = n11;
             }
-            __syncthreads();
-            if (act1 | act2) {
-                G[p1][p2] = n00;
-                G[p1][q2] = n01;
-                G[q1][p2] = n10;
-                G[q1][q2] = n11;
-                if (act1) {
-                    T[p1][p2] = t00;
-                    T[p1][q2] = t01;
-                    T[q1][p2] = t10;
-                    T[q1][q2] = t11;
-                }
+            if (bk == bk2 && r1.act) {
+                h00 = make_double2(Gc[p1][p1].x - r1.tg, 0.0);
+                h11 = make_double2(Gc[q1][q1].x + r1.tg, 0.0);
+                h01 = cz();
+                h10 = cz();
+                s_any = 1;
             }
+            Gn[p1][p2] = h00;
+            Gn[p1][q2] = h01;
+            Gn[q1][p2] = h10;
+            Gn[q1][q2] = h11;
+            Tn[p1][p2] = t00;
+            Tn[p1][q2] = t01;
+            Tn[q1][p2] = t10;
+            Tn[q1][q2] = t11;
             __syncthreads();
+            double2 (*sw)[JP + 1] = Gc;
+            Gc = Gn;
+            Gn = sw;
+            sw = Tc;
+            Tc = Tn;
+            Tn = sw;
         }
         // Every wave reads the flag before thread 0 clears it for the next sweep.
         const int again = s_any;
@@ -495,14 +545,15 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
         atomicAdd(&stats[1], 1ull);
     }
 
-    // Ordered rotation: row k of T moves to row dst[k]; G is free and holds the result.
-    pair_order(tid, s_row, &G[0][0].x, 2 * (JP + 2), s_dst);
+    // Ordered rotation: row k of T moves to row dst[k], into the free G buffer.
+    pair_order(tid, s_row, &Gc[0][0].x, 2 * (JP + 2), s_dst);
     __syncthreads();
     for (int e = tid; e < JP * JP; e += 256) {
         const int i = e / JP, j = e % JP;
-        G[s_dst[i]][j] = T[i][j];
+        Gn[s_dst[i]][j] = Tc[i][j];
     }
     __syncthreads();
+    double2 (*const Tp)[JP + 1] = Gn;
 
     // ---- 3. P <- T P on MFMA. Lane l supplies T[I*16 + l%16][4kk + l/16] (registers) and
     // P[4kk + l/16][c0 + l%16]; output element j of lane l is local row I*16 + 4j + l/16.
@@ -513,7 +564,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
         for (int I = 0; I < 2; ++I)
 #pragma unroll
             for (int kk = 0; kk < 8; ++kk) {
-                const double2 t = G[I * 16 + i][4 * kk + g];
+                const double2 t = Tp[I * 16 + i][4 * kk + g];
                 tr[I][kk] = t.x;
                 ti[I][kk] = t.y;
                 tni[I][kk] = -t.y;
@@ -600,15 +651,16 @@ __global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __re
     __shared__ float2 G[JP][JP + 1];
     __shared__ float2 T[JP][JP + 1];
     __shared__ double2 T64[JP][JP + 1];
-    __shared__ double2 W64[JP][JP + 1];
-    __shared__ float Gp[2][6][4][64];
+    // W64 of the Newton-Schulz steps; before them the partial Gram tiles, then the second G
+    // and T buffers of the eigensolve.
+    __shared__ __align__(16) double2 W64[JP][JP + 1];
     __shared__ int s_row[JP];
-    __shared__ int rp[16], rq[16], ract[16];
-    __shared__ float rcs[16], rtg[16], ra[16], rd[16];
-    __shared__ float2 rm01[16], rm10[16];
     __shared__ int s_any;
     __shared__ float s_low;
     __shared__ int s_dst[JP];
+    float (*Gp)[6][4][64] = reinterpret_cast<float (*)[6][4][64]>(&W64[0][0]);
+    float2 (*G2)[JP + 1] = reinterpret_cast<float2 (*)[JP + 1]>(&W64[0][0]);
+    float2 (*T2)[JP + 1] = reinterpret_cast<float2 (*)[JP + 1]>(reinterpret_cast<float*>(&W64[0][0]) + 2 * JP * (JP + 1));
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid < JP) {
@@ -723,106 +775,77 @@ __global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __re
     }
     __syncthreads();
 
+    float2 (*Gc)[JP + 1] = G;
+    float2 (*Tc)[JP + 1] = T;
+    float2 (*Gn)[JP + 1] = G2;
+    float2 (*Tn)[JP + 1] = T2;
     int sweeps_done = 0;
     const int bk = tid >> 4, bk2 = tid & 15;
+    auto fm = [](float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
+    auto fa = [](float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); };
+    auto fs = [](float2 a, float k) { return make_float2(a.x * k, a.y * k); };
+    auto fc = [](float2 a) { return make_float2(a.x, -a.y); };
+    auto f2 = [](double2 a) { return make_float2((float)a.x, (float)a.y); };
     for (int sweep = 0; sweep < inner_cap; ++sweep) {
         ++sweeps_done;
         for (int step = 0; step < JP - 1; ++step) {
-            if (tid < 16) {
-                int p, q;
-                if (tid == 0) {
-                    p = JP - 1;
-                    q = step;
-                } else {
-                    p = (step + tid) % (JP - 1);
-                    q = (step - tid + JP - 1) % (JP - 1);
-                }
-                const float a = G[p][p].x, d = G[q][q].x;
-                const float2 c = G[p][q];
-                const float gg = hypotf(c.x, c.y);
-                const int act = gg > 0.f && fminf(a, d) > low && gg > tol * sqrtf(fmaxf(a, 0.f) * fmaxf(d, 0.f));
-                rp[tid] = p;
-                rq[tid] = q;
-                ract[tid] = act;
-                if (act) {
-                    const float zeta = (d - a) / (2.f * gg);
-                    const float t = copysignf(1.f, zeta) / (fabsf(zeta) + sqrtf(1.f + zeta * zeta));
-                    const float cs = 1.f / sqrtf(1.f + t * t);
-                    const float sn = cs * t;
-                    const float2 ph = make_float2(c.x / gg, c.y / gg);
-                    rcs[tid] = cs;
-                    rm01[tid] = make_float2(-ph.x * sn, -ph.y * sn);
-                    rm10[tid] = make_float2(ph.x * sn, -ph.y * sn);
-                    rtg[tid] = t * gg;
-                    ra[tid] = a;
-                    rd[tid] = d;
-                    s_any = 1;
-                }
+            int p1, q1, p2, q2;
+            circle_pair(step, bk, p1, q1);
+            circle_pair(step, bk2, p2, q2);
+            // Parameters in FP64 from the FP32 Gram entries (no underflow of |G_pq|^2).
+            const float2 c1 = Gc[p1][q1], c2 = Gc[p2][q2];
+            const Rot64 r1 = rot_params64(Gc[p1][p1].x, Gc[q1][q1].x, make_double2(c1.x, c1.y), tol, low);
+            const Rot64 r2 = rot_params64(Gc[p2][p2].x, Gc[q2][q2].x, make_double2(c2.x, c2.y), tol, low);
+            float2 h00 = Gc[p1][p2], h01 = Gc[p1][q2], h10 = Gc[q1][p2], h11 = Gc[q1][q2];
+            float2 t00 = Tc[p1][p2], t01 = Tc[p1][q2], t10 = Tc[q1][p2], t11 = Tc[q1][q2];
+            if (r1.act) {
+                const float cs = (float)r1.cs;
+                const float2 m01 = f2(r1.m01), m10 = f2(r1.m10);
+                const float2 x00 = fa(fs(h00, cs), fm(m01, h10)), x01 = fa(fs(h01, cs), fm(m01, h11));
+                const float2 x10 = fa(fm(m10, h00), fs(h10, cs)), x11 = fa(fm(m10, h01), fs(h11, cs));
+                h00 = x00;
+                h01 = x01;
+                h10 = x10;
+                h11 = x11;
+                const float2 y00 = fa(fs(t00, cs), fm(m01, t10)), y01 = fa(fs(t01, cs), fm(m01, t11));
+                const float2 y10 = fa(fm(m10, t00), fs(t10, cs)), y11 = fa(fm(m10, t01), fs(t11, cs));
+                t00 = y00;
+                t01 = y01;
+                t10 = y10;
+                t11 = y11;
             }
-            __syncthreads();
-            const int act1 = ract[bk], act2 = ract[bk2];
-            const int p1 = rp[bk], q1 = rq[bk], p2 = rp[bk2], q2 = rq[bk2];
-            float2 n00, n01, n10, n11, t00, t01, t10, t11;
-            if (act1 | act2) {
-                float2 h00 = G[p1][p2], h01 = G[p1][q2], h10 = G[q1][p2], h11 = G[q1][q2];
-                t00 = T[p1][p2];
-                t01 = T[p1][q2];
-                t10 = T[q1][p2];
-                t11 = T[q1][q2];
-                auto fm = [](float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
-                auto fa = [](float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); };
-                auto fs = [](float2 a, float k) { return make_float2(a.x * k, a.y * k); };
-                auto fc = [](float2 a) { return make_float2(a.x, -a.y); };
-                if (act1) {
-                    const float cs = rcs[bk];
-                    const float2 m01 = rm01[bk], m10 = rm10[bk];
-                    float2 x00 = fa(fs(h00, cs), fm(m01, h10)), x01 = fa(fs(h01, cs), fm(m01, h11));
-                    float2 x10 = fa(fm(m10, h00), fs(h10, cs)), x11 = fa(fm(m10, h01), fs(h11, cs));
-                    h00 = x00;
-                    h01 = x01;
-                    h10 = x10;
-                    h11 = x11;
-                    float2 y00 = fa(fs(t00, cs), fm(m01, t10)), y01 = fa(fs(t01, cs), fm(m01, t11));
-                    float2 y10 = fa(fm(m10, t00), fs(t10, cs)), y11 = fa(fm(m10, t01), fs(t11, cs));
-                    t00 = y00;
-                    t01 = y01;
-                    t10 = y10;
-                    t11 = y11;
-                }
-                if (act2) {
-                    const float cs = rcs[bk2];
-                    const float2 c01 = fc(rm01[bk2]), c10 = fc(rm10[bk2]);
-                    n00 = fa(fs(h00, cs), fm(h01, c01));
-                    n01 = fa(fm(h00, c10), fs(h01, cs));
-                    n10 = fa(fs(h10, cs), fm(h11, c01));
-                    n11 = fa(fm(h10, c10), fs(h11, cs));
-                } else {
-                    n00 = h00;
-                    n01 = h01;
-                    n10 = h10;
-                    n11 = h11;
-                }
-                if (bk == bk2 && act1) {
-                    n00 = make_float2(ra[bk] - rtg[bk], 0.f);
-                    n11 = make_float2(rd[bk] + rtg[bk], 0.f);
-                    n01 = make_float2(0.f, 0.f);
-                    n10 = n01;
-                }
+            if (r2.act) {
+                const float cs = (float)r2.cs;
+                const float2 c01 = fc(f2(r2.m01)), c10 = fc(f2(r2.m10));
+                const float2 n00 = fa(fs(h00, cs), fm(h01, c01)), n01 = fa(fm(h00, c10), fs(h01, cs));
+                const float2 n10 = fa(fs(h10, cs), fm(h11, c01)), n11 = fa(fm(h10, c10), fs(h11, cs));
+                h00 = n00;
+                h01 = n01;
+                h10 = n10;
+                h11 = n11;
             }
-            __syncthreads();
-            if (act1 | act2) {
-                G[p1][p2] = n00;
-                G[p1][q2] = n01;
-                G[q1][p2] = n10;
-                G[q1][q2] = n11;
-                if (act1) {
-                    T[p1][p2] = t00;
-                    T[p1][q2] = t01;
-                    T[q1][p2] = t10;
-                    T[q1][q2] = t11;
-                }
+            if (bk == bk2 && r1.act) {
+                h00 = make_float2((float)(Gc[p1][p1].x - r1.tg), 0.f);
+                h11 = make_float2((float)(Gc[q1][q1].x + r1.tg), 0.f);
+                h01 = make_float2(0.f, 0.f);
+                h10 = h01;
+                s_any = 1;
             }
+            Gn[p1][p2] = h00;
+            Gn[p1][q2] = h01;
+            Gn[q1][p2] = h10;
+            Gn[q1][q2] = h11;
+            Tn[p1][p2] = t00;
+            Tn[p1][q2] = t01;
+            Tn[q1][p2] = t10;
+            Tn[q1][q2] = t11;
             __syncthreads();
+            float2 (*sw)[JP + 1] = Gc;
+            Gc = Gn;
+            Gn = sw;
+            sw = Tc;
+            Tc = Tn;
+            Tn = sw;
         }
         const int again = s_any;
         __syncthreads();
@@ -834,11 +857,13 @@ __global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __re
         atomicAdd(&stats[0], (unsigned long long)sweeps_done);
         atomicAdd(&stats[1], 1ull);
     }
+    // Row order from the rotated diagonal, before W64 (which may hold it) is reused.
+    pair_order(tid, s_row, &Gc[0][0].x, 2 * (JP + 2), s_dst);
 
     // T -> FP64, two Newton-Schulz steps T <- T (3I - T^H T) / 2 (unitary to FP64 precision).
     for (int e = tid; e < JP * JP; e += 256) {
         const int i = e / JP, j = e % JP;
-        T64[i][j] = make_double2((double)T[i][j].x, (double)T[i][j].y);
+        T64[i][j] = make_double2((double)Tc[i][j].x, (double)Tc[i][j].y);
     }
     __syncthreads();
     for (int it = 0; it < 2; ++it) {
@@ -867,8 +892,6 @@ __global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __re
     }
 
     // Ordered rotation (as in the FP64 round): row k of T moves to row dst[k], into W64.
-    pair_order(tid, s_row, &G[0][0].x, 2 * (JP + 2), s_dst);
-    __syncthreads();
     for (int e = tid; e < JP * JP; e += 256) {
         const int i = e / JP, j = e % JP;
         W64[s_dst[i]][j] = T64[i][j];
@@ -1316,9 +1339,9 @@ void launch_identity(double2* const* ptrs, const int* dims, int njobs, int max_d
 
 void launch_jacobi_round(const JacobiJob* jobs, int njobs, int max_pairs, int round, double tol,
                          double nu2, int inner_cap, int* rotated, const int* done, unsigned long long* stats,
-                         hipStream_t s) {
+                         unsigned long long* offmax, hipStream_t s) {
     hipLaunchKernelGGL(k_jacobi_round, dim3(max_pairs, njobs), dim3(256), 0, s, jobs, round, tol,
-                       nu2, inner_cap, rotated, done, stats);
+                       nu2, inner_cap, rotated, done, stats, offmax);
 }
 
 void launch_jacobi_round_lo(const JacobiLoJob* jobs, int njobs, int max_pairs, int round, float tol,
