@@ -69,6 +69,15 @@ typedef struct wqc_tn_gate {
     double angle;   /* RX/RY/RZ only */
 } wqc_tn_gate;
 
+/* One independent slice of a batch: a |0...0> register of qubit_count qubits with bond cap
+ * max_bond_dim, driven by gates[gate_offset .. gate_offset + gate_count) of a shared array. */
+typedef struct wqc_tn_slice {
+    size_t qubit_count;
+    size_t max_bond_dim;
+    size_t gate_offset;
+    size_t gate_count;
+} wqc_tn_slice;
+
 /* Arithmetic of the two-site split.
  *  FP64       : plain complex128 everywhere (the drop-in form).
  *  MIXED_FP32 : the Jacobi SVD runs its early sweeps on complex64 rows (FP32 MFMA Gram and
@@ -184,6 +193,51 @@ int wqc_tn_control_probability(wqc_tn_state *st, size_t control, double *out);
  * EXEC_FAILED "full statevector projection requires qubit_count <= 20".
  * `out` holds 2^n complex values (2^(n+1) doubles). */
 int wqc_tn_contract_to_statevector(wqc_tn_state *st, double *out, size_t len_complex);
+
+/* Replaces pauli_string_expectation over contract_to_statevector (src/expectation.rs:140-167,
+ * src/tn/mps.rs:215-260, which refuse qubit_count > 20): <psi|P_k|psi> for n_terms Pauli
+ * strings, contracted on the device with transfer matrices for any qubit_count. `labels` holds
+ * the strings back to back, each exactly qubit_count characters from {I, X, Y, Z}, in Qiskit
+ * order (the last character acts on qubit 0, src/expectation.rs:98-132). Un-normalized, as the
+ * reference. out[k] is the real part (the reference drops the rounding-level imaginary part);
+ * out_imag (optional, may be NULL) receives the imaginary part. INVALID_ARG on any other
+ * character. */
+int wqc_tn_pauli_expectations(wqc_tn_state *st, const char *labels, size_t n_terms, double *out,
+                              double *out_imag);
+/* A batch of independent slices, contract_slice (src/tn/contract.rs:10-30) per slice: new
+ * register, the slice's gates, <0...0|psi> into out_amps[2i], out_amps[2i+1]. Slices run
+ * concurrently on `workers` host threads (<= 0: 8), each driving one state on its own stream,
+ * largest estimated cost first. out_status[i] is the slice's status code (its amplitude is NaN
+ * when it is not OK); the call returns EXEC_FAILED naming the first failed slice, OK when all
+ * succeeded. */
+int wqc_tn_run_slices(wqc_tn_device *dev, const wqc_tn_slice *slices, size_t n_slices,
+                      const wqc_tn_gate *gates, int precision, int workers, double *out_amps,
+                      int *out_status);
+
+/* Terminal sampling, sample_terminal_measurements (src/sample.rs:192-211).
+ *  method 0: the reference's semantics (n <= 20, EXEC_FAILED above it, as the reference's
+ *            statevector projection): snapped Born table of the projected statevector keyed by
+ *            the classical register, StdRng::seed_from_u64(seed) (ChaCha12), the first
+ *            cumulative entry >= r per shot (src/sample.rs:85-189).
+ *  method 1: MPS sampler for any n <= 64: shot k draws qubits 0..n-1 from the MPS's
+ *            conditional Born probabilities using draw k*n+q of the same seeded stream, so
+ *            counts are exact samples of |psi_MPS|^2 (no counterpart in the reference, which
+ *            refuses n > 20).
+ *  method 2: method 0 when n <= 20, else method 1.
+ * measure k maps qubit measure_qubits[k] to cbit measure_cbits[k]; classical_bits in 1..64
+ * (EXEC_FAILED "classical_bit_count must be > 0 for sample_counts" at 0, as the reference).
+ * Writes up to `cap` (key, count) pairs in key order, key bit c = classical bit c (the
+ * reference's Qiskit bitstring has cbit 0 rightmost); *n_out = pairs written; INVALID_ARG if
+ * cap is too small (cap >= min(shots, 2^classical_bits) always suffices). */
+int wqc_tn_sample_counts(wqc_tn_state *st, const int32_t *measure_qubits,
+                         const int32_t *measure_cbits, size_t n_measures, size_t classical_bits,
+                         uint64_t shots, uint64_t seed, int method, uint64_t *keys,
+                         uint64_t *counts, size_t cap, size_t *n_out);
+/* observable_expectation + compute_expectations (src/expectation.rs:170-200): sum_k coeff_k
+ * <P_k> with complex coefficients (coeffs = n_terms (re, im) pairs), each component snapped to
+ * the 1e-6 grid (snap_complex_result, src/engine.rs:30-40). out = {re, im}. */
+int wqc_tn_observable_expectation(wqc_tn_state *st, const char *labels, const double *coeffs,
+                                  size_t n_terms, double out[2]);
 
 /* Bond dimensions: out[i] = right bond of site i, i < n (len >= n). */
 int wqc_tn_bond_dims(wqc_tn_state *st, size_t *out, size_t len);

@@ -32,10 +32,12 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
 #include "../../include/wqc_tn.h"
+#include "born.h"
 #include "kernels.h"
 
 namespace {
@@ -383,6 +385,7 @@ struct wqc_tn_state {
     hipStream_t stream = nullptr;
     DevBuf ws;        // split workspace
     DevBuf ws32;      // mixed-precision rows and the J X product
+    DevBuf ews;       // expectation environments and transfer buffers
     DevBuf tab;       // device job tables
     char* htab = nullptr;  // pinned staging for tables
     size_t htab_cap = 0, tab_off = 0;
@@ -413,6 +416,7 @@ struct wqc_tn_state {
             if (s.ptr) (void)hipFree(s.ptr);
         ws.release();
         ws32.release();
+        ews.release();
         tab.release();
         jstats_buf.release();
         trace_buf.release();
@@ -428,7 +432,7 @@ struct wqc_tn_state {
 
     // ------------------------------------------------------------ helpers
     void note_peak() {
-        uint64_t now = site_bytes + ws.bytes + ws32.bytes + tab.bytes;
+        uint64_t now = site_bytes + ws.bytes + ws32.bytes + ews.bytes + tab.bytes;
         if (now > stats.peak_vram_bytes) stats.peak_vram_bytes = now;
     }
 
@@ -508,6 +512,40 @@ struct wqc_tn_state {
             sites[i].l = sites[i].r = 1;
         }
         touched.assign(n, 0);
+        note_peak();
+    }
+
+    // A fresh |0...0> register of qn qubits and bond cap mb on the same stream, staging and
+    // workspaces; site buffers are kept where they are large enough. Used by the slice batch.
+    void reset(size_t qn, size_t mb) {
+        sync();
+        pend1.clear();
+        pend2.clear();
+        split_log.clear();
+        stats = wqc_tn_stats{};
+        cancel.store(0);
+        reject = false;
+        std::vector<Site> old = std::move(sites);
+        n = (int)qn;
+        chi = mb;
+        sites.assign(n, Site{});
+        for (int i = 0; i < n && i < (int)old.size(); ++i) {
+            sites[i] = old[i];
+            old[i] = Site{};
+        }
+        for (auto& s : old)
+            if (s.ptr) {
+                HIPC(hipFree(s.ptr));
+                site_bytes -= s.cap * sizeof(double2);
+            }
+        static const double2 zero_state[2] = {make_double2(1, 0), make_double2(0, 0)};
+        for (int i = 0; i < n; ++i) {
+            ensure_site_cap(i, bond_cap(i) * 2 * bond_cap(i + 1));
+            HIPC(hipMemcpyAsync(sites[i].ptr, zero_state, 2 * sizeof(double2), hipMemcpyHostToDevice, stream));
+            sites[i].l = sites[i].r = 1;
+        }
+        touched.assign(n, 0);
+        sync();
         note_peak();
     }
 
@@ -1527,6 +1565,244 @@ struct wqc_tn_state {
         out[3] = a1.y;
     }
 
+    // Left norm environments L_q (sites < q) and conjugated right norm environments Rb_q
+    // (sites >= q), q = 0..n, each l_q x l_q at env_off[q]; L may be null. FL and HR are
+    // scratch of 2 maxb^2 elements. L_{q+1} = A_stack^H (L_q A_flat) and
+    // Rb_q = (A_stack Rb_{q+1})_flat A_flat^H, where A_flat is the site as (l) x (2r) and
+    // A_stack the same memory as (2l) x (r).
+    void build_envs(double2* L, double2* Rb, const std::vector<size_t>& env_off, double2* FL, double2* HR) {
+        const double2 one = make_double2(1, 0), zero = make_double2(0, 0);
+        if (L) HIPC(hipMemcpyAsync(L + env_off[0], &kOne, sizeof(double2), hipMemcpyHostToDevice, stream));
+        HIPC(hipMemcpyAsync(Rb + env_off[n], &kOne, sizeof(double2), hipMemcpyHostToDevice, stream));
+        for (int i = 0; i < n; ++i) {
+            tables_reset();
+            const int ql = i, qr = n - 1 - i;
+            const int l1 = sites[ql].l, r1 = sites[ql].r, l2 = sites[qr].l, r2 = sites[qr].r;
+            std::vector<GemmJob> a, b;
+            if (L) {
+                a.push_back(GemmJob{L + env_off[ql], sites[ql].ptr, FL, l1, 2 * r1, l1, l1, 2 * r1, 2 * r1, 0, 0, one, zero});
+                b.push_back(GemmJob{sites[ql].ptr, FL, L + env_off[ql + 1], r1, r1, 2 * l1, r1, r1, r1, 1, 0, one, zero});
+            }
+            a.push_back(GemmJob{sites[qr].ptr, Rb + env_off[qr + 1], HR, 2 * l2, r2, r2, r2, r2, r2, 0, 0, one, zero});
+            b.push_back(GemmJob{HR, sites[qr].ptr, Rb + env_off[qr], l2, l2, 2 * r2, 2 * r2, 2 * r2, l2, 0, 1, one, zero});
+            wqc::launch_gemm(upload(a), (int)a.size(), std::max(L ? l1 : 1, 2 * l2), std::max(L ? 2 * r1 : 1, r2), stream);
+            wqc::launch_gemm(upload(b), (int)b.size(), std::max(L ? r1 : 1, l2), std::max(L ? r1 : 1, l2), stream);
+            check_launch();
+            sync();
+        }
+    }
+
+    // Terminal sampling (sample_terminal_measurements, src/sample.rs:192-211).
+    //  exact: the reference's semantics, n <= 20: device projection to the statevector, then
+    //         the snapped Born table and the seeded ChaCha12 draw on the host (born.cpp).
+    //  MPS:   any n. Shot k draws qubits 0..n-1 in order from the MPS's conditional Born
+    //         probabilities p(s | earlier outcomes) = <v A_s| R_{q+1} |v A_s>, with the right
+    //         norm environments R and the carried left vector v renormalized by sqrt(p_s);
+    //         the uniform for (shot k, qubit q) is draw k*n + q of the same seeded stream.
+    //         Shots are exact samples of |psi_MPS|^2 / <psi|psi>.
+    std::vector<std::pair<uint64_t, uint64_t>> sample_counts(const int32_t* mq, const int32_t* mc,
+                                                             size_t nm, size_t cbits, uint64_t shots,
+                                                             uint64_t seed, int method) {
+        if (cbits == 0) fail(WQC_TN_EXEC_FAILED, "classical_bit_count must be > 0 for sample_counts");
+        if (cbits > 64) fail(WQC_TN_INVALID_ARG, "classical_bit_count above 64 is not supported");
+        for (size_t k = 0; k < nm; ++k) {
+            if (mq[k] < 0 || mq[k] >= n)
+                fail(WQC_TN_INVALID_ARG, "qubit index " + std::to_string(mq[k]) + " out of bounds (limit: " +
+                                             std::to_string(n) + ")");
+            if (mc[k] < 0 || (size_t)mc[k] >= cbits)
+                fail(WQC_TN_EXEC_FAILED, "classical bit index " + std::to_string(mc[k]) +
+                                             " out of bounds (limit: " + std::to_string(cbits) + ")");
+        }
+        const bool exact = method == 0 || (method == 2 && n <= 20);
+        if (method < 0 || method > 2) fail(WQC_TN_INVALID_ARG, "unknown sampling method");
+        if (exact) {
+            if (n > 20) fail(WQC_TN_EXEC_FAILED, "full statevector projection requires qubit_count <= 20");
+            std::vector<double> sv((size_t)2 << n);
+            statevector(sv.data(), (size_t)1 << n);
+            return wqc::draw_counts(wqc::born_table(sv.data(), n, mq, mc, nm, cbits), shots, seed);
+        }
+        if (n > 64) fail(WQC_TN_INVALID_ARG, "the MPS sampler supports qubit_count <= 64");
+        flush();
+        sync();
+        size_t maxb = 1, env_elems = 0;
+        std::vector<size_t> env_off(n + 1);
+        for (int q = 0; q <= n; ++q) {
+            const size_t l = q < n ? sites[q].l : 1;
+            env_off[q] = env_elems;
+            env_elems += l * l;
+            maxb = std::max(maxb, l);
+        }
+        const uint64_t chunk = std::min<uint64_t>(std::max<uint64_t>(shots, 1), 16384);
+        const size_t fl = 2 * maxb * maxb;
+        // Rb, FL, HR, then per chunk: V, Vn (chunk x maxb), W (chunk x 2 maxb), Z (2 chunk x maxb),
+        // uniforms (chunk x n doubles) and bits.
+        const size_t vec = (size_t)chunk * maxb;
+        const size_t total = env_elems + 2 * fl + 2 * vec + 2 * vec + 2 * vec + ((size_t)chunk * n + 1) / 2 + chunk + 64;
+        ews.ensure(total * sizeof(double2));
+        note_peak();
+        double2* Rb = (double2*)ews.p;
+        double2* FL = Rb + env_elems;
+        double2* HR = FL + fl;
+        double2* V = HR + fl;
+        double2* Vn = V + vec;
+        double2* W = Vn + vec;
+        double2* Z = W + 2 * vec;
+        double* U = reinterpret_cast<double*>(Z + 2 * vec);
+        unsigned long long* B = reinterpret_cast<unsigned long long*>(U + (size_t)chunk * n + 2);
+        build_envs(nullptr, Rb, env_off, FL, HR);
+        const double2 one = make_double2(1, 0), zero = make_double2(0, 0);
+        wqc::ChaCha12Stream rng(seed);
+        std::map<uint64_t, uint64_t> counts;
+        std::vector<double> hu;
+        std::vector<unsigned long long> hb;
+        std::vector<double2> ones;
+        for (uint64_t s0 = 0; s0 < shots; s0 += chunk) {
+            const long m = (long)std::min<uint64_t>(chunk, shots - s0);
+            hu.resize((size_t)m * n);
+            for (auto& x : hu) x = rng.next_f64();
+            HIPC(hipMemcpyAsync(U, hu.data(), hu.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+            wqc::launch_fill_int(reinterpret_cast<int*>(B), 0, (int)(2 * m), stream);
+            ones.assign((size_t)m, make_double2(1, 0));
+            HIPC(hipMemcpyAsync(V, ones.data(), (size_t)m * sizeof(double2), hipMemcpyHostToDevice, stream));
+            for (int q = 0; q < n; ++q) {
+                check_cancel();
+                tables_reset();
+                const int l = sites[q].l, r = sites[q].r;
+                std::vector<GemmJob> a{GemmJob{V, sites[q].ptr, W, (int)m, 2 * r, l, l, 2 * r, 2 * r, 0, 0, one, zero}};
+                std::vector<GemmJob> b{GemmJob{W, Rb + env_off[q + 1], Z, (int)m, r, r, 2 * r, r, r, 0, 1, one, zero},
+                                       GemmJob{W + r, Rb + env_off[q + 1], Z + (size_t)m * r, (int)m, r, r, 2 * r, r, r, 0, 1, one, zero}};
+                wqc::launch_gemm(upload(a), 1, (int)m, 2 * r, stream);
+                wqc::launch_gemm(upload(b), 2, (int)m, r, stream);
+                wqc::launch_sample_step(W, Z, r, m, U + q, n, q, B, Vn, stream);
+                check_launch();
+                std::swap(V, Vn);
+                sync();
+            }
+            hb.resize((size_t)m);
+            HIPC(hipMemcpyAsync(hb.data(), B, (size_t)m * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+            sync();
+            for (long k = 0; k < m; ++k) {
+                uint64_t key = 0;
+                for (size_t j = 0; j < nm; ++j) {
+                    const uint64_t bit = (hb[k] >> mq[j]) & 1u;
+                    const uint64_t msk = (uint64_t)1 << mc[j];
+                    key = bit ? (key | msk) : (key & ~msk);
+                }
+                ++counts[key];
+            }
+        }
+        return std::vector<std::pair<uint64_t, uint64_t>>(counts.begin(), counts.end());
+    }
+
+    // <psi|P_k|psi> of Pauli strings by transfer matrices. The reference evaluates the same
+    // sum over the dense statevector, sum_i conj(psi_P(i)) phase_i psi_i
+    // (src/expectation.rs:140-167, n <= 20 through contract_to_statevector); here any n.
+    // Site q carries qubit q, whose Pauli is the label's character n-1-q (Qiskit order,
+    // src/expectation.rs:98-132). Left norm environments L_q (sites < q) and conjugated right
+    // environments Rb_q (sites >= q) are built once; a term runs the transfer
+    // E <- B_stack^H (E A_flat), B_t = sum_s' conj(P[s',t]) A_s', only over the sites from its
+    // first to its last non-identity character and closes with sum E conj(Rb).
+    void pauli_expectations(const char* labels, size_t nterms, double* out_re, double* out_im) {
+        flush();
+        sync();
+        std::vector<int> lo(nterms, -1), hi(nterms, -1);
+        for (size_t k = 0; k < nterms; ++k) {
+            const char* lab = labels + k * (size_t)n;
+            for (int c = 0; c < n; ++c) {
+                const char ch = lab[c];
+                if (ch != 'I' && ch != 'X' && ch != 'Y' && ch != 'Z')
+                    fail(WQC_TN_INVALID_ARG, std::string("invalid pauli character ") +
+                                                 (ch ? std::string("'") + ch + "'" : std::string("NUL")) +
+                                                 " in label " + std::to_string(k));
+            }
+            for (int q = 0; q < n; ++q)
+                if (lab[n - 1 - q] != 'I') {
+                    if (lo[k] < 0) lo[k] = q;
+                    hi[k] = q;
+                }
+        }
+        auto ldim = [&](int q) { return q < n ? sites[q].l : 1; };
+        size_t maxb = 1, max_site = 1, env_elems = 0;
+        std::vector<size_t> env_off(n + 1);
+        for (int q = 0; q <= n; ++q) {
+            env_off[q] = env_elems;
+            env_elems += (size_t)ldim(q) * ldim(q);
+            maxb = std::max(maxb, (size_t)ldim(q));
+        }
+        for (int q = 0; q < n; ++q) max_site = std::max(max_site, (size_t)sites[q].l * 2 * sites[q].r);
+        const size_t sq = maxb * maxb, fl = 2 * maxb * maxb;
+        // Terms per pass bound the per-term buffers (two environments and one F) to 2 GiB.
+        const size_t per_term = 2 * sq + fl;
+        const size_t chunk = std::max<size_t>(1, std::min<size_t>(nterms ? nterms : 1, ((size_t)2 << 30) / (per_term * sizeof(double2))));
+        const size_t fixed = 2 * env_elems + 2 * fl + 3 * max_site + 2 * nterms + 64;
+        ews.ensure((fixed + chunk * per_term) * sizeof(double2));
+        note_peak();
+        double2* base = (double2*)ews.p;
+        double2* L = base;
+        double2* Rb = L + env_elems;
+        double2* FL = Rb + env_elems;
+        double2* HR = FL + fl;
+        double2* BX = HR + fl;
+        double2* BY = BX + max_site;
+        double2* BZ = BY + max_site;
+        double2* res = BZ + max_site;
+        double2* terms = res + 2 * nterms + 64;
+        auto site_flat = [&](int q) { return sites[q].ptr; };
+        const double2 one = make_double2(1, 0), zero = make_double2(0, 0);
+        build_envs(L, Rb, env_off, FL, HR);
+        for (size_t k0 = 0; k0 < nterms; k0 += chunk) {
+            const size_t k1 = std::min(nterms, k0 + chunk);
+            std::vector<double2*> cur(k1 - k0, nullptr);
+            for (int q = 0; q < n; ++q) {
+                std::vector<size_t> act;
+                for (size_t k = k0; k < k1; ++k)
+                    if (lo[k] >= 0 && lo[k] <= q && q <= hi[k]) act.push_back(k);
+                if (act.empty()) continue;
+                tables_reset();
+                const int l = sites[q].l, r = sites[q].r;
+                wqc::launch_pauli_site(site_flat(q), BX, BY, BZ, l, r, stream);
+                std::vector<GemmJob> a, b;
+                for (size_t k : act) {
+                    double2* tb = terms + (k - k0) * per_term;
+                    double2* e0 = tb;
+                    double2* e1 = tb + sq;
+                    double2* f = tb + 2 * sq;
+                    const double2* ein = q == lo[k] ? L + env_off[q] : cur[k - k0];
+                    double2* eout = (ein == e0) ? e1 : e0;
+                    const char ch = labels[k * (size_t)n + (n - 1 - q)];
+                    const double2* bsite = ch == 'X' ? BX : ch == 'Y' ? BY : ch == 'Z' ? BZ : site_flat(q);
+                    a.push_back(GemmJob{ein, site_flat(q), f, l, 2 * r, l, l, 2 * r, 2 * r, 0, 0, one, zero});
+                    b.push_back(GemmJob{bsite, f, eout, r, r, 2 * l, r, r, r, 1, 0, one, zero});
+                    cur[k - k0] = eout;
+                }
+                wqc::launch_gemm(upload(a), (int)a.size(), l, 2 * r, stream);
+                wqc::launch_gemm(upload(b), (int)b.size(), r, r, stream);
+                check_launch();
+                sync();
+            }
+            tables_reset();
+            std::vector<wqc::CloseJob> cj;
+            for (size_t k = k0; k < k1; ++k) {
+                if (lo[k] < 0)
+                    cj.push_back(wqc::CloseJob{L + env_off[n], Rb + env_off[n], res + k, 1});
+                else {
+                    const int qe = hi[k] + 1;
+                    cj.push_back(wqc::CloseJob{cur[k - k0], Rb + env_off[qe], res + k, (long)ldim(qe) * ldim(qe)});
+                }
+            }
+            wqc::launch_env_close(upload(cj), (int)cj.size(), stream);
+            check_launch();
+            sync();
+        }
+        std::vector<double2> h(std::max<size_t>(nterms, 1));
+        if (nterms) HIPC(hipMemcpyAsync(h.data(), res, nterms * sizeof(double2), hipMemcpyDeviceToHost, stream));
+        sync();
+        for (size_t k = 0; k < nterms; ++k) {
+            out_re[k] = h[k].x;
+            if (out_im) out_im[k] = h[k].y;
+        }
+    }
+
     // contract_to_statevector (src/tn/mps.rs:215-260).
     void statevector(double* out, size_t len_complex) {
         flush();
@@ -1788,6 +2064,115 @@ int wqc_tn_control_probability(wqc_tn_state* st, size_t control, double* out) {
         double a[4];
         st->site_amplitudes((int)control, a);
         *out = a[2] * a[2] + a[3] * a[3];
+    });
+}
+
+int wqc_tn_pauli_expectations(wqc_tn_state* st, const char* labels, size_t n_terms, double* out,
+                              double* out_imag) {
+    return guarded([&] {
+        if (!st || (n_terms && (!labels || !out))) fail(WQC_TN_INVALID_ARG, "null argument");
+        st->pauli_expectations(labels, n_terms, out, out_imag);
+    });
+}
+
+int wqc_tn_observable_expectation(wqc_tn_state* st, const char* labels, const double* coeffs,
+                                  size_t n_terms, double out[2]) {
+    return guarded([&] {
+        if (!st || !out || (n_terms && (!labels || !coeffs))) fail(WQC_TN_INVALID_ARG, "null argument");
+        if (!n_terms) fail(WQC_TN_INVALID_ARG, "observable must have at least one term");
+        std::vector<double> v(n_terms);
+        st->pauli_expectations(labels, n_terms, v.data(), nullptr);
+        // observable_expectation (src/expectation.rs:170-186): sum of coeff * <P>, in term order;
+        // compute_expectations snaps each component to the 1e-6 grid (src/engine.rs:30-40).
+        double re = 0.0, im = 0.0;
+        for (size_t k = 0; k < n_terms; ++k) {
+            re += coeffs[2 * k] * v[k];
+            im += coeffs[2 * k + 1] * v[k];
+        }
+        out[0] = std::round(re * 1e6) / 1e6;
+        out[1] = std::round(im * 1e6) / 1e6;
+    });
+}
+
+int wqc_tn_sample_counts(wqc_tn_state* st, const int32_t* measure_qubits, const int32_t* measure_cbits,
+                         size_t n_measures, size_t classical_bits, uint64_t shots, uint64_t seed,
+                         int method, uint64_t* keys, uint64_t* counts, size_t cap, size_t* n_out) {
+    return guarded([&] {
+        if (!st || !n_out || (n_measures && (!measure_qubits || !measure_cbits)) || (cap && (!keys || !counts)))
+            fail(WQC_TN_INVALID_ARG, "null argument");
+        auto res = st->sample_counts(measure_qubits, measure_cbits, n_measures, classical_bits, shots, seed, method);
+        if (res.size() > cap) fail(WQC_TN_INVALID_ARG, "output capacity too small for the distinct outcomes");
+        for (size_t i = 0; i < res.size(); ++i) {
+            keys[i] = res[i].first;
+            counts[i] = res[i].second;
+        }
+        *n_out = res.size();
+    });
+}
+
+int wqc_tn_run_slices(wqc_tn_device* dev, const wqc_tn_slice* slices, size_t n_slices,
+                      const wqc_tn_gate* gates, int precision, int workers, double* out_amps,
+                      int* out_status) {
+    return guarded([&] {
+        if (!dev || (n_slices && (!slices || !out_amps || !out_status))) fail(WQC_TN_INVALID_ARG, "null argument");
+        if (precision != WQC_TN_PRECISION_FP64 && precision != WQC_TN_PRECISION_MIXED_FP32)
+            fail(WQC_TN_INVALID_ARG, "unknown precision mode");
+        for (size_t i = 0; i < n_slices; ++i) {
+            if (slices[i].qubit_count == 0) fail(WQC_TN_INVALID_ARG, "invalid qubit count: 0");
+            if (slices[i].max_bond_dim == 0) fail(WQC_TN_EXEC_FAILED, "MPS max bond dimension must be > 0");
+            if (slices[i].gate_count && !gates) fail(WQC_TN_INVALID_ARG, "null argument");
+        }
+        if (!n_slices) return;
+        const int nw = std::max(1, std::min(workers > 0 ? workers : 8, (int)std::min<size_t>(n_slices, 64)));
+        // Largest estimated cost first (two-qubit gates x chi^3 at the slice's reachable bond).
+        std::vector<size_t> order(n_slices);
+        std::vector<double> cost(n_slices);
+        for (size_t i = 0; i < n_slices; ++i) {
+            order[i] = i;
+            const double b = (double)std::min<size_t>(slices[i].max_bond_dim,
+                                                      (size_t)1 << std::min<size_t>(slices[i].qubit_count / 2, 30));
+            cost[i] = (double)slices[i].gate_count * b * b * b;
+        }
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+        std::atomic<size_t> next{0};
+        std::mutex emu;
+        std::string first_error;
+        auto worker = [&]() {
+            (void)hipSetDevice(dev->ordinal);
+            std::unique_ptr<wqc_tn_state> st;
+            for (;;) {
+                const size_t t = next.fetch_add(1);
+                if (t >= n_slices) break;
+                const size_t i = order[t];
+                const wqc_tn_slice& sl = slices[i];
+                out_amps[2 * i] = out_amps[2 * i + 1] = std::nan("");
+                const int rc = guarded([&] {
+                    if (!st) {
+                        st = std::make_unique<wqc_tn_state>();
+                        st->init(dev, sl.qubit_count, sl.max_bond_dim, precision);
+                    } else {
+                        st->reset(sl.qubit_count, sl.max_bond_dim);
+                    }
+                    st->queue_circuit(gates + sl.gate_offset, sl.gate_count);
+                    auto a = st->amplitude_zero();
+                    out_amps[2 * i] = a.first;
+                    out_amps[2 * i + 1] = a.second;
+                });
+                out_status[i] = rc;
+                if (rc != WQC_TN_OK) {
+                    std::lock_guard<std::mutex> lk(emu);
+                    if (first_error.empty()) first_error = "slice " + std::to_string(i) + ": " + g_last_error;
+                    if (st) {
+                        st.reset();
+                    }
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int w = 1; w < nw; ++w) pool.emplace_back(worker);
+        worker();
+        for (auto& th : pool) th.join();
+        if (!first_error.empty()) fail(WQC_TN_EXEC_FAILED, first_error);
     });
 }
 

@@ -132,4 +132,24 @@ void launch_add_identity(double2* C, int rows, int cols, int ldc, hipStream_t s)
 void launch_statevector_step(const double2* st, const double2* site, double2* next, int l,
                              int r, long prefix, hipStream_t s);
 
+// Expectation values: the Pauli-transformed copies of a site, B_t = sum_s' conj(P[s',t]) A_s'
+// for P = X, Y, Z, and the closing contraction out = sum_i E[i] conj(R[i]) of a transfer-matrix
+// chain with a conjugated right environment.
+struct CloseJob {
+    const double2* E;
+    const double2* R;
+    double2* out;
+    long count;
+};
+void launch_pauli_site(const double2* A, double2* bx, double2* by, double2* bz, int l, int r,
+                       hipStream_t s);
+void launch_env_close(const CloseJob* jobs, int njobs, hipStream_t s);
+
+// One site of the MPS sampler for `shots` shots: W (shots x 2r) holds v A_s for s = 0, 1,
+// Z (two blocks of shots x r) holds R (v A_s)^T; picks s with the uniform u[k * ustride],
+// sets bit `bit` of bits[k], writes v' = (v A_s) / sqrt(p_s) into vnext (shots x r).
+void launch_sample_step(const double2* W, const double2* Z, int r, long shots, const double* u,
+                        long ustride, int bit, unsigned long long* bits, double2* vnext,
+                        hipStream_t s);
+
 }  // namespace wqc

@@ -1282,6 +1282,75 @@ __global__ void k_add_identity(double2* C, int rows, int cols, int ldc) {
 }
 
 // next[b*(2p) + ph*p + bp] = sum_a st[a*p + bp] * site[a, ph, b]
+// ------------------------------------------------------------------ expectation values
+// Site A (l, 2, r) and its Pauli-transformed copies B_t = sum_s' conj(P[s', t]) A_s' for
+// P = X, Y, Z: X swaps the physical halves, Y gives (-i A_1, i A_0), Z gives (A_0, -A_1).
+// With them one transfer step of <psi|P|psi> is E' = B_stack^H (E A_flat) (two GEMMs).
+__global__ __launch_bounds__(256) void k_pauli_site(const double2* __restrict__ A, double2* __restrict__ bx,
+                                                    double2* __restrict__ by, double2* __restrict__ bz,
+                                                    int l, int r) {
+    const long half = (long)l * r;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < half; e += (long)gridDim.x * 256) {
+        const long a = e / r, b = e % r;
+        const long i0 = a * 2 * r + b, i1 = i0 + r;
+        const double2 v0 = A[i0], v1 = A[i1];
+        bx[i0] = v1;
+        bx[i1] = v0;
+        by[i0] = make_double2(v1.y, -v1.x);   // -i * v1
+        by[i1] = make_double2(-v0.y, v0.x);   //  i * v0
+        bz[i0] = v0;
+        bz[i1] = make_double2(-v1.x, -v1.y);
+    }
+}
+
+// out = sum_i E[i] conj(R[i]) per job, one workgroup per job, fixed-order reduction.
+__global__ __launch_bounds__(256) void k_env_close(const CloseJob* __restrict__ jobs) {
+    const CloseJob jb = jobs[blockIdx.x];
+    __shared__ double2 red[256];
+    double2 acc = cz();
+    for (long i = threadIdx.x; i < jb.count; i += 256) ccmac(acc, jb.R[i], jb.E[i]);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *jb.out = red[0];
+}
+
+// ------------------------------------------------------------------ MPS sampler step
+// One wave per shot: p_s = Re sum_b conj(W_s[b]) Z_s[b] = <v A_s| R |v A_s>, the unnormalized
+// probability of outcome s given the shot's earlier outcomes; s = 0 when u < p_0 / (p_0 + p_1).
+__global__ __launch_bounds__(256) void k_sample_step(const double2* __restrict__ W,
+                                                     const double2* __restrict__ Z, int r,
+                                                     long shots, const double* __restrict__ u,
+                                                     long ustride, int bit,
+                                                     unsigned long long* __restrict__ bits,
+                                                     double2* __restrict__ vnext) {
+    const long k = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (k >= shots) return;
+    const double2* w = W + k * 2 * r;
+    const double2* z0 = Z + k * r;
+    const double2* z1 = Z + (shots + k) * r;
+    double p0 = 0.0, p1 = 0.0;
+    for (int b = lane; b < r; b += 64) {
+        p0 = fma(w[b].x, z0[b].x, fma(w[b].y, z0[b].y, p0));
+        p1 = fma(w[r + b].x, z1[b].x, fma(w[r + b].y, z1[b].y, p1));
+    }
+    p0 = fmax(wave_sum(p0), 0.0);
+    p1 = fmax(wave_sum(p1), 0.0);
+    const double tot = p0 + p1;
+    const int s = (tot > 0.0 && u[k * ustride] < p0 / tot) ? 0 : 1;
+    const double ps = s ? p1 : p0;
+    const double scale = ps > 0.0 ? 1.0 / sqrt(ps) : 0.0;
+    for (int b = lane; b < r; b += 64) {
+        const double2 v = w[s * r + b];
+        vnext[k * r + b] = make_double2(v.x * scale, v.y * scale);
+    }
+    if (lane == 0 && s) bits[k] |= 1ull << bit;
+}
+
 __global__ __launch_bounds__(256) void k_statevector_step(const double2* __restrict__ st,
                                                           const double2* __restrict__ site,
                                                           double2* __restrict__ next, int l,
@@ -1361,6 +1430,23 @@ void launch_row_norms(const JacobiJob* jobs, int njobs, int max_n, double* sigma
                       const long* sigma_off, hipStream_t s) {
     hipLaunchKernelGGL(k_row_norms, dim3((max_n + 3) / 4, njobs), dim3(256), 0, s, jobs, sigma,
                        sigma_off);
+}
+
+void launch_pauli_site(const double2* A, double2* bx, double2* by, double2* bz, int l, int r,
+                       hipStream_t s) {
+    const long half = (long)l * r;
+    hipLaunchKernelGGL(k_pauli_site, dim3(blocks_for(half, 256, 4096)), dim3(256), 0, s, A, bx, by, bz, l, r);
+}
+
+void launch_sample_step(const double2* W, const double2* Z, int r, long shots, const double* u,
+                        long ustride, int bit, unsigned long long* bits, double2* vnext,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_sample_step, dim3((unsigned)((shots + 3) / 4)), dim3(256), 0, s, W, Z, r,
+                       shots, u, ustride, bit, bits, vnext);
+}
+
+void launch_env_close(const CloseJob* jobs, int njobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_env_close, dim3(njobs), dim3(256), 0, s, jobs);
 }
 
 void launch_gemm(const GemmJob* jobs, int njobs, int max_m, int max_n, hipStream_t s) {

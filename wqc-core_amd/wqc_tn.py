@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libwqc_tn.so")
 
 OK, INVALID_ARG, OUT_OF_MEMORY, EXEC_FAILED, CANCELLED, BOND_EXCEEDED, NO_DEVICE = range(7)
 PRECISION_FP64, PRECISION_MIXED_FP32 = 0, 1
+SAMPLE_EXACT, SAMPLE_MPS, SAMPLE_AUTO = 0, 1, 2
 
 EXPORTED = (
     "wqc_tn_last_error", "wqc_tn_backend_from_env", "wqc_tn_engine_status", "wqc_tn_device_open",
@@ -30,12 +31,19 @@ EXPORTED = (
     "wqc_tn_bond_dims", "wqc_tn_download_site", "wqc_tn_upload_site", "wqc_tn_backend_label",
     "wqc_tn_peak_vram_bytes", "wqc_tn_get_stats", "wqc_tn_kernel_times", "wqc_tn_enable_timing",
     "wqc_tn_kernel_launches", "wqc_tn_enable_split_log", "wqc_tn_split_log",
+    "wqc_tn_pauli_expectations", "wqc_tn_observable_expectation", "wqc_tn_sample_counts",
+    "wqc_tn_run_slices",
 )
 
 
 class Gate(ct.Structure):
     _fields_ = [("kind", ct.c_int32), ("q0", ct.c_int32), ("q1", ct.c_int32), ("q2", ct.c_int32),
                 ("angle", ct.c_double)]
+
+
+class Slice(ct.Structure):
+    _fields_ = [("qubit_count", ct.c_size_t), ("max_bond_dim", ct.c_size_t),
+                ("gate_offset", ct.c_size_t), ("gate_count", ct.c_size_t)]
 
 
 class Stats(ct.Structure):
@@ -92,6 +100,13 @@ def load():
     L.wqc_tn_site_amplitudes.argtypes = [vp, sz, dp]
     L.wqc_tn_control_probability.argtypes = [vp, sz, dp]
     L.wqc_tn_contract_to_statevector.argtypes = [vp, dp, sz]
+    L.wqc_tn_pauli_expectations.argtypes = [vp, ct.c_char_p, sz, dp, dp]
+    L.wqc_tn_observable_expectation.argtypes = [vp, ct.c_char_p, dp, sz, dp]
+    i32p, u64p = ct.POINTER(ct.c_int32), ct.POINTER(ct.c_uint64)
+    L.wqc_tn_run_slices.argtypes = [vp, ct.POINTER(Slice), sz, ct.c_void_p, ct.c_int, ct.c_int, dp,
+                                    ct.POINTER(ct.c_int)]
+    L.wqc_tn_sample_counts.argtypes = [vp, i32p, i32p, sz, sz, ct.c_uint64, ct.c_uint64, ct.c_int,
+                                       u64p, u64p, sz, ct.POINTER(ct.c_size_t)]
     L.wqc_tn_bond_dims.argtypes = [vp, ct.POINTER(sz), sz]
     L.wqc_tn_download_site.argtypes = [vp, sz, dp, sz, ct.POINTER(sz), ct.POINTER(sz)]
     L.wqc_tn_upload_site.argtypes = [vp, sz, dp, sz, sz]
@@ -150,6 +165,30 @@ def default_device() -> Device:
     if _default_device is None:
         _default_device = Device(0)
     return _default_device
+
+
+def run_slices(slices, precision: int = PRECISION_FP64, workers: int = 8, device=None):
+    """<0...0|psi> of independent slices [(qubit_count, max_bond_dim, gates), ...] run
+    concurrently (contract_slice per slice, src/tn/contract.rs:10-30); returns
+    (amplitudes, status codes)."""
+    L = load()
+    dev = device or default_device()
+    descs = (Slice * max(1, len(slices)))()
+    allg = []
+    for i, (n, chi, gates) in enumerate(slices):
+        descs[i].qubit_count, descs[i].max_bond_dim = n, chi
+        descs[i].gate_offset, descs[i].gate_count = len(allg), len(gates)
+        allg.extend(gates)
+    garr = gate_array(allg)
+    amps = np.zeros(2 * max(1, len(slices)))
+    status = (ct.c_int * max(1, len(slices)))()
+    rc = L.wqc_tn_run_slices(dev._h, descs, len(slices), ct.cast(garr, ct.c_void_p), precision,
+                             workers, _dp(amps), status)
+    out = amps[: 2 * len(slices)].view(np.complex128)
+    codes = [status[i] for i in range(len(slices))]
+    if rc != OK and all(c == OK for c in codes):
+        _check(rc)
+    return out, codes
 
 
 class MpsState:
@@ -240,6 +279,46 @@ class MpsState:
         out = np.zeros(2 ** (n + 1)) if n <= 20 else np.zeros(2)
         _check(load().wqc_tn_contract_to_statevector(self._h, _dp(out), out.size // 2))
         return out.view(np.complex128)
+
+    def pauli_expectations(self, labels, with_imag=False):
+        """<psi|P|psi> per Pauli label (Qiskit order, un-normalized), src/expectation.rs:140-167."""
+        labels = list(labels)
+        buf = "".join(labels).encode("ascii")
+        re = np.zeros(max(1, len(labels)))
+        im = np.zeros(max(1, len(labels)))
+        _check(load().wqc_tn_pauli_expectations(self._h, buf, len(labels), _dp(re), _dp(im)))
+        if with_imag:
+            return re[:len(labels)] + 1j * im[:len(labels)]
+        return re[:len(labels)]
+
+    def sample_counts(self, measures, classical_bits, shots, seed, method=None):
+        """{bitstring: count} with the reference's Qiskit keys (cbit 0 rightmost),
+        src/sample.rs:192-211; measures = [(qubit, cbit), ...]."""
+        if method is None:
+            method = SAMPLE_AUTO
+        mq = np.array([m[0] for m in measures] or [0], dtype=np.int32)
+        mc = np.array([m[1] for m in measures] or [0], dtype=np.int32)
+        cap = int(max(1, min(shots, 1 << min(classical_bits, 62))))
+        keys = np.zeros(cap, dtype=np.uint64)
+        cnt = np.zeros(cap, dtype=np.uint64)
+        nout = ct.c_size_t(0)
+        i32p, u64p = ct.POINTER(ct.c_int32), ct.POINTER(ct.c_uint64)
+        _check(load().wqc_tn_sample_counts(self._h, mq.ctypes.data_as(i32p), mc.ctypes.data_as(i32p),
+                                           len(measures), classical_bits, shots, seed, method,
+                                           keys.ctypes.data_as(u64p), cnt.ctypes.data_as(u64p), cap,
+                                           ct.byref(nout)))
+        out = {}
+        for i in range(nout.value):
+            out[format(int(keys[i]), "0%db" % classical_bits)] = int(cnt[i])
+        return out
+
+    def observable_expectation(self, terms):
+        """Snapped sum of coeff * <P> over (label, coeff) terms, src/expectation.rs:170-200."""
+        labels = "".join(t[0] for t in terms).encode("ascii")
+        co = np.array([[complex(t[1]).real, complex(t[1]).imag] for t in terms], dtype=np.float64).ravel()
+        out = np.zeros(2)
+        _check(load().wqc_tn_observable_expectation(self._h, labels, _dp(co), len(terms), _dp(out)))
+        return complex(out[0], out[1])
 
     def bond_dims(self):
         out = (ct.c_size_t * self.qubit_count)()
