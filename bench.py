@@ -106,6 +106,21 @@ def run_gpu(wt, n, chi, gates, precision, steps, warmup, prefix_layers=24):
     return info
 
 
+def profiled_traffic(form):
+    """HBM bytes per Jacobi launch from the committed rocprofv3 counter summary of this form
+    (profiles/*_traffic.md, FETCH_SIZE + WRITE_SIZE scaled by a copy kernel), or None."""
+    import glob
+    import re
+
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_traffic.md"))):
+        text = open(path).read()
+        for block in re.findall(r"```json\s*(\{.*?\})\s*```", text, flags=re.S):
+            rec = json.loads(block)
+            if rec.get("form") == form and rec.get("workload") == "config3":
+                return rec["traffic_bytes_per_launch"], os.path.relpath(path, ROOT)
+    return None, None
+
+
 def rel(a, b):
     return abs(a - b) / abs(b) if b != 0 else float("inf")
 
@@ -149,7 +164,22 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-units", type=int, default=1)
     ap.add_argument("--form", default="both", choices=["both", "fp64", "mixed"])
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="run exactly one pass per form (for rocprofv3 counter runs), no JSON line")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="copy 256 MiB device to device three times: the reference kernel whose "
+                         "FETCH_SIZE / WRITE_SIZE readings scale the counter runs")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        x = torch.ones((256 << 20) // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print("# copied 268435456 bytes three times", flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding design is documented, not built)", file=sys.stderr)
@@ -162,6 +192,17 @@ def main():
     units = C.two_qubit_units(gates, n)
     print(f"# workload: {n}q depth {args.depth} chi {chi}: {units} two-qubit units", file=sys.stderr,
           flush=True)
+    if args.profile_pass:
+        for name, prec in (("fp64", wt.PRECISION_FP64), ("mixed", wt.PRECISION_MIXED_FP32)):
+            if args.form not in ("both", name):
+                continue
+            st = wt.MpsState(n, chi, prec)
+            t0 = time.perf_counter()
+            st.apply_circuit(gates)
+            amp = st.amplitude_at_compact_zero()
+            print(f"# {name}: one pass {time.perf_counter() - t0:.2f} s, amplitude {amp}", flush=True)
+            st.free()
+        return
     forms = {}
     for name, prec in (("fp64", wt.PRECISION_FP64), ("mixed", wt.PRECISION_MIXED_FP32)):
         if args.form not in ("both", name):
@@ -180,10 +221,13 @@ def main():
     jac_ms = kms.get("jacobi", 0.0)
     dom = max(kms, key=lambda k: kms[k]) if kms else "jacobi"
     achieved = svd / (jac_ms * 1e-3) / 1e12 if jac_ms > 0 else None
+    traffic, traffic_src = profiled_traffic(head)
     roofline = {
         "bound": "mfma", "kernel": "k_jacobi_round" + ("_lo + k_jacobi_round" if head == "mixed" else ""),
         "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-        "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None, "traffic": None,
+        "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None, "traffic": traffic,
+        "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE)",
+        "traffic_source": traffic_src,
         "launches": launches.get("jacobi"),
         "avg_launch_us": 1e3 * jac_ms / max(1, launches.get("jacobi", 1)),
         "dominant_family": dom,

@@ -1455,29 +1455,32 @@ struct wqc_tn_state {
         double2* cur = bufA;
         double2* nxt = bufB;
         std::vector<VecMatJob> jobs;
-        std::vector<int> outl;
+        std::vector<int> outl, inl;
         if (!right_to_left) {
             int vlen = 1;
             for (int i = from; i < to; ++i) {
-                VecMatJob v{sites[i].ptr, cur, nxt, sites[i].l, sites[i].r, vlen, phys, 0, 0};
+                VecMatJob v{sites[i].ptr, cur, nxt, sites[i].l, sites[i].r, vlen, phys, 0, 0, vm_part};
                 jobs.push_back(v);
                 outl.push_back(sites[i].r);
+                inl.push_back(std::min(sites[i].l, vlen));
                 vlen = sites[i].r;
                 std::swap(cur, nxt);
             }
         } else {
             int vlen = 1;
             for (int i = from; i > to; --i) {
-                VecMatJob v{sites[i].ptr, cur, nxt, sites[i].l, sites[i].r, vlen, phys, 1, 0};
+                VecMatJob v{sites[i].ptr, cur, nxt, sites[i].l, sites[i].r, vlen, phys, 1, 0, vm_part};
                 jobs.push_back(v);
                 outl.push_back(sites[i].l);
+                inl.push_back(std::min(sites[i].r, vlen));
                 vlen = sites[i].l;
                 std::swap(cur, nxt);
             }
         }
         if (!jobs.empty()) {
             VecMatJob* d = upload(jobs);
-            for (size_t k = 0; k < jobs.size(); ++k) wqc::launch_vecmat(d + k, outl[k], stream);
+            for (size_t k = 0; k < jobs.size(); ++k)
+                wqc::launch_vecmat(d + k, outl[k], inl[k], right_to_left ? 1 : 0, stream);
             check_launch();
             len = outl.back();
         }
@@ -1485,16 +1488,20 @@ struct wqc_tn_state {
     }
     static constexpr double2 kOne = {1.0, 0.0};
 
+    // Four readout vectors and the left pass's chunk partials (vm_part).
+    double2* vm_part = nullptr;
     void readout_buffers(double2*& a, double2*& b, double2*& c, double2*& d) {
         size_t mx = 2;
         for (auto& s : sites) mx = std::max(mx, (size_t)std::max(s.l, s.r) + 2);
-        ws.ensure(4 * mx * sizeof(double2) + 4096);
+        const size_t part = (mx / wqc::kVecMatChunk + 2) * mx;
+        ws.ensure((4 * mx + part) * sizeof(double2) + 4096);
         note_peak();
         double2* base = (double2*)ws.p;
         a = base;
         b = base + mx;
         c = base + 2 * mx;
         d = base + 3 * mx;
+        vm_part = base + 4 * mx;
     }
 
     std::pair<double, double> amplitude_zero() {
@@ -1537,13 +1544,13 @@ struct wqc_tn_state {
         double2 *u0 = scratch0, *u1 = nullptr;
         double2* freec = (re == c) ? d : c;
         u1 = freec;
-        VecMatJob j0{sites[t].ptr, le, u0, sites[t].l, sites[t].r, llen, 0, 0, 0};
-        VecMatJob j1{sites[t].ptr, le, u1, sites[t].l, sites[t].r, llen, 1, 0, 0};
+        VecMatJob j0{sites[t].ptr, le, u0, sites[t].l, sites[t].r, llen, 0, 0, 0, vm_part};
+        VecMatJob j1{sites[t].ptr, le, u1, sites[t].l, sites[t].r, llen, 1, 0, 0, vm_part};
         jobs.push_back(j0);
         jobs.push_back(j1);
         VecMatJob* dj = upload(jobs);
-        wqc::launch_vecmat(dj, sites[t].r, stream);
-        wqc::launch_vecmat(dj + 1, sites[t].r, stream);
+        wqc::launch_vecmat(dj, sites[t].r, std::min(sites[t].l, llen), 0, stream);
+        wqc::launch_vecmat(dj + 1, sites[t].r, std::min(sites[t].l, llen), 0, stream);
         check_launch();
         const int rr = sites[t].r;
         std::vector<double2> hu0(rr), hu1(rr), hre(std::max(rlen, 1));

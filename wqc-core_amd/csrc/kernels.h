@@ -95,11 +95,14 @@ struct HouseJob {
 // v_next[b] = sum_a v[a] * w(a, b) over one site (l, 2, r), for a < min(l, vlen), where
 // w = site[a,0,b] (phys = 0), site[a,1,b] (phys = 1) or their sum (phys = 2). With
 // right != 0 the contraction runs from the right: v_next[a] = sum_b w(a,b) v[b].
+// v' = v . site[:, s, :] (right = 0, length r) or site[:, s, :] . v (right = 1, length l), with
+// s = phys or the sum over both (phys = 2); `part` is scratch for the left pass's row chunks.
 struct VecMatJob {
     const double2* site;
     const double2* vin;
     double2* vout;
     int l, r, vlen, phys, right, pad;
+    double2* part;
 };
 
 void launch_one_qubit(const OneQubitJob* jobs, int njobs, int max_elems, hipStream_t s);
@@ -125,7 +128,8 @@ void launch_gemm(const GemmJob* jobs, int njobs, int max_m, int max_n, hipStream
 void launch_factor(const FactorJob* jobs, int njobs, long max_left, long max_right,
                    hipStream_t s);
 void launch_householder(const HouseJob* jobs, int njobs, hipStream_t s);
-void launch_vecmat(const VecMatJob* job, int out_len, hipStream_t s);
+constexpr int kVecMatChunk = 16;  // rows per workgroup of the left pass
+void launch_vecmat(const VecMatJob* job, int out_len, int in_len, int right, hipStream_t s);
 void launch_fill_int(int* p, int value, int count, hipStream_t s);
 void launch_conj_transpose(const double2* src, double2* dst, int rows, int cols, hipStream_t s);
 void launch_add_identity(double2* C, int rows, int cols, int ldc, hipStream_t s);

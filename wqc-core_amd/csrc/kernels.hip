@@ -1227,30 +1227,54 @@ __global__ __launch_bounds__(256) void k_householder(const HouseJob* __restrict_
 }
 
 // ------------------------------------------------------------------ vector x site
-__global__ __launch_bounds__(256) void k_vecmat(const VecMatJob* __restrict__ jobp, int out_len) {
+// Left pass: workgroup (x, y) sums rows y*kVecMatChunk.. of v[a] site[a, s, o] for 256
+// consecutive o (coalesced along the row); a second kernel adds the chunks in order.
+// Right pass: one wave per output row o, coalesced along the row, wave reduction.
+__global__ __launch_bounds__(256) void k_vecmat_left(const VecMatJob* __restrict__ jobp, int out_len) {
+    const VecMatJob jb = *jobp;
+    const int o = blockIdx.x * 256 + threadIdx.x;
+    if (o >= out_len) return;
+    const int lim = jb.l < jb.vlen ? jb.l : jb.vlen;
+    const int a0 = blockIdx.y * kVecMatChunk;
+    const int a1 = a0 + kVecMatChunk < lim ? a0 + kVecMatChunk : lim;
+    const long rs = 2L * jb.r;
+    double2 acc = cz();
+    for (int a = a0; a < a1; ++a) {
+        const double2* s = jb.site + a * rs + o;
+        double2 w = jb.phys == 1 ? s[jb.r] : s[0];
+        if (jb.phys == 2) w = cadd(w, s[jb.r]);
+        cmac(acc, jb.vin[a], w);
+    }
+    jb.part[(long)blockIdx.y * out_len + o] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_vecmat_sum(const VecMatJob* __restrict__ jobp, int out_len, int nchunk) {
     const VecMatJob jb = *jobp;
     const int o = blockIdx.x * 256 + threadIdx.x;
     if (o >= out_len) return;
     double2 acc = cz();
-    const long rs = 2L * jb.r;
-    if (!jb.right) {
-        const int lim = jb.l < jb.vlen ? jb.l : jb.vlen;
-        for (int a = 0; a < lim; ++a) {
-            const double2* s = jb.site + a * rs + o;
-            double2 w = jb.phys == 1 ? s[jb.r] : s[0];
-            if (jb.phys == 2) w = cadd(w, s[jb.r]);
-            cmac(acc, jb.vin[a], w);
-        }
-    } else {
-        const int lim = jb.r < jb.vlen ? jb.r : jb.vlen;
-        const double2* s = jb.site + (long)o * rs;
-        for (int b = 0; b < lim; ++b) {
-            double2 w = jb.phys == 1 ? s[jb.r + b] : s[b];
-            if (jb.phys == 2) w = cadd(w, s[jb.r + b]);
-            cmac(acc, w, jb.vin[b]);
-        }
-    }
+    for (int c = 0; c < nchunk; ++c) acc = cadd(acc, jb.part[(long)c * out_len + o]);
     jb.vout[o] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_vecmat_right(const VecMatJob* __restrict__ jobp, int out_len) {
+    const VecMatJob jb = *jobp;
+    const int o = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (o >= out_len) return;
+    const int lim = jb.r < jb.vlen ? jb.r : jb.vlen;
+    const double2* s = jb.site + (long)o * 2 * jb.r;
+    double re = 0.0, im = 0.0;
+    for (int b = lane; b < lim; b += 64) {
+        double2 w = jb.phys == 1 ? s[jb.r + b] : s[b];
+        if (jb.phys == 2) w = cadd(w, s[jb.r + b]);
+        const double2 v = jb.vin[b];
+        re = fma(w.x, v.x, fma(-w.y, v.y, re));
+        im = fma(w.x, v.y, fma(w.y, v.x, im));
+    }
+    re = wave_sum(re);
+    im = wave_sum(im);
+    if (lane == 0) jb.vout[o] = make_double2(re, im);
 }
 
 // dst (cols x rows) = src (rows x cols)^H
@@ -1464,8 +1488,14 @@ void launch_householder(const HouseJob* jobs, int njobs, hipStream_t s) {
     hipLaunchKernelGGL(k_householder, dim3(njobs), dim3(256), 0, s, jobs);
 }
 
-void launch_vecmat(const VecMatJob* job, int out_len, hipStream_t s) {
-    hipLaunchKernelGGL(k_vecmat, dim3((out_len + 255) / 256), dim3(256), 0, s, job, out_len);
+void launch_vecmat(const VecMatJob* job, int out_len, int in_len, int right, hipStream_t s) {
+    if (right) {
+        hipLaunchKernelGGL(k_vecmat_right, dim3((out_len + 3) / 4), dim3(256), 0, s, job, out_len);
+        return;
+    }
+    const int nchunk = in_len > 0 ? (in_len + kVecMatChunk - 1) / kVecMatChunk : 1;
+    hipLaunchKernelGGL(k_vecmat_left, dim3((out_len + 255) / 256, nchunk), dim3(256), 0, s, job, out_len);
+    hipLaunchKernelGGL(k_vecmat_sum, dim3((out_len + 255) / 256), dim3(256), 0, s, job, out_len, nchunk);
 }
 
 void launch_conj_transpose(const double2* src, double2* dst, int rows, int cols, hipStream_t s) {
