@@ -304,7 +304,7 @@ __device__ __forceinline__ d4v mfma_f64(double a, double b, d4v c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-__global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restrict__ jobs,
+__global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacobiJob* __restrict__ jobs,
                                                       int round, double tol, double nu2,
                                                       int inner_cap, int* __restrict__ rotated,
                                                       const int* __restrict__ done,
@@ -358,32 +358,47 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
         const int r0 = s_row[i], r1 = s_row[16 + i];
         const double2* x0 = r0 >= 0 ? jb.X + (long)r0 * jb.ldx : nullptr;
         const double2* x1 = r1 >= 0 ? jb.X + (long)r1 * jb.ldx : nullptr;
-        d4v a00r = {0, 0, 0, 0}, a00i = a00r, a01r = a00r, a01i = a00r, a11r = a00r, a11i = a00r;
-        for (int c0 = wave * 16; c0 < jb.m; c0 += 64) {
-            double2 u[4], v[4];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int c = c0 + 4 * g + s;
-                u[s] = (x0 && c < jb.m) ? x0[c] : cz();
-                v[s] = (x1 && c < jb.m) ? x1[c] : cz();
-            }
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                // G_IJ += P_I P_J^H: re = ar*br + ai*bi, im = ai*br - ar*bi
-                a00r = mfma_f64(u[s].x, u[s].x, a00r);
-                a00r = mfma_f64(u[s].y, u[s].y, a00r);
-                a00i = mfma_f64(u[s].y, u[s].x, a00i);
-                a00i = mfma_f64(-u[s].x, u[s].y, a00i);
-                a01r = mfma_f64(u[s].x, v[s].x, a01r);
-                a01r = mfma_f64(u[s].y, v[s].y, a01r);
-                a01i = mfma_f64(u[s].y, v[s].x, a01i);
-                a01i = mfma_f64(-u[s].x, v[s].y, a01i);
-                a11r = mfma_f64(v[s].x, v[s].x, a11r);
-                a11r = mfma_f64(v[s].y, v[s].y, a11r);
-                a11i = mfma_f64(v[s].y, v[s].x, a11i);
-                a11i = mfma_f64(-v[s].x, v[s].y, a11i);
-            }
+        // 3M products: T1 = sum x.re y.re, T2 = sum x.im y.im, T3 = sum (x.re + x.im)(y.re - y.im);
+        // Re G = T1 + T2, Im G = T3 - T1 + T2. Column chunks are double-buffered in registers.
+        d4v p00 = {0, 0, 0, 0}, q00 = p00, t00 = p00, p01 = p00, q01 = p00, t01 = p00, p11 = p00, q11 = p00, t11 = p00;
+        double2 ua[4], va[4], ub[4], vb[4];
+#define WQC_GRAM_LOAD(C0, U, V)                                  \
+    _Pragma("unroll") for (int s = 0; s < 4; ++s) {              \
+        const int c = (C0) + 4 * g + s;                          \
+        U[s] = (x0 && c < jb.m) ? x0[c] : cz();                  \
+        V[s] = (x1 && c < jb.m) ? x1[c] : cz();                  \
+    }
+#define WQC_GRAM_ACC(U, V)                                       \
+    _Pragma("unroll") for (int s = 0; s < 4; ++s) {              \
+        const double up = U[s].x + U[s].y, um = U[s].x - U[s].y; \
+        const double vp = V[s].x + V[s].y, vm = V[s].x - V[s].y; \
+        p00 = mfma_f64(U[s].x, U[s].x, p00);                     \
+        q00 = mfma_f64(U[s].y, U[s].y, q00);                     \
+        t00 = mfma_f64(up, um, t00);                             \
+        p01 = mfma_f64(U[s].x, V[s].x, p01);                     \
+        q01 = mfma_f64(U[s].y, V[s].y, q01);                     \
+        t01 = mfma_f64(up, vm, t01);                             \
+        p11 = mfma_f64(V[s].x, V[s].x, p11);                     \
+        q11 = mfma_f64(V[s].y, V[s].y, q11);                     \
+        t11 = mfma_f64(vp, vm, t11);                             \
+    }
+        int c0 = wave * 16;
+        if (c0 < jb.m) WQC_GRAM_LOAD(c0, ua, va)
+        while (c0 < jb.m) {
+            const int c1 = c0 + 64;
+            if (c1 < jb.m) WQC_GRAM_LOAD(c1, ub, vb)
+            WQC_GRAM_ACC(ua, va)
+            if (c1 >= jb.m) break;
+            const int c2 = c1 + 64;
+            if (c2 < jb.m) WQC_GRAM_LOAD(c2, ua, va)
+            WQC_GRAM_ACC(ub, vb)
+            c0 = c2;
         }
+#undef WQC_GRAM_LOAD
+#undef WQC_GRAM_ACC
+        d4v a00r = p00 + q00, a00i = t00 - p00 + q00;
+        d4v a01r = p01 + q01, a01i = t01 - p01 + q01;
+        d4v a11r = p11 + q11, a11i = t11 - p11 + q11;
         // Deterministic reduction over the waves: (w0 + w2) + (w1 + w3).
 #define WQC_GP_ALL(OP) OP(0, a00r) OP(1, a00i) OP(2, a01r) OP(3, a01i) OP(4, a11r) OP(5, a11i)
 #define WQC_GP_STORE(t, v) for (int j = 0; j < 4; ++j) Gp[slot][t][j][lane] = v[j];
@@ -559,7 +574,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
     // P[4kk + l/16][c0 + l%16]; output element j of lane l is local row I*16 + 4j + l/16.
     {
         const int i = lane & 15, g = lane >> 4;
-        double tr[2][8], ti[2][8], tni[2][8];
+        double tr[2][8], ti[2][8], ts[2][8];
 #pragma unroll
         for (int I = 0; I < 2; ++I)
 #pragma unroll
@@ -567,7 +582,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
                 const double2 t = Tp[I * 16 + i][4 * kk + g];
                 tr[I][kk] = t.x;
                 ti[I][kk] = t.y;
-                tni[I][kk] = -t.y;
+                ts[I][kk] = t.x + t.y;
             }
         int prow[8];
 #pragma unroll
@@ -577,37 +592,55 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacobiJob* __restric
         for (int I = 0; I < 2; ++I)
 #pragma unroll
             for (int j = 0; j < 4; ++j) orow[I][j] = s_row[I * 16 + 4 * j + g];
+        // 3M products: E1 = Tr Pr, E2 = Ti Pi, E3 = (Tr + Ti)(Pr + Pi); Re = E1 - E2,
+        // Im = E3 - E1 - E2. Column chunks are double-buffered in registers.
         for (int pass = 0; pass < 2; ++pass) {
             double2* M = pass == 0 ? jb.X : jb.J;
             if (!M) break;
             const int cols = pass == 0 ? jb.m : jb.n;
             const long ld = pass == 0 ? jb.ldx : jb.n;
-            for (int c0 = wave * 16; c0 < cols; c0 += 64) {
-                const int c = c0 + i;
-                const bool cok = c < cols;
-                double2 b[8];
-#pragma unroll
-                for (int kk = 0; kk < 8; ++kk)
-                    b[kk] = (cok && prow[kk] >= 0) ? M[(long)prow[kk] * ld + c] : cz();
-#pragma unroll
-                for (int I = 0; I < 2; ++I) {
-                    d4v dr = {0, 0, 0, 0}, di = dr;
-#pragma unroll
-                    for (int kk = 0; kk < 8; ++kk) {
-                        dr = mfma_f64(tr[I][kk], b[kk].x, dr);
-                        dr = mfma_f64(tni[I][kk], b[kk].y, dr);
-                        di = mfma_f64(tr[I][kk], b[kk].y, di);
-                        di = mfma_f64(ti[I][kk], b[kk].x, di);
-                    }
-                    if (cok) {
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) {
-                            const int gr = orow[I][j];
-                            if (gr >= 0) M[(long)gr * ld + c] = make_double2(dr[j], di[j]);
-                        }
-                    }
-                }
+            double2 ba[8], bb[8];
+#define WQC_APPLY_LOAD(C0, B)                                                        \
+    {                                                                                \
+        const int c = (C0) + i;                                                      \
+        _Pragma("unroll") for (int kk = 0; kk < 8; ++kk)                             \
+            B[kk] = (c < cols && prow[kk] >= 0) ? M[(long)prow[kk] * ld + c] : cz(); \
+    }
+#define WQC_APPLY_ROT(C0, B)                                                         \
+    {                                                                                \
+        const int c = (C0) + i;                                                      \
+        double bs[8];                                                                \
+        _Pragma("unroll") for (int kk = 0; kk < 8; ++kk) bs[kk] = B[kk].x + B[kk].y; \
+        _Pragma("unroll") for (int I = 0; I < 2; ++I) {                              \
+            d4v e1 = {0, 0, 0, 0}, e2 = e1, e3 = e1;                                 \
+            _Pragma("unroll") for (int kk = 0; kk < 8; ++kk) {                       \
+                e1 = mfma_f64(tr[I][kk], B[kk].x, e1);                               \
+                e2 = mfma_f64(ti[I][kk], B[kk].y, e2);                               \
+                e3 = mfma_f64(ts[I][kk], bs[kk], e3);                                \
+            }                                                                        \
+            if (c < cols) {                                                          \
+                _Pragma("unroll") for (int j = 0; j < 4; ++j) {                      \
+                    const int gr = orow[I][j];                                       \
+                    if (gr >= 0)                                                     \
+                        M[(long)gr * ld + c] = make_double2(e1[j] - e2[j], e3[j] - e1[j] - e2[j]); \
+                }                                                                    \
+            }                                                                        \
+        }                                                                            \
+    }
+            int c0 = wave * 16;
+            if (c0 < cols) WQC_APPLY_LOAD(c0, ba)
+            while (c0 < cols) {
+                const int c1 = c0 + 64;
+                if (c1 < cols) WQC_APPLY_LOAD(c1, bb)
+                WQC_APPLY_ROT(c0, ba)
+                if (c1 >= cols) break;
+                const int c2 = c1 + 64;
+                if (c2 < cols) WQC_APPLY_LOAD(c2, ba)
+                WQC_APPLY_ROT(c1, bb)
+                c0 = c2;
             }
+#undef WQC_APPLY_LOAD
+#undef WQC_APPLY_ROT
         }
     }
 }
