@@ -204,6 +204,22 @@ int wqc_tn_contract_to_statevector(wqc_tn_state *st, double *out, size_t len_com
  * character. */
 int wqc_tn_pauli_expectations(wqc_tn_state *st, const char *labels, size_t n_terms, double *out,
                               double *out_imag);
+/* Adjoint gradient of E(theta) = sum_k coeffs[k] <psi|P_k|psi> (real coefficients, Pauli
+ * labels as wqc_tn_pauli_expectations) with respect to every RX/RY/RZ angle of the circuit; no
+ * counterpart in the reference (its expectation mode has no gradients). Forward pass from
+ * |0...0> with the state saved at the end of every checkpoint_every-th run of one-qubit gates
+ * (0: every run); lambda = H psi as an MPO-MPS product (bond <= adjoint_bond, 0: 4 *
+ * max_bond_dim, exact splits only), carried back through the circuit; each run's forward state
+ * is replayed from its checkpoint, and a rotation exp(-i theta P / 2) gets
+ * dE/dtheta = Im <lambda| W P W^H |phi> (W: the later gates on its qubit in the run, phi and
+ * lambda at the run's end). Truncations of the forward pass enter as fixed projections (no
+ * derivative of the truncated SVD). grad[i] (count entries) is 0 for non-rotation gates;
+ * value = {Re E, Im E}. */
+int wqc_tn_backward(wqc_tn_device *dev, size_t qubit_count, size_t max_bond_dim, int precision,
+                    const wqc_tn_gate *gates, size_t count, const char *labels,
+                    const double *coeffs, size_t n_terms, size_t checkpoint_every,
+                    size_t adjoint_bond, double *grad, double value[2]);
+
 /* A batch of independent slices, contract_slice (src/tn/contract.rs:10-30) per slice: new
  * register, the slice's gates, <0...0|psi> into out_amps[2i], out_amps[2i+1]. Slices run
  * concurrently on `workers` host threads (<= 0: 8), each driving one state on its own stream,

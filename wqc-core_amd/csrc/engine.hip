@@ -22,6 +22,7 @@
 //   execute_with_trace apply loop     src/tn/trace.rs:71-161
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -380,6 +381,7 @@ struct wqc_tn_state {
     int n = 0;
     size_t chi = 0;
     bool reject = false;
+    bool exact_only = false;  // every split exact (the adjoint state of the gradient sweep)
     int precision = WQC_TN_PRECISION_FP64;
     std::vector<Site> sites;
     hipStream_t stream = nullptr;
@@ -879,7 +881,8 @@ struct wqc_tn_state {
             j.cols = 2 * j.dr;
             const size_t rows = j.rows, cols = j.cols;
             j.ell = std::min(std::min(chi + RSVD_OVERSAMPLE, rows), cols);
-            j.rsvd = !reject && !(j.ell == 0 || j.ell >= std::min(rows, cols) || !rsvd_worth(rows, cols, j.ell));
+            j.rsvd = !reject && !exact_only &&
+                     !(j.ell == 0 || j.ell >= std::min(rows, cols) || !rsvd_worth(rows, cols, j.ell));
             j.om = nullptr;
             j.fallback = false;
             const size_t n_ex = std::min(rows, cols);
@@ -1701,6 +1704,165 @@ struct wqc_tn_state {
         return std::vector<std::pair<uint64_t, uint64_t>>(counts.begin(), counts.end());
     }
 
+    // ------------------------------------------------------------ snapshots (gradient checkpoints)
+    struct Snapshot {
+        std::vector<double2*> ptr;
+        std::vector<int> l, r;
+        size_t bytes = 0;
+        void release() {
+            for (auto p : ptr)
+                if (p) (void)hipFree(p);
+            ptr.clear();
+            bytes = 0;
+        }
+    };
+    void snapshot(Snapshot& sn) {
+        flush();
+        sync();
+        sn.release();
+        sn.ptr.assign(n, nullptr);
+        sn.l.assign(n, 1);
+        sn.r.assign(n, 1);
+        for (int i = 0; i < n; ++i) {
+            const size_t cnt = (size_t)sites[i].l * 2 * sites[i].r;
+            HIPC(hipMalloc(&sn.ptr[i], cnt * sizeof(double2)));
+            HIPC(hipMemcpyAsync(sn.ptr[i], sites[i].ptr, cnt * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+            sn.l[i] = sites[i].l;
+            sn.r[i] = sites[i].r;
+            sn.bytes += cnt * sizeof(double2);
+        }
+        sync();
+    }
+    void restore(const Snapshot& sn) {
+        flush();
+        sync();
+        for (int i = 0; i < n; ++i) {
+            const size_t cnt = (size_t)sn.l[i] * 2 * sn.r[i];
+            ensure_site_cap(i, cnt);
+            HIPC(hipMemcpyAsync(sites[i].ptr, sn.ptr[i], cnt * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+            sites[i].l = sn.l[i];
+            sites[i].r = sn.r[i];
+        }
+        sync();
+    }
+
+    // ------------------------------------------------------------ MPO x MPS
+    // Sites of this state <- MPO x (sites of src). mpo[q] is D_q x D_{q+1} x 2 x 2 (row-major
+    // complex); the left boundary keeps MPO row 0 ("initial") and the right boundary column 1
+    // ("final").
+    void set_mpo_product(const wqc_tn_state& src, const std::vector<std::vector<double2>>& mpo,
+                         const std::vector<int>& D) {
+        flush();
+        sync();
+        size_t wmax = 1;
+        for (auto& w : mpo) wmax = std::max(wmax, w.size());
+        DevBuf wbuf;
+        wbuf.ensure(wmax * sizeof(double2));
+        for (int q = 0; q < n; ++q) {
+            const Site& a = src.sites[q];
+            const int w0 = q == 0 ? 0 : 0, wn = q == 0 ? 1 : D[q];
+            const int v0 = q == n - 1 ? 1 : 0, vn = q == n - 1 ? 1 : D[q + 1];
+            const size_t cnt = (size_t)a.l * wn * 2 * a.r * vn;
+            ensure_site_cap(q, cnt);
+            HIPC(hipMemcpyAsync(wbuf.p, mpo[q].data(), mpo[q].size() * sizeof(double2), hipMemcpyHostToDevice, stream));
+            wqc::launch_mpo_site(a.ptr, (const double2*)wbuf.p, sites[q].ptr, a.l, a.r, D[q], D[q + 1], w0, wn, v0, vn,
+                                 stream);
+            check_launch();
+            sites[q].l = a.l * wn;
+            sites[q].r = a.r * vn;
+            sync();
+        }
+        wbuf.release();
+    }
+
+    // ------------------------------------------------------------ <bra| O_q |ket>
+    // For single-site operators ops[k] = (site q, 2x2 O): <bra| O on qubit q |ket> with
+    // two-state left environments Lb_q = sum_s Bra_s^H Lb_{q-1} Ket_s and conjugated right ones;
+    // one transfer step through the operator site (bra transformed by O) and the closing sum.
+    std::vector<double2> overlap_ops(const wqc_tn_state& bra, const std::vector<std::pair<int, std::array<double2, 4>>>& ops) {
+        flush();
+        sync();
+        const wqc_tn_state& ket = *this;
+        size_t env = 0, maxbk = 1, maxs = 1;
+        std::vector<size_t> off(n + 1);
+        auto lb = [&](int q) { return q < n ? bra.sites[q].l : 1; };
+        auto lk = [&](int q) { return q < n ? ket.sites[q].l : 1; };
+        for (int q = 0; q <= n; ++q) {
+            off[q] = env;
+            env += (size_t)lb(q) * lk(q);
+            maxbk = std::max(maxbk, (size_t)std::max(lb(q), lk(q)));
+        }
+        for (int q = 0; q < n; ++q) maxs = std::max(maxs, (size_t)bra.sites[q].l * 2 * bra.sites[q].r);
+        const size_t fl = 2 * maxbk * maxbk;
+        const size_t nops = ops.size();
+        const size_t total = 2 * env + 2 * fl + nops * (maxs + 2 * fl) + nops + 64;
+        ews.ensure(total * sizeof(double2));
+        note_peak();
+        double2* L = (double2*)ews.p;
+        double2* Rb = L + env;
+        double2* FL = Rb + env;
+        double2* HR = FL + fl;
+        double2* opb = HR + fl;                       // transformed bra sites, one per op
+        double2* opf = opb + nops * maxs;             // F = L_q Ket_flat, one per op
+        double2* ope = opf + nops * fl;               // E' per op
+        double2* res = ope + nops * fl;
+        const double2 one = make_double2(1, 0), zero = make_double2(0, 0);
+        HIPC(hipMemcpyAsync(L + off[0], &kOne, sizeof(double2), hipMemcpyHostToDevice, stream));
+        HIPC(hipMemcpyAsync(Rb + off[n], &kOne, sizeof(double2), hipMemcpyHostToDevice, stream));
+        for (int i = 0; i < n; ++i) {
+            tables_reset();
+            const int ql = i, qr = n - 1 - i;
+            const Site &bl = bra.sites[ql], &kl = ket.sites[ql], &br = bra.sites[qr], &kr = ket.sites[qr];
+            std::vector<GemmJob> a{
+                GemmJob{L + off[ql], kl.ptr, FL, bl.l, 2 * kl.r, kl.l, kl.l, 2 * kl.r, 2 * kl.r, 0, 0, one, zero},
+                GemmJob{br.ptr, Rb + off[qr + 1], HR, 2 * br.l, kr.r, br.r, br.r, kr.r, kr.r, 0, 0, one, zero}};
+            std::vector<GemmJob> b{
+                GemmJob{bl.ptr, FL, L + off[ql + 1], bl.r, kl.r, 2 * bl.l, bl.r, kl.r, kl.r, 1, 0, one, zero},
+                GemmJob{HR, kr.ptr, Rb + off[qr], br.l, kr.l, 2 * kr.r, 2 * kr.r, 2 * kr.r, kr.l, 0, 1, one, zero}};
+            wqc::launch_gemm(upload(a), 2, std::max(bl.l, 2 * br.l), std::max(2 * kl.r, kr.r), stream);
+            wqc::launch_gemm(upload(b), 2, std::max(bl.r, br.l), std::max(kl.r, kr.l), stream);
+            check_launch();
+            sync();
+        }
+        std::vector<double2> out(nops, make_double2(0, 0));
+        if (!nops) return out;
+        tables_reset();
+        std::vector<wqc::OpSiteJob> oj;
+        std::vector<GemmJob> fa, fb;
+        std::vector<wqc::CloseJob> cj;
+        long max_half = 1;
+        int mam = 1, man = 1, mbm = 1, mbn = 1;
+        for (size_t k = 0; k < nops; ++k) {
+            const int q = ops[k].first;
+            const Site &bs = bra.sites[q], &ks = ket.sites[q];
+            wqc::OpSiteJob j;
+            j.A = bs.ptr;
+            j.B = opb + k * maxs;
+            j.l = bs.l;
+            j.r = bs.r;
+            for (int u = 0; u < 4; ++u) j.o[u] = ops[k].second[u];
+            oj.push_back(j);
+            max_half = std::max(max_half, (long)bs.l * bs.r);
+            double2* f = opf + k * fl;
+            double2* e = ope + k * fl;
+            fa.push_back(GemmJob{L + off[q], ks.ptr, f, bs.l, 2 * ks.r, ks.l, ks.l, 2 * ks.r, 2 * ks.r, 0, 0, one, zero});
+            fb.push_back(GemmJob{opb + k * maxs, f, e, bs.r, ks.r, 2 * bs.l, bs.r, ks.r, ks.r, 1, 0, one, zero});
+            cj.push_back(wqc::CloseJob{e, Rb + off[q + 1], res + k, (long)bs.r * ks.r});
+            mam = std::max(mam, bs.l);
+            man = std::max(man, 2 * ks.r);
+            mbm = std::max(mbm, bs.r);
+            mbn = std::max(mbn, ks.r);
+        }
+        wqc::launch_op_site(upload(oj), (int)nops, max_half, stream);
+        wqc::launch_gemm(upload(fa), (int)nops, mam, man, stream);
+        wqc::launch_gemm(upload(fb), (int)nops, mbm, mbn, stream);
+        wqc::launch_env_close(upload(cj), (int)nops, stream);
+        check_launch();
+        HIPC(hipMemcpyAsync(out.data(), res, nops * sizeof(double2), hipMemcpyDeviceToHost, stream));
+        sync();
+        return out;
+    }
+
     // <psi|P_k|psi> of Pauli strings by transfer matrices. The reference evaluates the same
     // sum over the dense statevector, sum_i conj(psi_P(i)) phase_i psi_i
     // (src/expectation.rs:140-167, n <= 20 through contract_to_statevector); here any n.
@@ -2180,6 +2342,234 @@ int wqc_tn_run_slices(wqc_tn_device* dev, const wqc_tn_slice* slices, size_t n_s
         worker();
         for (auto& th : pool) th.join();
         if (!first_error.empty()) fail(WQC_TN_EXEC_FAILED, first_error);
+    });
+}
+
+namespace {
+
+// Automaton MPO of sum_j c_j P_j for Pauli strings in Qiskit order (site q = character n-1-q).
+// The bond at cut c (left of site q = c) is 2 + the number of terms whose support spans the
+// cut: state 0 = nothing placed yet, 1 = term complete, 2 + k = k-th spanning term in progress.
+// W[q] is D[q] x D[q+1] x 2 x 2 row-major; the coefficient sits on the term's first site.
+void pauli_sum_mpo(int n, const char* labels, const double* coeffs, size_t nt,
+                   std::vector<std::vector<double2>>& W, std::vector<int>& D) {
+    std::vector<int> f(nt, 0), l(nt, 0);
+    for (size_t j = 0; j < nt; ++j) {
+        int lo = -1, hi = -1;
+        for (int q = 0; q < n; ++q)
+            if (labels[j * (size_t)n + (n - 1 - q)] != 'I') {
+                if (lo < 0) lo = q;
+                hi = q;
+            }
+        f[j] = lo < 0 ? 0 : lo;
+        l[j] = lo < 0 ? 0 : hi;
+    }
+    std::vector<std::map<size_t, int>> pos(n + 1);
+    D.assign(n + 1, 2);
+    for (int c = 0; c <= n; ++c)
+        for (size_t j = 0; j < nt; ++j)
+            if (f[j] < c && c <= l[j]) pos[c][j] = D[c]++;
+    auto pauli = [](char ch, double2 m[4]) {
+        for (int i = 0; i < 4; ++i) m[i] = make_double2(0, 0);
+        switch (ch) {
+            case 'X': m[1] = m[2] = make_double2(1, 0); break;
+            case 'Y': m[1] = make_double2(0, -1); m[2] = make_double2(0, 1); break;
+            case 'Z': m[0] = make_double2(1, 0); m[3] = make_double2(-1, 0); break;
+            default: m[0] = m[3] = make_double2(1, 0); break;
+        }
+    };
+    W.assign(n, {});
+    for (int q = 0; q < n; ++q) {
+        const int Dl = D[q], Dr = D[q + 1];
+        auto& w = W[q];
+        w.assign((size_t)Dl * Dr * 4, make_double2(0, 0));
+        auto add = [&](int a, int b, const double2 m[4], double c) {
+            for (int i = 0; i < 4; ++i) {
+                w[((size_t)a * Dr + b) * 4 + i].x += c * m[i].x;
+                w[((size_t)a * Dr + b) * 4 + i].y += c * m[i].y;
+            }
+        };
+        double2 id[4];
+        pauli('I', id);
+        add(0, 0, id, 1.0);
+        add(1, 1, id, 1.0);
+        for (size_t j = 0; j < nt; ++j) {
+            double2 m[4];
+            pauli(labels[j * (size_t)n + (n - 1 - q)], m);
+            if (f[j] == q && l[j] == q) add(0, 1, m, coeffs[j]);
+            else if (f[j] == q && l[j] > q) add(0, pos[q + 1][j], m, coeffs[j]);
+            else if (f[j] < q && q < l[j]) add(pos[q][j], pos[q + 1][j], m, 1.0);
+            else if (f[j] < q && q == l[j]) add(pos[q][j], 1, m, 1.0);
+        }
+    }
+}
+
+void mat2_mul(const double2 a[4], const double2 b[4], double2 c[4]) {
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            double2 acc = make_double2(0, 0);
+            for (int k = 0; k < 2; ++k) {
+                const double2 x = a[2 * i + k], y = b[2 * k + j];
+                acc.x += x.x * y.x - x.y * y.y;
+                acc.y += x.x * y.y + x.y * y.x;
+            }
+            c[2 * i + j] = acc;
+        }
+}
+
+void mat2_adj(const double2 a[4], double2 c[4]) {
+    c[0] = make_double2(a[0].x, -a[0].y);
+    c[1] = make_double2(a[2].x, -a[2].y);
+    c[2] = make_double2(a[1].x, -a[1].y);
+    c[3] = make_double2(a[3].x, -a[3].y);
+}
+
+bool is_rotation(int k) { return k == WQC_TN_GATE_RX || k == WQC_TN_GATE_RY || k == WQC_TN_GATE_RZ; }
+bool is_noop(int k) { return k == WQC_TN_GATE_MEASURE || k == WQC_TN_GATE_RESET || k == WQC_TN_GATE_IF; }
+
+}  // namespace
+
+int wqc_tn_backward(wqc_tn_device* dev, size_t qubit_count, size_t max_bond_dim, int precision,
+                    const wqc_tn_gate* gates, size_t count, const char* labels, const double* coeffs,
+                    size_t n_terms, size_t checkpoint_every, size_t adjoint_bond, double* grad,
+                    double* value) {
+    return guarded([&] {
+        if (!dev || !grad || !value || (count && !gates) || !labels || !coeffs) fail(WQC_TN_INVALID_ARG, "null argument");
+        if (qubit_count == 0) fail(WQC_TN_INVALID_ARG, "invalid qubit count: 0");
+        if (max_bond_dim == 0) fail(WQC_TN_EXEC_FAILED, "MPS max bond dimension must be > 0");
+        if (!n_terms) fail(WQC_TN_INVALID_ARG, "observable must have at least one term");
+        if (precision != WQC_TN_PRECISION_FP64 && precision != WQC_TN_PRECISION_MIXED_FP32)
+            fail(WQC_TN_INVALID_ARG, "unknown precision mode");
+        const int n = (int)qubit_count;
+        for (size_t j = 0; j < n_terms * qubit_count; ++j) {
+            const char ch = labels[j];
+            if (ch != 'I' && ch != 'X' && ch != 'Y' && ch != 'Z')
+                fail(WQC_TN_INVALID_ARG, "invalid pauli character in label " + std::to_string(j / qubit_count));
+        }
+        HIPC(hipSetDevice(dev->ordinal));
+        // Runs of one-qubit gates and the multi-qubit blocks after them: run r = [rb[r], re[r]),
+        // block r = [re[r], rb[r+1]) (no-op gates are skipped where they stand).
+        std::vector<size_t> rb, re;
+        {
+            size_t i = 0;
+            while (i <= count) {
+                rb.push_back(i);
+                while (i < count && (is_one_qubit(gates[i].kind) || is_noop(gates[i].kind))) ++i;
+                re.push_back(i);
+                while (i < count && !is_one_qubit(gates[i].kind)) ++i;
+                if (i >= count) break;
+            }
+        }
+        const size_t R = rb.size();
+        auto block_end = [&](size_t r) { return r + 1 < R ? rb[r + 1] : count; };
+        const size_t k = checkpoint_every ? checkpoint_every : 1;
+        auto phi = std::make_unique<wqc_tn_state>();
+        phi->init(dev, qubit_count, max_bond_dim, precision);
+        auto apply = [&](wqc_tn_state& st, size_t b, size_t e) {
+            for (size_t i = b; i < e; ++i)
+                if (!is_noop(gates[i].kind)) st.queue_gate(gates[i]);
+            st.flush();
+        };
+        std::vector<wqc_tn_state::Snapshot> ck(R);
+        struct Guard {
+            std::vector<wqc_tn_state::Snapshot>& v;
+            ~Guard() {
+                for (auto& s : v) s.release();
+            }
+        } guard{ck};
+        // Forward with checkpoints at the end of every k-th run.
+        for (size_t r = 0; r < R; ++r) {
+            apply(*phi, rb[r], re[r]);
+            if (r % k == 0) phi->snapshot(ck[r]);
+            apply(*phi, re[r], block_end(r));
+        }
+        // Value: sum_j c_j <psi|P_j|psi>.
+        {
+            std::vector<double> vr(n_terms), vi(n_terms);
+            phi->pauli_expectations(labels, n_terms, vr.data(), vi.data());
+            double a = 0, b = 0;
+            for (size_t j = 0; j < n_terms; ++j) {
+                a += coeffs[j] * vr[j];
+                b += coeffs[j] * vi[j];
+            }
+            value[0] = a;
+            value[1] = b;
+        }
+        // lambda = H psi through the MPO, then carried back through the circuit with exact splits.
+        std::vector<std::vector<double2>> W;
+        std::vector<int> D;
+        pauli_sum_mpo(n, labels, coeffs, n_terms, W, D);
+        size_t dmax = 1;
+        for (int d : D) dmax = std::max(dmax, (size_t)d);
+        const size_t lam_bond = adjoint_bond ? adjoint_bond : 4 * max_bond_dim;
+        auto lam = std::make_unique<wqc_tn_state>();
+        lam->init(dev, qubit_count, lam_bond, WQC_TN_PRECISION_FP64);
+        lam->exact_only = true;
+        lam->set_mpo_product(*phi, W, D);
+        for (size_t i = 0; i < count; ++i) grad[i] = 0.0;
+        for (size_t rr = R; rr-- > 0;) {
+            // lambda at the end of run rr: undo block rr (CNOT, CZ and CCNOT are self-inverse).
+            for (size_t i = block_end(rr); i-- > re[rr];)
+                if (!is_noop(gates[i].kind)) lam->queue_gate(gates[i]);
+            lam->flush();
+            // phi at the end of run rr: replay from its checkpoint.
+            const size_t c = (rr / k) * k;
+            phi->restore(ck[c]);
+            for (size_t x = c; x < rr; ++x) {
+                apply(*phi, re[x], block_end(x));
+                apply(*phi, rb[x + 1], re[x + 1]);
+            }
+            // dE/dtheta = Im <lambda| W P W^H |phi> for the run's rotations, W = the later gates
+            // on the same qubit within the run.
+            std::vector<std::pair<int, std::array<double2, 4>>> ops;
+            std::vector<size_t> which;
+            for (size_t i = rb[rr]; i < re[rr]; ++i) {
+                const wqc_tn_gate& g = gates[i];
+                if (!is_rotation(g.kind)) continue;
+                double2 P[4];
+                for (int u = 0; u < 4; ++u) P[u] = make_double2(0, 0);
+                if (g.kind == WQC_TN_GATE_RX) P[1] = P[2] = make_double2(1, 0);
+                if (g.kind == WQC_TN_GATE_RY) {
+                    P[1] = make_double2(0, -1);
+                    P[2] = make_double2(0, 1);
+                }
+                if (g.kind == WQC_TN_GATE_RZ) {
+                    P[0] = make_double2(1, 0);
+                    P[3] = make_double2(-1, 0);
+                }
+                double2 Wm[4] = {make_double2(1, 0), make_double2(0, 0), make_double2(0, 0), make_double2(1, 0)};
+                for (size_t j = i + 1; j < re[rr]; ++j) {
+                    if (is_noop(gates[j].kind) || gates[j].q0 != g.q0) continue;
+                    Mat2 u;
+                    unary_matrix(gates[j].kind, gates[j].angle, u);
+                    double2 t[4];
+                    mat2_mul(u, Wm, t);
+                    for (int x = 0; x < 4; ++x) Wm[x] = t[x];
+                }
+                double2 Wa[4], t[4], O[4];
+                mat2_adj(Wm, Wa);
+                mat2_mul(Wm, P, t);
+                mat2_mul(t, Wa, O);
+                std::array<double2, 4> o;
+                for (int x = 0; x < 4; ++x) o[x] = O[x];
+                ops.emplace_back(g.q0, o);
+                which.push_back(i);
+            }
+            if (!ops.empty()) {
+                auto vals = phi->overlap_ops(*lam, ops);
+                for (size_t x = 0; x < which.size(); ++x) grad[which[x]] = vals[x].y;
+            }
+            // lambda at the start of run rr: undo its one-qubit gates.
+            for (size_t i = re[rr]; i-- > rb[rr];) {
+                const wqc_tn_gate& g = gates[i];
+                if (is_noop(g.kind)) continue;
+                Mat2 u, ua;
+                unary_matrix(g.kind, g.angle, u);
+                mat2_adj(u, ua);
+                lam->queue_one(g.q0, ua);
+            }
+            lam->flush();
+        }
     });
 }
 

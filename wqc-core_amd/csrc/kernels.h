@@ -149,6 +149,20 @@ void launch_pauli_site(const double2* A, double2* bx, double2* by, double2* bz, 
                        hipStream_t s);
 void launch_env_close(const CloseJob* jobs, int njobs, hipStream_t s);
 
+// MPO x MPS site: out[(a, w), t, (b, v)] = sum_s W[w][v][t][s] A[a, s, b] for MPO bond indices
+// w in [w0, w0 + wn) of Dl and v in [v0, v0 + vn) of Dr (W is Dl x Dr x 2 x 2, row-major);
+// the combined bond index is mps_index * wn + (w - w0). out is (l*wn) x 2 x (r*vn).
+void launch_mpo_site(const double2* A, const double2* W, double2* out, int l, int r, int Dl, int Dr,
+                     int w0, int wn, int v0, int vn, hipStream_t s);
+// Bra transform for a single-site operator O (2x2 row-major): B_t = sum_s' conj(O[s', t]) A_s'.
+struct OpSiteJob {
+    const double2* A;
+    double2* B;
+    int l, r;
+    double2 o[4];
+};
+void launch_op_site(const OpSiteJob* jobs, int njobs, long max_half, hipStream_t s);
+
 // One site of the MPS sampler for `shots` shots: W (shots x 2r) holds v A_s for s = 0, 1,
 // Z (two blocks of shots x r) holds R (v A_s)^T; picks s with the uniform u[k * ustride],
 // sets bit `bit` of bits[k], writes v' = (v A_s) / sqrt(p_s) into vnext (shots x r).

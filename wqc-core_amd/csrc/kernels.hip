@@ -1375,6 +1375,45 @@ __global__ __launch_bounds__(256) void k_env_close(const CloseJob* __restrict__ 
     if (threadIdx.x == 0) *jb.out = red[0];
 }
 
+// ------------------------------------------------------------------ MPO x MPS, operator sites
+__global__ __launch_bounds__(256) void k_mpo_site(const double2* __restrict__ A, const double2* __restrict__ W,
+                                                  double2* __restrict__ out, int l, int r, int Dl, int Dr,
+                                                  int w0, int wn, int v0, int vn) {
+    const long total = (long)l * wn * 2 * r * vn;
+    const long ro = (long)r * vn;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+        const long row = e / (2 * ro), rem = e % (2 * ro);
+        const int t = (int)(rem / ro);
+        const long col = rem % ro;
+        const int a = (int)(row / wn), w = w0 + (int)(row % wn);
+        const int b = (int)(col / vn), v = v0 + (int)(col % vn);
+        const double2* wv = W + (((long)w * Dr + v) * 2 + t) * 2;
+        const double2 x0 = A[(long)a * 2 * r + b], x1 = A[(long)a * 2 * r + r + b];
+        double2 acc = cz();
+        cmac(acc, wv[0], x0);
+        cmac(acc, wv[1], x1);
+        out[e] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_op_site(const OpSiteJob* __restrict__ jobs) {
+    const OpSiteJob jb = jobs[blockIdx.y];
+    const long half = (long)jb.l * jb.r;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < half; e += (long)gridDim.x * 256) {
+        const long a = e / jb.r, b = e % jb.r;
+        const long i0 = a * 2 * jb.r + b, i1 = i0 + jb.r;
+        const double2 v0 = jb.A[i0], v1 = jb.A[i1];
+        double2 b0 = cz(), b1 = cz();
+        // B_0 = conj(O00) A_0 + conj(O10) A_1, B_1 = conj(O01) A_0 + conj(O11) A_1
+        ccmac(b0, jb.o[0], v0);
+        ccmac(b0, jb.o[2], v1);
+        ccmac(b1, jb.o[1], v0);
+        ccmac(b1, jb.o[3], v1);
+        jb.B[i0] = b0;
+        jb.B[i1] = b1;
+    }
+}
+
 // ------------------------------------------------------------------ MPS sampler step
 // One wave per shot: p_s = Re sum_b conj(W_s[b]) Z_s[b] = <v A_s| R |v A_s>, the unnormalized
 // probability of outcome s given the shot's earlier outcomes; s = 0 when u < p_0 / (p_0 + p_1).
@@ -1493,6 +1532,17 @@ void launch_pauli_site(const double2* A, double2* bx, double2* by, double2* bz, 
                        hipStream_t s) {
     const long half = (long)l * r;
     hipLaunchKernelGGL(k_pauli_site, dim3(blocks_for(half, 256, 4096)), dim3(256), 0, s, A, bx, by, bz, l, r);
+}
+
+void launch_mpo_site(const double2* A, const double2* W, double2* out, int l, int r, int Dl, int Dr,
+                     int w0, int wn, int v0, int vn, hipStream_t s) {
+    const long total = (long)l * wn * 2 * r * vn;
+    hipLaunchKernelGGL(k_mpo_site, dim3(blocks_for(total, 256, 4096)), dim3(256), 0, s, A, W, out, l, r, Dl,
+                       Dr, w0, wn, v0, vn);
+}
+
+void launch_op_site(const OpSiteJob* jobs, int njobs, long max_half, hipStream_t s) {
+    hipLaunchKernelGGL(k_op_site, dim3(blocks_for(max_half, 256, 1024), njobs), dim3(256), 0, s, jobs);
 }
 
 void launch_sample_step(const double2* W, const double2* Z, int r, long shots, const double* u,

@@ -32,7 +32,7 @@ EXPORTED = (
     "wqc_tn_peak_vram_bytes", "wqc_tn_get_stats", "wqc_tn_kernel_times", "wqc_tn_enable_timing",
     "wqc_tn_kernel_launches", "wqc_tn_enable_split_log", "wqc_tn_split_log",
     "wqc_tn_pauli_expectations", "wqc_tn_observable_expectation", "wqc_tn_sample_counts",
-    "wqc_tn_run_slices",
+    "wqc_tn_run_slices", "wqc_tn_backward",
 )
 
 
@@ -103,6 +103,8 @@ def load():
     L.wqc_tn_pauli_expectations.argtypes = [vp, ct.c_char_p, sz, dp, dp]
     L.wqc_tn_observable_expectation.argtypes = [vp, ct.c_char_p, dp, sz, dp]
     i32p, u64p = ct.POINTER(ct.c_int32), ct.POINTER(ct.c_uint64)
+    L.wqc_tn_backward.argtypes = [vp, sz, sz, ct.c_int, ct.c_void_p, sz, ct.c_char_p, dp, sz, sz, sz,
+                                  dp, dp]
     L.wqc_tn_run_slices.argtypes = [vp, ct.POINTER(Slice), sz, ct.c_void_p, ct.c_int, ct.c_int, dp,
                                     ct.POINTER(ct.c_int)]
     L.wqc_tn_sample_counts.argtypes = [vp, i32p, i32p, sz, sz, ct.c_uint64, ct.c_uint64, ct.c_int,
@@ -189,6 +191,23 @@ def run_slices(slices, precision: int = PRECISION_FP64, workers: int = 8, device
     if rc != OK and all(c == OK for c in codes):
         _check(rc)
     return out, codes
+
+
+def backward(qubit_count, max_bond_dim, gates, terms, checkpoint_every=2, adjoint_bond=0,
+             precision: int = PRECISION_FP64, device=None):
+    """Adjoint gradient of sum_k c_k <psi|P_k|psi> over (label, real coeff) terms with respect
+    to every RX/RY/RZ angle; returns (value, grad) with grad[i] for gates[i]."""
+    L = load()
+    dev = device or default_device()
+    garr = gate_array(gates)
+    labels = "".join(t[0] for t in terms).encode("ascii")
+    co = np.ascontiguousarray([float(t[1]) for t in terms], dtype=np.float64)
+    grad = np.zeros(max(1, len(gates)))
+    val = np.zeros(2)
+    _check(L.wqc_tn_backward(dev._h, qubit_count, max_bond_dim, precision, ct.cast(garr, ct.c_void_p),
+                             len(gates), labels, _dp(co), len(terms), checkpoint_every, adjoint_bond,
+                             _dp(grad), _dp(val)))
+    return complex(val[0], val[1]), grad[: len(gates)]
 
 
 class MpsState:
