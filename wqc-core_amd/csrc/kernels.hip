@@ -1061,63 +1061,110 @@ __global__ __launch_bounds__(256) void k_row_norms(const JacobiJob* __restrict__
 }
 
 // ------------------------------------------------------------------ batched ZGEMM
-constexpr int GT = 32;
-constexpr int GK = 16;
+// ------------------------------------------------------------------ MFMA GEMM
+// Batched complex FP64 GEMM C = alpha op(A) op(B) + beta C on v_mfma_f64_16x16x4f64 with 3M
+// products (E1 = Ar Br, E2 = Ai Bi, E3 = (Ar + Ai)(Br + Bi); Re = E1 - E2, Im = E3 - E1 - E2).
+// A workgroup computes a 64 x 64 tile, each wave a 32 x 32 quarter (2 x 2 MFMA tiles); K advances
+// in steps of 16 staged through LDS as real / imaginary planes (op C transposes and conjugates
+// while staging). Edges are zero-padded.
+constexpr int MG = 64, MGK = 16;
 
-__global__ __launch_bounds__(256) void k_gemm(const GemmJob* __restrict__ jobs) {
+__global__ __launch_bounds__(256, 2) void k_gemm_mfma(const GemmJob* __restrict__ jobs) {
     const GemmJob jb = jobs[blockIdx.z];
-    const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+    const int m0 = blockIdx.y * MG, n0 = blockIdx.x * MG;
     if (m0 >= jb.M || n0 >= jb.N) return;
-    __shared__ double2 As[GT][GK + 1];
-    __shared__ double2 Bs[GK][GT + 1];
-    const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
-    double2 acc[2][2] = {{cz(), cz()}, {cz(), cz()}};
-    for (int k0 = 0; k0 < jb.K; k0 += GK) {
-        if (jb.opA == 0) {
-            for (int e = tid; e < GT * GK; e += 256) {
-                const int mm = e / GK, kk = e % GK, m = m0 + mm, k = k0 + kk;
-                As[mm][kk] = (m < jb.M && k < jb.K) ? jb.A[(long)m * jb.lda + k] : cz();
-            }
-        } else {
-            for (int e = tid; e < GT * GK; e += 256) {
-                const int kk = e / GT, mm = e % GT, m = m0 + mm, k = k0 + kk;
-                As[mm][kk] = (m < jb.M && k < jb.K) ? cconj(jb.A[(long)k * jb.lda + m]) : cz();
-            }
+    __shared__ double Ar[MG][MGK + 1], Ai[MG][MGK + 1], Br[MGK][MG + 1], Bi[MGK][MG + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    d4v e1[2][2], e2[2][2], e3[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            e1[a][b] = d4v{0, 0, 0, 0};
+            e2[a][b] = e1[a][b];
+            e3[a][b] = e1[a][b];
         }
-        if (jb.opB == 0) {
-            for (int e = tid; e < GK * GT; e += 256) {
-                const int kk = e / GT, nn = e % GT, n = n0 + nn, k = k0 + kk;
-                Bs[kk][nn] = (n < jb.N && k < jb.K) ? jb.B[(long)k * jb.ldb + n] : cz();
+    for (int k0 = 0; k0 < jb.K; k0 += MGK) {
+#pragma unroll
+        for (int u = 0; u < (MG * MGK) / 256; ++u) {
+            const int e = tid + 256 * u;
+            double2 v;
+            int mm, kk;
+            if (jb.opA == 0) {
+                mm = e / MGK;
+                kk = e % MGK;
+                const int m = m0 + mm, k = k0 + kk;
+                v = (m < jb.M && k < jb.K) ? jb.A[(long)m * jb.lda + k] : cz();
+            } else {
+                kk = e / MG;
+                mm = e % MG;
+                const int m = m0 + mm, k = k0 + kk;
+                v = (m < jb.M && k < jb.K) ? cconj(jb.A[(long)k * jb.lda + m]) : cz();
             }
-        } else {
-            for (int e = tid; e < GK * GT; e += 256) {
-                const int nn = e / GK, kk = e % GK, n = n0 + nn, k = k0 + kk;
-                Bs[kk][nn] = (n < jb.N && k < jb.K) ? cconj(jb.B[(long)n * jb.ldb + k]) : cz();
+            Ar[mm][kk] = v.x;
+            Ai[mm][kk] = v.y;
+        }
+#pragma unroll
+        for (int u = 0; u < (MG * MGK) / 256; ++u) {
+            const int e = tid + 256 * u;
+            double2 v;
+            int nn, kk;
+            if (jb.opB == 0) {
+                kk = e / MG;
+                nn = e % MG;
+                const int n = n0 + nn, k = k0 + kk;
+                v = (n < jb.N && k < jb.K) ? jb.B[(long)k * jb.ldb + n] : cz();
+            } else {
+                nn = e / MGK;
+                kk = e % MGK;
+                const int n = n0 + nn, k = k0 + kk;
+                v = (n < jb.N && k < jb.K) ? cconj(jb.B[(long)n * jb.ldb + k]) : cz();
             }
+            Br[kk][nn] = v.x;
+            Bi[kk][nn] = v.y;
         }
         __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < GK; ++kk) {
-            const double2 a0 = As[ty][kk], a1 = As[ty + 16][kk];
-            const double2 b0 = Bs[kk][tx], b1 = Bs[kk][tx + 16];
-            cmac(acc[0][0], a0, b0);
-            cmac(acc[0][1], a0, b1);
-            cmac(acc[1][0], a1, b0);
-            cmac(acc[1][1], a1, b1);
+#pragma unroll
+        for (int ks = 0; ks < MGK / 4; ++ks) {
+            const int kk = 4 * ks + lk;
+            double ar[2], ai[2], as[2], br[2], bi[2], bs[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                ar[a] = Ar[wm + 16 * a + li][kk];
+                ai[a] = Ai[wm + 16 * a + li][kk];
+                as[a] = ar[a] + ai[a];
+                br[a] = Br[kk][wn + 16 * a + li];
+                bi[a] = Bi[kk][wn + 16 * a + li];
+                bs[a] = br[a] + bi[a];
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    e1[a][b] = mfma_f64(ar[a], br[b], e1[a][b]);
+                    e2[a][b] = mfma_f64(ai[a], bi[b], e2[a][b]);
+                    e3[a][b] = mfma_f64(as[a], bs[b], e3[a][b]);
+                }
         }
         __syncthreads();
     }
+    // MFMA f64 output: element j of lane l is row 4j + l/16, column l%16 of the 16 x 16 tile.
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int m = m0 + ty + 16 * i, n = n0 + tx + 16 * j;
-            if (m >= jb.M || n >= jb.N) continue;
-            double2* c = jb.C + (long)m * jb.ldc + n;
-            double2 v = cmul(jb.alpha, acc[i][j]);
-            if (jb.beta.x != 0.0 || jb.beta.y != 0.0) cmac(v, jb.beta, *c);
-            *c = v;
-        }
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int m = m0 + wm + 16 * a + 4 * j + lk, n = n0 + wn + 16 * b + li;
+                if (m >= jb.M || n >= jb.N) continue;
+                const double re = e1[a][b][j] - e2[a][b][j], im = e3[a][b][j] - e1[a][b][j] - e2[a][b][j];
+                double2* c = jb.C + (long)m * jb.ldc + n;
+                double2 v = cmul(jb.alpha, make_double2(re, im));
+                if (jb.beta.x != 0.0 || jb.beta.y != 0.0) cmac(v, jb.beta, *c);
+                *c = v;
+            }
 }
 
 // ------------------------------------------------------------------ factors
@@ -1557,8 +1604,8 @@ void launch_env_close(const CloseJob* jobs, int njobs, hipStream_t s) {
 }
 
 void launch_gemm(const GemmJob* jobs, int njobs, int max_m, int max_n, hipStream_t s) {
-    dim3 grid((max_n + GT - 1) / GT, (max_m + GT - 1) / GT, njobs);
-    hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, s, jobs);
+    dim3 grid((max_n + MG - 1) / MG, (max_m + MG - 1) / MG, njobs);
+    hipLaunchKernelGGL(k_gemm_mfma, grid, dim3(256), 0, s, jobs);
 }
 
 void launch_factor(const FactorJob* jobs, int njobs, long max_left, long max_right,
