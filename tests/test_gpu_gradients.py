@@ -149,7 +149,7 @@ def test_config5_chi64_vs_finite_differences_and_shift(wt):
     rot = [i for i, g in enumerate(gates) if g[0] in (C.RY, C.RZ)]
     h = 1e-4
     worst_fd = worst_ps = worst_fd_ps = 0.0
-    for i in rot[:: len(rot) // 16]:
+    for i in rot[:: len(rot) // 6]:
         fd = (gpu_energy(wt, n, chi, shifted(gates, i, h), terms)
               - gpu_energy(wt, n, chi, shifted(gates, i, -h), terms)) / (2 * h)
         ps = 0.5 * (gpu_energy(wt, n, chi, shifted(gates, i, math.pi / 2), terms)

@@ -100,10 +100,12 @@ const int MAX_SWEEPS = 60;
 // Rows whose squared norm is below NULL_ROWS_SQ / n are not rotated: together they carry at
 // most 1e-13 of singular value, an order below SINGULAR_CUTOFF.
 const double NULL_ROWS_SQ = 1e-26;
-// Relative orthogonality tolerances of the Jacobi sweeps, times sqrt(row length): an order
-// above the rounding of an FP64 (FP32) Gram entry over that many terms.
+// Relative orthogonality tolerances of the Jacobi sweeps, times sqrt(row length): for FP64 an
+// order above the rounding of a Gram entry over that many terms; the mixed form's low rounds
+// stop at 1e-4 (where the FP64 finish takes over with quadratic convergence; measured best of
+// 1e-6, 1e-5, 1e-4 on saturated chi = 512 splits).
 const double JACOBI_TOL = 1e-14;
-const double JACOBI_TOL_LO = 1e-6;
+const double JACOBI_TOL_LO = 1e-4;
 const int MAX_SWEEPS_LO = 30;
 
 size_t retained_chi(const std::vector<double>& v, size_t max_bond, size_t rows, size_t cols) {
@@ -485,6 +487,9 @@ struct wqc_tn_state {
     // ------------------------------------------------------------ lifetime
     // Sweeps of the in-block eigensolve per round (WQC_TN_JACOBI_INNER overrides, 1..32).
     int inner_cap = 2;
+    int inner_cap_lo = 1;    // the same for the mixed form's low rounds (WQC_TN_JACOBI_INNER_LO)
+    double tol_lo = JACOBI_TOL_LO;  // their relative tolerance (WQC_TN_JACOBI_TOL_LO)
+    double floor_lo = 1e-12;        // squared-norm floor relative to a pair's largest row (WQC_TN_JACOBI_FLOOR_LO)
     bool trace_jacobi = false;
     DevBuf trace_buf;
 
@@ -495,6 +500,18 @@ struct wqc_tn_state {
             if (v >= 1 && v <= 32) inner_cap = v;
         }
         if (const char* e = std::getenv("WQC_TN_JACOBI_TRACE")) trace_jacobi = std::atoi(e) != 0;
+        if (const char* e = std::getenv("WQC_TN_JACOBI_INNER_LO")) {
+            const int v = std::atoi(e);
+            if (v >= 1 && v <= 32) inner_cap_lo = v;
+        }
+        if (const char* e = std::getenv("WQC_TN_JACOBI_FLOOR_LO")) {
+            const double v = std::atof(e);
+            if (v >= 1e-14 && v <= 1e-2) floor_lo = v;
+        }
+        if (const char* e = std::getenv("WQC_TN_JACOBI_TOL_LO")) {
+            const double v = std::atof(e);
+            if (v >= 1e-7 && v <= 1e-2) tol_lo = v;
+        }
         n = (int)qn;
         chi = mb;
         precision = prec;
@@ -709,13 +726,19 @@ struct wqc_tn_state {
         if (!nj) return;
         std::vector<int> lo_count(nj, 0);
         size_t off = 0;
-        std::vector<size_t> o32(nj), o64(nj);
+        std::vector<size_t> o32(nj), o64(nj), oj32(nj), ow(nj), ojn(nj);
+        auto take = [&](size_t bytes) {
+            const size_t o = off;
+            off = (off + bytes + 255) & ~(size_t)255;
+            return o;
+        };
         for (int k = 0; k < nj; ++k) {
-            const size_t cnt = (size_t)jobs[k].n * jobs[k].m;
-            o32[k] = off;
-            off = (off + cnt * sizeof(float2) + 255) & ~(size_t)255;
-            o64[k] = off;
-            off = (off + cnt * sizeof(double2) + 255) & ~(size_t)255;
+            const size_t cnt = (size_t)jobs[k].n * jobs[k].m, nn = (size_t)jobs[k].n * jobs[k].n;
+            o32[k] = take(cnt * sizeof(float2));
+            o64[k] = take(cnt * sizeof(double2));
+            oj32[k] = take(nn * sizeof(float2));
+            ow[k] = take(nn * sizeof(double2));
+            ojn[k] = take(nn * sizeof(double2));
         }
         ws32.ensure(off);
         note_peak();
@@ -732,7 +755,7 @@ struct wqc_tn_state {
             counts[k] = (long)jobs[k].n * jobs[k].m;
             mx = std::max(mx, counts[k]);
             lj[k].X = dst[k];
-            lj[k].J = jobs[k].J;
+            lj[k].J = reinterpret_cast<float2*>(b32 + oj32[k]);
             lj[k].n = jobs[k].n;
             lj[k].m = jobs[k].m;
             lj[k].ldx = jobs[k].m;
@@ -741,12 +764,26 @@ struct wqc_tn_state {
             max_m = std::max(max_m, jobs[k].m);
         }
         wqc::launch_to_f32(upload(src), upload(dst), upload(counts), nj, mx, stream);
+        // J32 = (complex64) J, the identity the split set up.
+        {
+            std::vector<const double2*> js(nj);
+            std::vector<float2*> jd(nj);
+            std::vector<long> jc(nj);
+            long jm = 1;
+            for (int k = 0; k < nj; ++k) {
+                js[k] = jobs[k].J;
+                jd[k] = lj[k].J;
+                jc[k] = (long)jobs[k].n * jobs[k].n;
+                jm = std::max(jm, jc[k]);
+            }
+            wqc::launch_to_f32(upload(js), upload(jd), upload(jc), nj, jm, stream);
+        }
         check_launch();
         wqc::JacobiLoJob* dj = upload(lj);
         const int nb = (max_n + 15) / 16;
         const int nbe = nb + (nb & 1);
         const int rounds = nbe - 1, pairs = nbe / 2;
-        const float tol = (float)(JACOBI_TOL_LO * std::sqrt((double)max_m));
+        const float tol = (float)(tol_lo * std::sqrt((double)max_m));
         const double nu2 = NULL_ROWS_SQ / max_n;
         int* d_rot = d_flags;
         int* d_done = d_flags + nj;
@@ -754,17 +791,39 @@ struct wqc_tn_state {
         wqc::launch_fill_int(d_done, 0, nj, stream);
         std::vector<int> done(nj, 0);
         int active = nj;
+        unsigned long long* d_off = nullptr;
+        if (trace_jacobi) {
+            trace_buf.ensure((size_t)nj * sizeof(unsigned long long));
+            d_off = reinterpret_cast<unsigned long long*>(trace_buf.p);
+        }
         for (int sweep = 0; sweep < MAX_SWEEPS_LO && active > 0; ++sweep) {
             check_cancel();
             wqc::launch_fill_int(d_rot, 0, nj, stream);
+            if (d_off) wqc::launch_fill_int(reinterpret_cast<int*>(d_off), 0, 2 * nj, stream);
             {
                 Timer tm(this, 3);
                 for (int r = 0; r < rounds; ++r)
-                    wqc::launch_jacobi_round_lo(dj, nj, pairs, r, tol, nu2, inner_cap, d_rot, d_done, d_jstats, stream);
+                    wqc::launch_jacobi_round_lo(dj, nj, pairs, r, tol, nu2, (float)floor_lo, inner_cap_lo, d_rot, d_done,
+                                                d_jstats, d_off, stream);
                 check_launch();
                 klaunch[3] += rounds;
             }
             stats.jacobi_sweeps_lo += active;
+            if (d_off) {
+                std::vector<unsigned long long> offv(nj);
+                HIPC(hipMemcpyAsync(offv.data(), d_off, nj * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+                sync();
+                std::fprintf(stderr, "low sweep %d active %d:", sweep + 1, active);
+                int shown = 0;
+                for (int j = 0; j < nj && shown < 4; ++j) {
+                    if (done[j] || jobs[j].n < max_n) continue;
+                    double v;
+                    std::memcpy(&v, &offv[j], sizeof v);
+                    std::fprintf(stderr, " [%dx%d] %.2e", jobs[j].n, jobs[j].m, v);
+                    ++shown;
+                }
+                std::fprintf(stderr, "\n");
+            }
             HIPC(hipMemcpyAsync(hread, d_rot, nj * sizeof(int), hipMemcpyDeviceToHost, stream));
             sync();
             const int* rot = reinterpret_cast<const int*>(hread);
@@ -779,6 +838,37 @@ struct wqc_tn_state {
                 }
             if (changed && active > 0)
                 HIPC(hipMemcpyAsync(d_done, upload_ints(done), nj * sizeof(int), hipMemcpyDeviceToDevice, stream));
+        }
+        // J = (complex128) J32, made unitary to FP64 precision by two Newton-Schulz steps
+        // J <- J (3I - J^H J) / 2 (the FP32 rounds leave it unitary to about 1e-6).
+        {
+            Timer tm(this, 2);
+            std::vector<const float2*> js(nj);
+            std::vector<double2*> jd(nj);
+            std::vector<long> jc(nj);
+            long jm = 1;
+            for (int k = 0; k < nj; ++k) {
+                js[k] = lj[k].J;
+                jd[k] = jobs[k].J;
+                jc[k] = (long)jobs[k].n * jobs[k].n;
+                jm = std::max(jm, jc[k]);
+            }
+            wqc::launch_to_f64(upload(js), upload(jd), upload(jc), nj, jm, stream);
+            check_launch();
+            for (int it = 0; it < 2; ++it) {
+                std::vector<GemmJob> g1, g2;
+                for (int k = 0; k < nj; ++k) {
+                    const int nn = jobs[k].n;
+                    double2* src_j = it == 0 ? jobs[k].J : reinterpret_cast<double2*>(b32 + ojn[k]);
+                    double2* dst_j = it == 0 ? reinterpret_cast<double2*>(b32 + ojn[k]) : jobs[k].J;
+                    double2* w = reinterpret_cast<double2*>(b32 + ow[k]);
+                    g1.push_back(gj(src_j, nn, 1, src_j, nn, 0, w, nn, nn, nn, nn, 1.0, 0.0));
+                    HIPC(hipMemcpyAsync(dst_j, src_j, (size_t)nn * nn * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+                    g2.push_back(gj(src_j, nn, 0, w, nn, 0, dst_j, nn, nn, nn, nn, -0.5, 1.5));
+                }
+                gemm(g1);
+                gemm(g2);
+            }
         }
         // X <- J X in FP64 (through a scratch copy), then the FP64 rounds.
         {

@@ -42,7 +42,7 @@ struct JacobiJob {
 // (n x n, complex128) accumulated so that J stays unitary to FP64 precision.
 struct JacobiLoJob {
     float2* X;
-    double2* J;
+    float2* J;  // n x n, complex64
     int n, m, ldx, pad;
 };
 
@@ -118,9 +118,11 @@ void launch_jacobi_round(const JacobiJob* jobs, int njobs, int max_pairs, int ro
                          double tol, double nu2, int inner_cap, int* rotated, const int* done,
                          unsigned long long* stats, unsigned long long* offmax, hipStream_t s);
 void launch_jacobi_round_lo(const JacobiLoJob* jobs, int njobs, int max_pairs, int round,
-                            float tol, double nu2, int inner_cap, int* rotated, const int* done,
-                            unsigned long long* stats, hipStream_t s);
+                            float tol, double nu2, float rel_floor, int inner_cap, int* rotated, const int* done,
+                            unsigned long long* stats, unsigned long long* offmax, hipStream_t s);
 void launch_to_f32(const double2* const* src, float2* const* dst, const long* counts, int njobs,
+                   long max_count, hipStream_t s);
+void launch_to_f64(const float2* const* src, double2* const* dst, const long* counts, int njobs,
                    long max_count, hipStream_t s);
 void launch_row_norms(const JacobiJob* jobs, int njobs, int max_n, double* sigma,
                       const long* sigma_off, hipStream_t s);

@@ -661,9 +661,11 @@ __device__ __forceinline__ f4v mfma_f32(float a, float b, f4v c) {
 
 __global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __restrict__ jobs,
                                                          int round, float tol, double nu2,
-                                                         int inner_cap, int* __restrict__ rotated,
+                                                         float rel_floor, int inner_cap,
+                                                         int* __restrict__ rotated,
                                                          const int* __restrict__ done,
-                                                         unsigned long long* __restrict__ stats) {
+                                                         unsigned long long* __restrict__ stats,
+                                                         unsigned long long* __restrict__ offmax) {
     const int job = blockIdx.y;
     if (done[job]) return;
     const JacobiLoJob jb = jobs[job];
@@ -683,17 +685,15 @@ __global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __re
 
     __shared__ float2 G[JP][JP + 1];
     __shared__ float2 T[JP][JP + 1];
-    __shared__ double2 T64[JP][JP + 1];
-    // W64 of the Newton-Schulz steps; before them the partial Gram tiles, then the second G
-    // and T buffers of the eigensolve.
-    __shared__ __align__(16) double2 W64[JP][JP + 1];
+    // Partial Gram tiles of two waves; afterwards the second G and T buffers of the eigensolve.
+    __shared__ __align__(16) float s_aux[4 * JP * (JP + 1)];
     __shared__ int s_row[JP];
     __shared__ int s_any;
     __shared__ float s_low;
     __shared__ int s_dst[JP];
-    float (*Gp)[6][4][64] = reinterpret_cast<float (*)[6][4][64]>(&W64[0][0]);
-    float2 (*G2)[JP + 1] = reinterpret_cast<float2 (*)[JP + 1]>(&W64[0][0]);
-    float2 (*T2)[JP + 1] = reinterpret_cast<float2 (*)[JP + 1]>(reinterpret_cast<float*>(&W64[0][0]) + 2 * JP * (JP + 1));
+    float (*Gp)[6][4][64] = reinterpret_cast<float (*)[6][4][64]>(s_aux);
+    float2 (*G2)[JP + 1] = reinterpret_cast<float2 (*)[JP + 1]>(s_aux);
+    float2 (*T2)[JP + 1] = reinterpret_cast<float2 (*)[JP + 1]>(s_aux + 2 * JP * (JP + 1));
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid < JP) {
@@ -709,32 +709,47 @@ __global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __re
         const int r0 = s_row[i], r1 = s_row[16 + i];
         const float2* x0 = r0 >= 0 ? jb.X + (long)r0 * jb.ldx : nullptr;
         const float2* x1 = r1 >= 0 ? jb.X + (long)r1 * jb.ldx : nullptr;
-        f4v a00r = {0, 0, 0, 0}, a00i = a00r, a01r = a00r, a01i = a00r, a11r = a00r, a11i = a00r;
+        // 3M products and register double-buffered column chunks, as in the FP64 round.
+        f4v p00 = {0, 0, 0, 0}, q00 = p00, t00 = p00, p01 = p00, q01 = p00, t01 = p00, p11 = p00, q11 = p00, t11 = p00;
         const float2 zf = make_float2(0.f, 0.f);
-        for (int c0 = wave * 16; c0 < jb.m; c0 += 64) {
-            float2 u[4], v[4];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int c = c0 + 4 * g + s;
-                u[s] = (x0 && c < jb.m) ? x0[c] : zf;
-                v[s] = (x1 && c < jb.m) ? x1[c] : zf;
-            }
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                a00r = mfma_f32(u[s].x, u[s].x, a00r);
-                a00r = mfma_f32(u[s].y, u[s].y, a00r);
-                a00i = mfma_f32(u[s].y, u[s].x, a00i);
-                a00i = mfma_f32(-u[s].x, u[s].y, a00i);
-                a01r = mfma_f32(u[s].x, v[s].x, a01r);
-                a01r = mfma_f32(u[s].y, v[s].y, a01r);
-                a01i = mfma_f32(u[s].y, v[s].x, a01i);
-                a01i = mfma_f32(-u[s].x, v[s].y, a01i);
-                a11r = mfma_f32(v[s].x, v[s].x, a11r);
-                a11r = mfma_f32(v[s].y, v[s].y, a11r);
-                a11i = mfma_f32(v[s].y, v[s].x, a11i);
-                a11i = mfma_f32(-v[s].x, v[s].y, a11i);
-            }
+        float2 ua[4], va[4], ub[4], vb[4];
+#define WQC_GRAM_LOAD(C0, U, V)                                  \
+    _Pragma("unroll") for (int s = 0; s < 4; ++s) {              \
+        const int c = (C0) + 4 * g + s;                          \
+        U[s] = (x0 && c < jb.m) ? x0[c] : zf;                    \
+        V[s] = (x1 && c < jb.m) ? x1[c] : zf;                    \
+    }
+#define WQC_GRAM_ACC(U, V)                                       \
+    _Pragma("unroll") for (int s = 0; s < 4; ++s) {              \
+        const float up = U[s].x + U[s].y, um = U[s].x - U[s].y;  \
+        const float vp = V[s].x + V[s].y, vm = V[s].x - V[s].y;  \
+        p00 = mfma_f32(U[s].x, U[s].x, p00);                     \
+        q00 = mfma_f32(U[s].y, U[s].y, q00);                     \
+        t00 = mfma_f32(up, um, t00);                             \
+        p01 = mfma_f32(U[s].x, V[s].x, p01);                     \
+        q01 = mfma_f32(U[s].y, V[s].y, q01);                     \
+        t01 = mfma_f32(up, vm, t01);                             \
+        p11 = mfma_f32(V[s].x, V[s].x, p11);                     \
+        q11 = mfma_f32(V[s].y, V[s].y, q11);                     \
+        t11 = mfma_f32(vp, vm, t11);                             \
+    }
+        int c0 = wave * 16;
+        if (c0 < jb.m) WQC_GRAM_LOAD(c0, ua, va)
+        while (c0 < jb.m) {
+            const int c1 = c0 + 64;
+            if (c1 < jb.m) WQC_GRAM_LOAD(c1, ub, vb)
+            WQC_GRAM_ACC(ua, va)
+            if (c1 >= jb.m) break;
+            const int c2 = c1 + 64;
+            if (c2 < jb.m) WQC_GRAM_LOAD(c2, ua, va)
+            WQC_GRAM_ACC(ub, vb)
+            c0 = c2;
         }
+#undef WQC_GRAM_LOAD
+#undef WQC_GRAM_ACC
+        f4v a00r = p00 + q00, a00i = t00 - p00 + q00;
+        f4v a01r = p01 + q01, a01i = t01 - p01 + q01;
+        f4v a11r = p11 + q11, a11i = t11 - p11 + q11;
 #define WQC_GP_ALL(OP) OP(0, a00r) OP(1, a00i) OP(2, a01r) OP(3, a01i) OP(4, a11r) OP(5, a11i)
 #define WQC_GP_STORE(t, v) for (int j = 0; j < 4; ++j) Gp[slot][t][j][lane] = v[j];
 #define WQC_GP_ADD(t, v) for (int j = 0; j < 4; ++j) v[j] += Gp[slot][t][j][lane];
@@ -773,26 +788,34 @@ __global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __re
         }
     }
     __syncthreads();
-    // Rows below FP32 resolution of the pair's largest row are left to the FP64 finish.
+    // Rows whose squared norm is below rel_floor of the pair's largest are left to the FP64
+    // finish: FP32 rotations add noise of about 1e-7 of the largest row to them, so they could
+    // not reach the relative tolerance here.
     if (tid == 0) {
         float mx = 0.f;
         for (int i = 0; i < JP; ++i) mx = fmaxf(mx, G[i][i].x);
-        s_low = fmaxf((float)nu2, mx * 1e-12f);
+        s_low = fmaxf((float)nu2, mx * rel_floor);
     }
     __syncthreads();
     const float low = s_low;
     {
         bool off = false;
+        double worst = 0.0;
         for (int e = tid; e < JP * JP; e += 256) {
             const int i = e / JP, j = e % JP;
             if (i < j) {
                 const float2 c = G[i][j];
                 const float gg = hypotf(c.x, c.y);
                 const float di = G[i][i].x, dj = G[j][j].x;
-                if (gg > 0.f && fminf(di, dj) > low && gg > tol * sqrtf(fmaxf(di, 0.f) * fmaxf(dj, 0.f))) off = true;
+                if (gg > 0.f && fminf(di, dj) > low) {
+                    const float den = sqrtf(fmaxf(di, 0.f) * fmaxf(dj, 0.f));
+                    if (gg > tol * den) off = true;
+                    if (offmax) worst = fmax(worst, (double)(gg / den));
+                }
             }
         }
         if (off) s_any = 1;
+        if (offmax && worst > 0.0) atomicMax(&offmax[job], (unsigned long long)__double_as_longlong(worst));
     }
     __syncthreads();
     const int need = s_any;
@@ -890,138 +913,86 @@ __global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __re
         atomicAdd(&stats[0], (unsigned long long)sweeps_done);
         atomicAdd(&stats[1], 1ull);
     }
-    // Row order from the rotated diagonal, before W64 (which may hold it) is reused.
+    // Ordered rotation (as in the FP64 round): row k of T moves to row dst[k], into the free G
+    // buffer.
     pair_order(tid, s_row, &Gc[0][0].x, 2 * (JP + 2), s_dst);
-
-    // T -> FP64, two Newton-Schulz steps T <- T (3I - T^H T) / 2 (unitary to FP64 precision).
+    __syncthreads();
     for (int e = tid; e < JP * JP; e += 256) {
         const int i = e / JP, j = e % JP;
-        T64[i][j] = make_double2((double)Tc[i][j].x, (double)Tc[i][j].y);
+        Gn[s_dst[i]][j] = Tc[i][j];
     }
     __syncthreads();
-    for (int it = 0; it < 2; ++it) {
-        for (int e = tid; e < JP * JP; e += 256) {
-            const int i = e / JP, j = e % JP;
-            double2 acc = cz();
-            for (int k = 0; k < JP; ++k) ccmac(acc, T64[k][i], T64[k][j]);
-            W64[i][j] = make_double2((i == j ? 1.5 : 0.0) - 0.5 * acc.x, -0.5 * acc.y);
-        }
-        __syncthreads();
-        double2 out[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int e = tid + 256 * u, i = e / JP, j = e % JP;
-            double2 acc = cz();
-            for (int k = 0; k < JP; ++k) cmac(acc, T64[i][k], W64[k][j]);
-            out[u] = acc;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int e = tid + 256 * u, i = e / JP, j = e % JP;
-            T64[i][j] = out[u];
-        }
-        __syncthreads();
-    }
+    float2 (*const Tp)[JP + 1] = Gn;
 
-    // Ordered rotation (as in the FP64 round): row k of T moves to row dst[k], into W64.
-    for (int e = tid; e < JP * JP; e += 256) {
-        const int i = e / JP, j = e % JP;
-        W64[s_dst[i]][j] = T64[i][j];
-    }
-    __syncthreads();
-
-    // X32 <- T X32 (FP32 MFMA), J <- T J (FP64 MFMA), T in W64.
+    // X32 <- T X32 and J32 <- T J32 on FP32 MFMA with 3M products (E1 = Tr Pr, E2 = Ti Pi,
+    // E3 = (Tr + Ti)(Pr + Pi); Re = E1 - E2, Im = E3 - E1 - E2). J stays unitary to FP32
+    // precision; the host makes it unitary in FP64 once, after the last low round.
     {
         const int i = lane & 15, g = lane >> 4;
+        float tr[2][8], ti[2][8], ts[2][8];
+#pragma unroll
+        for (int I = 0; I < 2; ++I)
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) {
+                const float2 t = Tp[I * 16 + i][4 * kk + g];
+                tr[I][kk] = t.x;
+                ti[I][kk] = t.y;
+                ts[I][kk] = t.x + t.y;
+            }
         int prow[8];
 #pragma unroll
         for (int kk = 0; kk < 8; ++kk) prow[kk] = s_row[4 * kk + g];
-        {
-            float tr[2][8], ti[2][8], tni[2][8];
+        // MFMA f32 output: element j of lane l is row 4 (l / 16) + j, column l % 16.
+        int orow[2][4];
 #pragma unroll
-            for (int I = 0; I < 2; ++I)
+        for (int I = 0; I < 2; ++I)
 #pragma unroll
-                for (int kk = 0; kk < 8; ++kk) {
-                    const double2 t = W64[I * 16 + i][4 * kk + g];
-                    tr[I][kk] = (float)t.x;
-                    ti[I][kk] = (float)t.y;
-                    tni[I][kk] = -(float)t.y;
-                }
-            int orow[2][4];
-#pragma unroll
-            for (int I = 0; I < 2; ++I)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) orow[I][j] = s_row[I * 16 + 4 * g + j];
-            for (int c0 = wave * 16; c0 < jb.m; c0 += 64) {
-                const int c = c0 + i;
-                const bool cok = c < jb.m;
-                float2 b[8];
-#pragma unroll
-                for (int kk = 0; kk < 8; ++kk)
-                    b[kk] = (cok && prow[kk] >= 0) ? jb.X[(long)prow[kk] * jb.ldx + c] : make_float2(0.f, 0.f);
-#pragma unroll
-                for (int I = 0; I < 2; ++I) {
-                    f4v dr = {0, 0, 0, 0}, di = dr;
-#pragma unroll
-                    for (int kk = 0; kk < 8; ++kk) {
-                        dr = mfma_f32(tr[I][kk], b[kk].x, dr);
-                        dr = mfma_f32(tni[I][kk], b[kk].y, dr);
-                        di = mfma_f32(tr[I][kk], b[kk].y, di);
-                        di = mfma_f32(ti[I][kk], b[kk].x, di);
-                    }
-                    if (cok) {
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) {
-                            const int gr = orow[I][j];
-                            if (gr >= 0) jb.X[(long)gr * jb.ldx + c] = make_float2(dr[j], di[j]);
-                        }
-                    }
-                }
+            for (int j = 0; j < 4; ++j) orow[I][j] = s_row[I * 16 + 4 * g + j];
+        for (int pass = 0; pass < 2; ++pass) {
+            float2* M = pass == 0 ? jb.X : jb.J;
+            const int cols = pass == 0 ? jb.m : jb.n;
+            const long ld = pass == 0 ? jb.ldx : jb.n;
+#define WQC_APPLY_LOAD(C0, B)                                                                      \
+    {                                                                                              \
+        const int c = (C0) + i;                                                                    \
+        _Pragma("unroll") for (int kk = 0; kk < 8; ++kk)                                           \
+            B[kk] = (c < cols && prow[kk] >= 0) ? M[(long)prow[kk] * ld + c] : make_float2(0.f, 0.f); \
+    }
+#define WQC_APPLY_ROT(C0, B)                                                                       \
+    {                                                                                              \
+        const int c = (C0) + i;                                                                    \
+        float bs[8];                                                                               \
+        _Pragma("unroll") for (int kk = 0; kk < 8; ++kk) bs[kk] = B[kk].x + B[kk].y;               \
+        _Pragma("unroll") for (int I = 0; I < 2; ++I) {                                            \
+            f4v e1 = {0, 0, 0, 0}, e2 = e1, e3 = e1;                                               \
+            _Pragma("unroll") for (int kk = 0; kk < 8; ++kk) {                                     \
+                e1 = mfma_f32(tr[I][kk], B[kk].x, e1);                                             \
+                e2 = mfma_f32(ti[I][kk], B[kk].y, e2);                                             \
+                e3 = mfma_f32(ts[I][kk], bs[kk], e3);                                              \
+            }                                                                                      \
+            if (c < cols) {                                                                        \
+                _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                    \
+                    const int gr = orow[I][j];                                                     \
+                    if (gr >= 0) M[(long)gr * ld + c] = make_float2(e1[j] - e2[j], e3[j] - e1[j] - e2[j]); \
+                }                                                                                  \
+            }                                                                                      \
+        }                                                                                          \
+    }
+            float2 ba[8], bb[8];
+            int c0 = wave * 16;
+            if (c0 < cols) WQC_APPLY_LOAD(c0, ba)
+            while (c0 < cols) {
+                const int c1 = c0 + 64;
+                if (c1 < cols) WQC_APPLY_LOAD(c1, bb)
+                WQC_APPLY_ROT(c0, ba)
+                if (c1 >= cols) break;
+                const int c2 = c1 + 64;
+                if (c2 < cols) WQC_APPLY_LOAD(c2, ba)
+                WQC_APPLY_ROT(c1, bb)
+                c0 = c2;
             }
-        }
-        {
-            double tr[2][8], ti[2][8], tni[2][8];
-#pragma unroll
-            for (int I = 0; I < 2; ++I)
-#pragma unroll
-                for (int kk = 0; kk < 8; ++kk) {
-                    const double2 t = W64[I * 16 + i][4 * kk + g];
-                    tr[I][kk] = t.x;
-                    ti[I][kk] = t.y;
-                    tni[I][kk] = -t.y;
-                }
-            int orow[2][4];
-#pragma unroll
-            for (int I = 0; I < 2; ++I)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) orow[I][j] = s_row[I * 16 + 4 * j + g];
-            for (int c0 = wave * 16; c0 < jb.n; c0 += 64) {
-                const int c = c0 + i;
-                const bool cok = c < jb.n;
-                double2 b[8];
-#pragma unroll
-                for (int kk = 0; kk < 8; ++kk)
-                    b[kk] = (cok && prow[kk] >= 0) ? jb.J[(long)prow[kk] * jb.n + c] : cz();
-#pragma unroll
-                for (int I = 0; I < 2; ++I) {
-                    d4v dr = {0, 0, 0, 0}, di = dr;
-#pragma unroll
-                    for (int kk = 0; kk < 8; ++kk) {
-                        dr = mfma_f64(tr[I][kk], b[kk].x, dr);
-                        dr = mfma_f64(tni[I][kk], b[kk].y, dr);
-                        di = mfma_f64(tr[I][kk], b[kk].y, di);
-                        di = mfma_f64(ti[I][kk], b[kk].x, di);
-                    }
-                    if (cok) {
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) {
-                            const int gr = orow[I][j];
-                            if (gr >= 0) jb.J[(long)gr * jb.n + c] = make_double2(dr[j], di[j]);
-                        }
-                    }
-                }
-            }
+#undef WQC_APPLY_LOAD
+#undef WQC_APPLY_ROT
         }
     }
 }
@@ -1037,6 +1008,20 @@ __global__ __launch_bounds__(256) void k_to_f32(const double2* const* __restrict
     for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
         const double2 v = s[e];
         d[e] = make_float2((float)v.x, (float)v.y);
+    }
+}
+
+// (complex128) of a complex64 buffer.
+__global__ __launch_bounds__(256) void k_to_f64(const float2* const* __restrict__ src,
+                                                double2* const* __restrict__ dst,
+                                                const long* __restrict__ counts) {
+    const int job = blockIdx.y;
+    const float2* s = src[job];
+    double2* d = dst[job];
+    const long n = counts[job];
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+        const float2 v = s[e];
+        d[e] = make_double2((double)v.x, (double)v.y);
     }
 }
 
@@ -1557,15 +1542,20 @@ void launch_jacobi_round(const JacobiJob* jobs, int njobs, int max_pairs, int ro
 }
 
 void launch_jacobi_round_lo(const JacobiLoJob* jobs, int njobs, int max_pairs, int round, float tol,
-                            double nu2, int inner_cap, int* rotated, const int* done, unsigned long long* stats,
-                            hipStream_t s) {
-    hipLaunchKernelGGL(k_jacobi_round_lo, dim3(max_pairs, njobs), dim3(256), 0, s, jobs, round, tol,
-                       nu2, inner_cap, rotated, done, stats);
+                            double nu2, float rel_floor, int inner_cap, int* rotated, const int* done, unsigned long long* stats,
+                            unsigned long long* offmax, hipStream_t s) {
+    hipLaunchKernelGGL(k_jacobi_round_lo, dim3(max_pairs, njobs), dim3(256), 0, s, jobs, round, tol, nu2, rel_floor, inner_cap, rotated, done, stats, offmax);
 }
 
 void launch_to_f32(const double2* const* src, float2* const* dst, const long* counts, int njobs,
                    long max_count, hipStream_t s) {
     hipLaunchKernelGGL(k_to_f32, dim3(blocks_for(max_count, 256, 2048), njobs), dim3(256), 0, s, src,
+                       dst, counts);
+}
+
+void launch_to_f64(const float2* const* src, double2* const* dst, const long* counts, int njobs,
+                   long max_count, hipStream_t s) {
+    hipLaunchKernelGGL(k_to_f64, dim3(blocks_for(max_count, 256, 2048), njobs), dim3(256), 0, s, src,
                        dst, counts);
 }
 
