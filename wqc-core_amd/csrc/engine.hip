@@ -653,8 +653,9 @@ struct wqc_tn_state {
         const int nb = (max_n + 15) / 16;
         const int nbe = nb + (nb & 1);
         const int rounds = nbe - 1, pairs = nbe / 2;
-        const double tol = JACOBI_TOL * std::sqrt((double)max_m);
-        const double nu2 = NULL_ROWS_SQ / max_n;
+        // Per job: tolerance JACOBI_TOL sqrt(m), null-row floor NULL_ROWS_SQ / n (in the kernel).
+        const double tol = JACOBI_TOL;
+        const double nu2 = NULL_ROWS_SQ;
         int* d_rot = d_flags;
         int* d_done = d_flags + nj;
         unsigned long long* d_jstats = jstats_dev();
@@ -783,8 +784,8 @@ struct wqc_tn_state {
         const int nb = (max_n + 15) / 16;
         const int nbe = nb + (nb & 1);
         const int rounds = nbe - 1, pairs = nbe / 2;
-        const float tol = (float)(tol_lo * std::sqrt((double)max_m));
-        const double nu2 = NULL_ROWS_SQ / max_n;
+        const float tol = (float)tol_lo;
+        const double nu2 = NULL_ROWS_SQ;
         int* d_rot = d_flags;
         int* d_done = d_flags + nj;
         unsigned long long* d_jstats = jstats_dev();
@@ -1033,12 +1034,27 @@ struct wqc_tn_state {
         }
         stats.two_qubit_units += nu;
 
-        std::vector<int> exact_ids, rsvd_ids;
-        for (int i = 0; i < nu; ++i) (J[i].rsvd ? rsvd_ids : exact_ids).push_back(i);
-        if (!rsvd_ids.empty()) run_rsvd(J, rsvd_ids, base, d_flags, d_fro, d_part, nblk);
-        for (int i : rsvd_ids)
-            if (J[i].fallback) exact_ids.push_back(i);
-        if (!exact_ids.empty()) run_exact(J, exact_ids, base, d_flags, d_fro + nu, d_part + (size_t)nu * nblk, nblk);
+        SplitBatch rs, ex;
+        for (int i = 0; i < nu; ++i) (J[i].rsvd ? rs.ids : ex.ids).push_back(i);
+        rs.d_fro = d_fro;
+        rs.d_part = d_part;
+        ex.d_fro = d_fro + nu;
+        ex.d_part = d_part + (size_t)nu * nblk;
+        if (!rs.ids.empty()) prep_rsvd(J, rs, base, nblk);
+        if (!ex.ids.empty()) prep_exact(J, ex, base, nblk);
+        jacobi_splits(J, {&rs, &ex}, base, d_flags);
+        if (!rs.ids.empty()) finish_rsvd(J, rs, base);
+        if (!ex.ids.empty()) finish_exact(J, ex, base);
+        SplitBatch fb;
+        for (int i : rs.ids)
+            if (J[i].fallback) fb.ids.push_back(i);
+        if (!fb.ids.empty()) {
+            fb.d_fro = d_fro + nu;
+            fb.d_part = d_part + (size_t)nu * nblk;
+            prep_exact(J, fb, base, nblk);
+            jacobi_splits(J, {&fb}, base, d_flags);
+            finish_exact(J, fb, base);
+        }
         sync();
         resolve_timings();
         if (split_log_on)
@@ -1051,11 +1067,56 @@ struct wqc_tn_state {
             }
     }
 
-    // Exact split (truncated_svd_split) for the listed jobs.
-    void run_exact(std::vector<JobInfo>& J, const std::vector<int>& ids, char* base, int* d_flags,
-                   double* d_fro, double* d_part, int nblk) {
+    void gemm(std::vector<GemmJob>& g) {
+        if (g.empty()) return;
+        int mm = 1, mn = 1;
+        for (auto& x : g) {
+            mm = std::max(mm, x.M);
+            mn = std::max(mn, x.N);
+        }
+        wqc::launch_gemm(upload(g), (int)g.size(), mm, mn, stream);
+        check_launch();
+        klaunch[2]++;
+    }
+
+    static GemmJob gj(const double2* A, int lda, int opA, const double2* B, int ldb, int opB, double2* C,
+                      int ldc, int M, int N, int K, double alpha, double beta) {
+        GemmJob g;
+        g.A = A;
+        g.B = B;
+        g.C = C;
+        g.M = M;
+        g.N = N;
+        g.K = K;
+        g.lda = lda;
+        g.ldb = ldb;
+        g.ldc = ldc;
+        g.opA = opA;
+        g.opB = opB;
+        g.alpha = make_double2(alpha, 0);
+        g.beta = make_double2(beta, 0);
+        return g;
+    }
+
+    // A batch of splits of one kind: its jobs, their Jacobi problems (rows of theta' or of the
+    // range finder's B), sigma offsets, |A|_F^2 slots and, after jacobi_splits, where the
+    // host copies of sigma and |A|_F^2 sit in the readback buffer.
+    struct SplitBatch {
+        std::vector<int> ids;
+        std::vector<JacobiJob> jj;
+        std::vector<long> offs;
+        double* d_fro = nullptr;
+        double* d_part = nullptr;
+        std::vector<size_t> hoff;
+        size_t hfro = 0;
+    };
+
+    // Exact split (truncated_svd_split): theta' (or its adjoint when tall) as the Jacobi rows,
+    // the dust rule, J = I.
+    void prep_exact(std::vector<JobInfo>& J, SplitBatch& b, char* base, int nblk) {
         auto P = [&](size_t o) { return reinterpret_cast<double2*>(base + o); };
         auto PD = [&](size_t o) { return reinterpret_cast<double*>(base + o); };
+        const std::vector<int>& ids = b.ids;
         const int ne = (int)ids.size();
         // Fallback jobs from the range finder hold theta' untransposed.
         for (int i : ids) {
@@ -1065,9 +1126,12 @@ struct wqc_tn_state {
                 check_launch();
             }
         }
-        std::vector<JacobiJob> jj(ne);
+        std::vector<JacobiJob>& jj = b.jj;
+        jj.assign(ne, JacobiJob{});
         std::vector<double2*> mats(ne), Jm(ne);
-        std::vector<long> counts(ne), offs(ne);
+        std::vector<long> counts(ne);
+        std::vector<long>& offs = b.offs;
+        offs.assign(ne, 0);
         std::vector<int> dims(ne);
         int max_n = 1;
         for (int k = 0; k < ne; ++k) {
@@ -1086,36 +1150,143 @@ struct wqc_tn_state {
             max_n = std::max(max_n, jj[k].n);
             offs[k] = (long)((j.o_sig) / sizeof(double));
         }
-        dust(mats, counts, d_fro, d_part, nblk);
+        dust(mats, counts, b.d_fro, b.d_part, nblk);
         wqc::launch_identity(upload(Jm), upload(dims), ne, max_n, stream);
         check_launch();
-        // sigma offsets are relative to the workspace base viewed as doubles
-        jacobi_any(jj, PD(0), offs, d_flags);
-        for (int k = 0; k < ne; ++k) {
-            J[ids[k]].sweeps += last_sweeps[k];
-            J[ids[k]].sweeps_lo += last_sweeps_lo[k];
+    }
+
+    // Range finder (rsvd_sites): Y = A (A^H (A Omega_I)), compact-WY Householder of its
+    // independent columns, B = Q^H A; B's rows are the Jacobi rows.
+    void prep_rsvd(std::vector<JobInfo>& J, SplitBatch& b, char* base, int nblk) {
+        auto P = [&](size_t o) { return reinterpret_cast<double2*>(base + o); };
+        auto PD = [&](size_t o) { return reinterpret_cast<double*>(base + o); };
+        const std::vector<int>& ids = b.ids;
+        const int nr = (int)ids.size();
+        {
+            Timer tm(this, 2);
+            std::vector<GemmJob> g1, g2, g3;
+            for (int i : ids) {
+                JobInfo& j = J[i];
+                const int r = j.om->r;
+                // Y0 = A Omega_I; Z = A^H Y0; Y = A Z
+                g1.push_back(gj(P(j.o_theta), j.cols, 0, j.om->omega_cols, r, 0, P(j.o_Y0), r, j.rows, r, j.cols, 1, 0));
+                g2.push_back(gj(P(j.o_theta), j.cols, 1, P(j.o_Y0), r, 0, P(j.o_Z), r, j.cols, r, j.rows, 1, 0));
+                g3.push_back(gj(P(j.o_theta), j.cols, 0, P(j.o_Z), r, 0, P(j.o_Y), r, j.rows, r, j.cols, 1, 0));
+            }
+            gemm(g1);
+            gemm(g2);
+            gemm(g3);
+            std::vector<HouseJob> hj;
+            for (int i : ids) {
+                JobInfo& j = J[i];
+                if (j.om->r > wqc::kMaxHouseCols) fail(WQC_TN_EXEC_FAILED, "range finder rank above the QR limit");
+                HouseJob h;
+                h.Y = P(j.o_Y);
+                h.V = P(j.o_V);
+                h.T = P(j.o_T);
+                h.pos = j.om->pos;
+                h.rows = j.rows;
+                h.r = j.om->r;
+                h.pad0 = h.pad1 = 0;
+                h.tau = QR_RANK_TOL;
+                hj.push_back(h);
+            }
+            wqc::launch_householder(upload(hj), (int)hj.size(), stream);
+            check_launch();
+            klaunch[2]++;
+            // B = A[0:ell, :] - V[0:ell, :] (T^H (V^H A))
+            std::vector<GemmJob> g4, g5, g6;
+            for (int i : ids) {
+                JobInfo& j = J[i];
+                const int r = j.om->r;
+                g4.push_back(gj(P(j.o_V), r, 1, P(j.o_theta), j.cols, 0, P(j.o_Z2), j.cols, r, j.cols, j.rows, 1, 0));
+                g5.push_back(gj(P(j.o_T), r, 1, P(j.o_Z2), j.cols, 0, P(j.o_Z), j.cols, r, j.cols, r, 1, 0));
+                HIPC(hipMemcpyAsync(P(j.o_B), P(j.o_theta), j.ell * (size_t)j.cols * sizeof(double2),
+                                    hipMemcpyDeviceToDevice, stream));
+                g6.push_back(gj(P(j.o_V), r, 0, P(j.o_Z), j.cols, 0, P(j.o_B), j.cols, (int)j.ell, j.cols, r, -1, 1));
+            }
+            gemm(g4);
+            gemm(g5);
+            gemm(g6);
         }
-        // read back sigma and |theta|_F^2
-        size_t tot = 0;
-        for (int k = 0; k < ne; ++k) tot += jj[k].n;
-        if (tot + ne > hread_cap) fail(WQC_TN_EXEC_FAILED, "readback buffer too small");
-        size_t cur = 0;
-        std::vector<size_t> hoff(ne);
-        for (int k = 0; k < ne; ++k) {
-            hoff[k] = cur;
-            HIPC(hipMemcpyAsync(hread + cur, PD(J[ids[k]].o_sig), jj[k].n * sizeof(double), hipMemcpyDeviceToHost, stream));
-            cur += jj[k].n;
+        // checked_svd(B): dust rule, Jacobi on the rows of B (ell <= cols), norms.
+        std::vector<JacobiJob>& jj = b.jj;
+        jj.assign(nr, JacobiJob{});
+        std::vector<double2*> mats(nr), Jm(nr);
+        std::vector<long> counts(nr);
+        std::vector<long>& offs = b.offs;
+        offs.assign(nr, 0);
+        std::vector<int> dims(nr);
+        int max_n = 1;
+        for (int k = 0; k < nr; ++k) {
+            JobInfo& j = J[ids[k]];
+            jj[k].X = P(j.o_B);
+            jj[k].J = P(j.o_JB);
+            jj[k].n = (int)j.ell;
+            jj[k].m = j.cols;
+            jj[k].ldx = j.cols;
+            mats[k] = P(j.o_B);
+            Jm[k] = P(j.o_JB);
+            counts[k] = (long)j.ell * j.cols;
+            dims[k] = (int)j.ell;
+            max_n = std::max(max_n, (int)j.ell);
+            offs[k] = (long)(j.o_sigB / sizeof(double));
         }
-        HIPC(hipMemcpyAsync(hread + cur, d_fro, ne * sizeof(double), hipMemcpyDeviceToHost, stream));
+        dust(mats, counts, b.d_fro, b.d_part, nblk);
+        wqc::launch_identity(upload(Jm), upload(dims), nr, max_n, stream);
+        check_launch();
+    }
+
+    // One Jacobi over the jobs of all batches (range-finder B and exact theta' of a layer
+    // share the launches), then sigma and |A|_F^2 of every job read back with one sync.
+    void jacobi_splits(std::vector<JobInfo>& J, const std::vector<SplitBatch*>& batches, char* base, int* d_flags) {
+        auto PD = [&](size_t o) { return reinterpret_cast<double*>(base + o); };
+        std::vector<JacobiJob> all;
+        std::vector<long> offs;
+        for (auto* b : batches) {
+            all.insert(all.end(), b->jj.begin(), b->jj.end());
+            offs.insert(offs.end(), b->offs.begin(), b->offs.end());
+        }
+        if (all.empty()) return;
+        jacobi_any(all, PD(0), offs, d_flags);
+        size_t k = 0, cur = 0;
+        for (auto* b : batches) {
+            b->hoff.assign(b->ids.size(), 0);
+            for (size_t i = 0; i < b->ids.size(); ++i, ++k) {
+                J[b->ids[i]].sweeps += last_sweeps[k];
+                J[b->ids[i]].sweeps_lo += last_sweeps_lo[k];
+                if (cur + b->jj[i].n + 1 > hread_cap) fail(WQC_TN_EXEC_FAILED, "readback buffer too small");
+                b->hoff[i] = cur;
+                HIPC(hipMemcpyAsync(hread + cur, PD(0) + b->offs[i], b->jj[i].n * sizeof(double),
+                                    hipMemcpyDeviceToHost, stream));
+                cur += b->jj[i].n;
+            }
+        }
+        for (auto* b : batches) {
+            if (b->ids.empty()) continue;
+            if (cur + b->ids.size() > hread_cap) fail(WQC_TN_EXEC_FAILED, "readback buffer too small");
+            b->hfro = cur;
+            HIPC(hipMemcpyAsync(hread + cur, b->d_fro, b->ids.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+            cur += b->ids.size();
+        }
         sync();
+    }
+
+    // Exact split, after the Jacobi: stable descending sigma, the norm check of
+    // svd_reproduces, retained_chi, reject_truncation, sqrt(sigma) factors.
+    void finish_exact(std::vector<JobInfo>& J, const SplitBatch& b, char* base) {
+        auto PD = [&](size_t o) { return reinterpret_cast<double*>(base + o); };
+        const std::vector<int>& ids = b.ids;
+        const std::vector<JacobiJob>& jj = b.jj;
+        const int ne = (int)ids.size();
         std::vector<FactorJob> fj(ne);
         std::vector<std::vector<int>> perms(ne);
         long mxl = 1, mxr = 1;
         for (int k = 0; k < ne; ++k) {
             JobInfo& j = J[ids[k]];
             const int nn = jj[k].n;
-            std::vector<double> sig(hread + hoff[k], hread + hoff[k] + nn);
-            const double fro2 = hread[cur + k];
+            std::vector<double> sig(hread + b.hoff[k], hread + b.hoff[k] + nn);
+            const double fro2 = hread[b.hfro + k];
             std::vector<int> order(nn);
             for (int i = 0; i < nn; ++i) order[i] = i;
             std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sig[a] > sig[b]; });
@@ -1179,128 +1350,14 @@ struct wqc_tn_state {
         }
     }
 
-    void gemm(std::vector<GemmJob>& g) {
-        if (g.empty()) return;
-        int mm = 1, mn = 1;
-        for (auto& x : g) {
-            mm = std::max(mm, x.M);
-            mn = std::max(mn, x.N);
-        }
-        wqc::launch_gemm(upload(g), (int)g.size(), mm, mn, stream);
-        check_launch();
-        klaunch[2]++;
-    }
-
-    static GemmJob gj(const double2* A, int lda, int opA, const double2* B, int ldb, int opB, double2* C,
-                      int ldc, int M, int N, int K, double alpha, double beta) {
-        GemmJob g;
-        g.A = A;
-        g.B = B;
-        g.C = C;
-        g.M = M;
-        g.N = N;
-        g.K = K;
-        g.lda = lda;
-        g.ldb = ldb;
-        g.ldc = ldc;
-        g.opA = opA;
-        g.opB = opB;
-        g.alpha = make_double2(alpha, 0);
-        g.beta = make_double2(beta, 0);
-        return g;
-    }
-
-    // Range finder (rsvd_sites) for the listed jobs; jobs whose cut is ambiguous are
-    // marked fallback for the exact split.
-    void run_rsvd(std::vector<JobInfo>& J, const std::vector<int>& ids, char* base, int* d_flags,
-                  double* d_fro, double* d_part, int nblk) {
+    // Range finder, after the Jacobi of B: dust on sigma, retained_chi, needs_f64_fallback
+    // (fallback jobs are marked for an exact split), factors, left = Q W.
+    void finish_rsvd(std::vector<JobInfo>& J, const SplitBatch& b, char* base) {
         auto P = [&](size_t o) { return reinterpret_cast<double2*>(base + o); };
         auto PD = [&](size_t o) { return reinterpret_cast<double*>(base + o); };
+        const std::vector<int>& ids = b.ids;
+        const std::vector<JacobiJob>& jj = b.jj;
         const int nr = (int)ids.size();
-        {
-            Timer tm(this, 2);
-            std::vector<GemmJob> g1, g2, g3;
-            for (int i : ids) {
-                JobInfo& j = J[i];
-                const int r = j.om->r;
-                // Y0 = A Omega_I; Z = A^H Y0; Y = A Z
-                g1.push_back(gj(P(j.o_theta), j.cols, 0, j.om->omega_cols, r, 0, P(j.o_Y0), r, j.rows, r, j.cols, 1, 0));
-                g2.push_back(gj(P(j.o_theta), j.cols, 1, P(j.o_Y0), r, 0, P(j.o_Z), r, j.cols, r, j.rows, 1, 0));
-                g3.push_back(gj(P(j.o_theta), j.cols, 0, P(j.o_Z), r, 0, P(j.o_Y), r, j.rows, r, j.cols, 1, 0));
-            }
-            gemm(g1);
-            gemm(g2);
-            gemm(g3);
-            std::vector<HouseJob> hj;
-            for (int i : ids) {
-                JobInfo& j = J[i];
-                if (j.om->r > wqc::kMaxHouseCols) fail(WQC_TN_EXEC_FAILED, "range finder rank above the QR limit");
-                HouseJob h;
-                h.Y = P(j.o_Y);
-                h.V = P(j.o_V);
-                h.T = P(j.o_T);
-                h.pos = j.om->pos;
-                h.rows = j.rows;
-                h.r = j.om->r;
-                h.pad0 = h.pad1 = 0;
-                h.tau = QR_RANK_TOL;
-                hj.push_back(h);
-            }
-            wqc::launch_householder(upload(hj), (int)hj.size(), stream);
-            check_launch();
-            klaunch[2]++;
-            // B = A[0:ell, :] - V[0:ell, :] (T^H (V^H A))
-            std::vector<GemmJob> g4, g5, g6;
-            for (int i : ids) {
-                JobInfo& j = J[i];
-                const int r = j.om->r;
-                g4.push_back(gj(P(j.o_V), r, 1, P(j.o_theta), j.cols, 0, P(j.o_Z2), j.cols, r, j.cols, j.rows, 1, 0));
-                g5.push_back(gj(P(j.o_T), r, 1, P(j.o_Z2), j.cols, 0, P(j.o_Z), j.cols, r, j.cols, r, 1, 0));
-                HIPC(hipMemcpyAsync(P(j.o_B), P(j.o_theta), j.ell * (size_t)j.cols * sizeof(double2),
-                                    hipMemcpyDeviceToDevice, stream));
-                g6.push_back(gj(P(j.o_V), r, 0, P(j.o_Z), j.cols, 0, P(j.o_B), j.cols, (int)j.ell, j.cols, r, -1, 1));
-            }
-            gemm(g4);
-            gemm(g5);
-            gemm(g6);
-        }
-        // checked_svd(B): dust rule, Jacobi on the rows of B (ell <= cols), norms.
-        std::vector<JacobiJob> jj(nr);
-        std::vector<double2*> mats(nr), Jm(nr);
-        std::vector<long> counts(nr), offs(nr);
-        std::vector<int> dims(nr);
-        int max_n = 1;
-        for (int k = 0; k < nr; ++k) {
-            JobInfo& j = J[ids[k]];
-            jj[k].X = P(j.o_B);
-            jj[k].J = P(j.o_JB);
-            jj[k].n = (int)j.ell;
-            jj[k].m = j.cols;
-            jj[k].ldx = j.cols;
-            mats[k] = P(j.o_B);
-            Jm[k] = P(j.o_JB);
-            counts[k] = (long)j.ell * j.cols;
-            dims[k] = (int)j.ell;
-            max_n = std::max(max_n, (int)j.ell);
-            offs[k] = (long)(j.o_sigB / sizeof(double));
-        }
-        dust(mats, counts, d_fro, d_part, nblk);
-        wqc::launch_identity(upload(Jm), upload(dims), nr, max_n, stream);
-        check_launch();
-        jacobi_any(jj, PD(0), offs, d_flags);
-        for (int k = 0; k < nr; ++k) {
-            J[ids[k]].sweeps += last_sweeps[k];
-            J[ids[k]].sweeps_lo += last_sweeps_lo[k];
-        }
-        size_t cur = 0;
-        std::vector<size_t> hoff(nr);
-        for (int k = 0; k < nr; ++k) {
-            hoff[k] = cur;
-            HIPC(hipMemcpyAsync(hread + cur, PD(J[ids[k]].o_sigB), jj[k].n * sizeof(double), hipMemcpyDeviceToHost, stream));
-            cur += jj[k].n;
-        }
-        HIPC(hipMemcpyAsync(hread + cur, d_fro, nr * sizeof(double), hipMemcpyDeviceToHost, stream));
-        sync();
         std::vector<FactorJob> fj;
         std::vector<int> kept;
         std::vector<std::vector<int>> perms(nr);
@@ -1308,8 +1365,8 @@ struct wqc_tn_state {
         for (int k = 0; k < nr; ++k) {
             JobInfo& j = J[ids[k]];
             const int nn = jj[k].n;
-            std::vector<double> sig(hread + hoff[k], hread + hoff[k] + nn);
-            const double fro2 = hread[cur + k];
+            std::vector<double> sig(hread + b.hoff[k], hread + b.hoff[k] + nn);
+            const double fro2 = hread[b.hfro + k];
             std::vector<int> order(nn);
             for (int i = 0; i < nn; ++i) order[i] = i;
             std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sig[a] > sig[b]; });

@@ -187,11 +187,11 @@ __global__ __launch_bounds__(256) void k_identity(double2* const* __restrict__ p
 //     (G <- M G M^H and T <- M T in one phase);
 //  3. P <- T P over the columns of X and of J, again on FP64 MFMA with T in registers.
 // A pair (i, j) needs a rotation when |G_ij| > tol sqrt(G_ii G_jj) (relative, as the
-// reference's Jacobi, src/tn/svd_guard.rs:178): small singular values come out accurate
-// relative to themselves, so the rank count above 1e-12 and the dust rule see the same
-// values an exact SVD gives. Rows of squared norm <= nu2 (the host passes 1e-26/n, so
-// that all such rows together hold at most 1e-13 of singular value, an order below the
-// 1e-12 cutoff) take part in no rotation: they cannot move a kept value or a decision,
+// reference's Jacobi, src/tn/svd_guard.rs:178), tol = tol_base sqrt(m) per job: small
+// singular values come out accurate relative to themselves, so the rank count above 1e-12
+// and the dust rule see the same values an exact SVD gives. Rows of squared norm <= nu2 =
+// nu2_base / n (the host passes 1e-26, so that all such rows of a job together hold at most
+// 1e-13 of singular value, an order below the 1e-12 cutoff) take part in no rotation: they cannot move a kept value or a decision,
 // and orthogonalizing rounding noise against itself would never settle. The in-block eigensolve runs at most JINNER sweeps; the
 // next outer round recomputes the pair's Gram matrix from the rows themselves, so an
 // unfinished inner solve only costs another round. rotated[job] records that the sweep
@@ -305,7 +305,7 @@ __device__ __forceinline__ d4v mfma_f64(double a, double b, d4v c) {
 }
 
 __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacobiJob* __restrict__ jobs,
-                                                      int round, double tol, double nu2,
+                                                      int round, double tol_base, double nu2_base,
                                                       int inner_cap, int* __restrict__ rotated,
                                                       const int* __restrict__ done,
                                                       unsigned long long* __restrict__ stats,
@@ -313,6 +313,9 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacobiJob* __rest
     const int job = blockIdx.y;
     if (done[job]) return;
     const JacobiJob jb = jobs[job];
+    // Per-job tolerance tol_base sqrt(m) and null-row floor nu2_base / n (see the header comment).
+    const double tol = tol_base * sqrt((double)jb.m);
+    const double nu2 = nu2_base / (double)jb.n;
     const int nb = (jb.n + JB - 1) / JB;
     const int nbe = nb + (nb & 1);
     if (round >= nbe - 1) return;
@@ -660,7 +663,7 @@ __device__ __forceinline__ f4v mfma_f32(float a, float b, f4v c) {
 }
 
 __global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __restrict__ jobs,
-                                                         int round, float tol, double nu2,
+                                                         int round, float tol_base, double nu2_base,
                                                          float rel_floor, int inner_cap,
                                                          int* __restrict__ rotated,
                                                          const int* __restrict__ done,
@@ -669,6 +672,8 @@ __global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __re
     const int job = blockIdx.y;
     if (done[job]) return;
     const JacobiLoJob jb = jobs[job];
+    const float tol = (float)((double)tol_base * sqrt((double)jb.m));
+    const double nu2 = nu2_base / (double)jb.n;
     const int nb = (jb.n + JB - 1) / JB;
     const int nbe = nb + (nb & 1);
     if (round >= nbe - 1) return;
