@@ -108,6 +108,7 @@ typedef struct wqc_tn_stats {
     uint64_t jacobi_inner_sweeps; /* sweeps of the per-block-pair eigensolves               */
     uint64_t jacobi_pair_solves;  /* block-pair eigensolves run                             */
     uint64_t jacobi_sweeps_lo;    /* low-precision sweeps of the mixed form                 */
+    uint64_t jacobi_stalled;      /* Jacobi jobs stopped at their rounding floor            */
 } wqc_tn_stats;
 
 /* ---- errors ------------------------------------------------------------------ */
@@ -279,7 +280,8 @@ int wqc_tn_get_stats(const wqc_tn_state *st, wqc_tn_stats *out);
 
 /* Device time (ms) accumulated per kernel family since the state was created, for
  * measurement: index 0 = one-qubit, 1 = merge, 2 = range-finder GEMMs, 3 = Jacobi,
- * 4 = factors, 5 = readouts. Returns the number of entries written. */
+ * 4 = factors, 5 = readouts, 6 = LQ preconditioning of the Jacobi. Returns the number of
+ * entries written. */
 int wqc_tn_kernel_times(const wqc_tn_state *st, double *ms, size_t len);
 int wqc_tn_enable_timing(wqc_tn_state *st, int enable);
 /* Launch counts per kernel family (same indexing as wqc_tn_kernel_times). */

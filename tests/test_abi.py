@@ -53,4 +53,4 @@ def test_null_arguments_are_rejected_without_a_device():
 
 def test_gate_struct_layout_matches_header():
     assert ct.sizeof(wqc_tn.Gate) == 24
-    assert ct.sizeof(wqc_tn.Stats) == 9 * 8 + 2 * 8 + 3 * 8
+    assert ct.sizeof(wqc_tn.Stats) == 9 * 8 + 2 * 8 + 4 * 8

@@ -105,8 +105,22 @@ const double NULL_ROWS_SQ = 1e-26;
 // stop at 1e-4 (where the FP64 finish takes over with quadratic convergence; measured best of
 // 1e-6, 1e-5, 1e-4 on saturated chi = 512 splits).
 const double JACOBI_TOL = 1e-14;
+// A job whose largest relative off-diagonal is at most this and no longer halves per sweep
+// stops: its rotations have reached their rounding floor (rows orthogonal to 3e-11 leave
+// singular values exact to ~1e-21 relative and vectors to ~3e-11). WQC_TN_JACOBI_STALL
+// overrides it (0 turns the rule off).
+const double STALL_OFF = 3e-11;
 const double JACOBI_TOL_LO = 1e-4;
 const int MAX_SWEEPS_LO = 30;
+// The low rounds run at most this many sweeps per job: they pay while the off-diagonal mass
+// falls linearly; after that, rows near FP32 resolution keep single jobs rotating without
+// helping the FP64 finish (config 3 with LQ: cap 30 -> 939 merges/s, 6 -> 1387, 3 -> 1363).
+const int LO_SWEEP_CAP = 6;
+// With LQ preconditioning (the mixed form), Jacobi jobs of at least this many rows (and at
+// most LQ_MAX_COLS columns) are rotated as R = L^H of a Householder LQ, X = L Q: on graded
+// and rank-deficient spectra the rows of R converge in about half the sweeps of the rows of
+// X, and a wide X (m = 2n) leaves an n x n R, half the work per sweep.
+const int LQ_MIN_N = 64;
 
 size_t retained_chi(const std::vector<double>& v, size_t max_bond, size_t rows, size_t cols) {
     if (v.empty() || max_bond == 0 || rows == 0 || cols == 0) return 0;
@@ -399,8 +413,8 @@ struct wqc_tn_state {
     wqc_tn_stats stats{};
     uint64_t site_bytes = 0;
     bool timing = false;
-    double kms[6] = {0, 0, 0, 0, 0, 0};
-    uint64_t klaunch[6] = {0, 0, 0, 0, 0, 0};
+    double kms[7] = {0, 0, 0, 0, 0, 0, 0};
+    uint64_t klaunch[7] = {0, 0, 0, 0, 0, 0, 0};
     struct TimedSpan {
         hipEvent_t a, b;
         int fam;
@@ -490,7 +504,10 @@ struct wqc_tn_state {
     int inner_cap_lo = 1;    // the same for the mixed form's low rounds (WQC_TN_JACOBI_INNER_LO)
     double tol_lo = JACOBI_TOL_LO;  // their relative tolerance (WQC_TN_JACOBI_TOL_LO)
     double floor_lo = 1e-12;        // squared-norm floor relative to a pair's largest row (WQC_TN_JACOBI_FLOOR_LO)
+    int max_sweeps_lo = LO_SWEEP_CAP;  // cap on the low sweeps per job (WQC_TN_JACOBI_MAX_LO)
     bool trace_jacobi = false;
+    double stall_off = STALL_OFF;
+    bool use_lq = true;
     DevBuf trace_buf;
 
     void init(wqc_tn_device* d, size_t qn, size_t mb, int prec) {
@@ -500,6 +517,13 @@ struct wqc_tn_state {
             if (v >= 1 && v <= 32) inner_cap = v;
         }
         if (const char* e = std::getenv("WQC_TN_JACOBI_TRACE")) trace_jacobi = std::atoi(e) != 0;
+        if (const char* e = std::getenv("WQC_TN_JACOBI_STALL")) stall_off = std::max(0.0, std::atof(e));
+        // LQ preconditioning: on in the mixed form, off in the plain FP64 form (whose Jacobi on
+        // the rows of theta' keeps small singular values relatively accurate; R = L^H carries
+        // the LQ's rounding at the scale of the largest rows, which moves near-cutoff
+        // decisions); WQC_TN_LQ=0/1 overrides either.
+        use_lq = prec == WQC_TN_PRECISION_MIXED_FP32;
+        if (const char* e = std::getenv("WQC_TN_LQ")) use_lq = std::atoi(e) != 0;
         if (const char* e = std::getenv("WQC_TN_JACOBI_INNER_LO")) {
             const int v = std::atoi(e);
             if (v >= 1 && v <= 32) inner_cap_lo = v;
@@ -507,6 +531,10 @@ struct wqc_tn_state {
         if (const char* e = std::getenv("WQC_TN_JACOBI_FLOOR_LO")) {
             const double v = std::atof(e);
             if (v >= 1e-14 && v <= 1e-2) floor_lo = v;
+        }
+        if (const char* e = std::getenv("WQC_TN_JACOBI_MAX_LO")) {
+            const int v = std::atoi(e);
+            if (v >= 1 && v <= MAX_SWEEPS_LO) max_sweeps_lo = v;
         }
         if (const char* e = std::getenv("WQC_TN_JACOBI_TOL_LO")) {
             const double v = std::atof(e);
@@ -662,13 +690,17 @@ struct wqc_tn_state {
         wqc::launch_fill_int(d_done, 0, nj, stream);
         std::vector<int> done(nj, 0);
         int active = nj;
-        // WQC_TN_JACOBI_TRACE=1 prints, per sweep, the largest relative off-diagonal
-        // |G_ij| / sqrt(G_ii G_jj) seen before rotation in each of the widest jobs.
-        unsigned long long* d_off = nullptr;
-        if (trace_jacobi) {
-            trace_buf.ensure((size_t)nj * sizeof(unsigned long long));
-            d_off = reinterpret_cast<unsigned long long*>(trace_buf.p);
-        }
+        // Largest relative off-diagonal |G_ij| / sqrt(G_ii G_jj) seen before rotation, per job
+        // and sweep (always collected): a job whose value is at most stall_off and did not
+        // halve since the previous sweep has reached the rounding floor of its rotations
+        // (quadratic convergence would have taken it far below) and stops.
+        // WQC_TN_JACOBI_TRACE=1 prints it per sweep for the widest jobs, and the history of
+        // the jobs that needed the most sweeps.
+        trace_buf.ensure((size_t)nj * sizeof(unsigned long long));
+        unsigned long long* d_off = reinterpret_cast<unsigned long long*>(trace_buf.p);
+        std::vector<double> prev_off(nj, 0.0);
+        std::vector<unsigned long long> offv(nj);
+        std::vector<std::vector<double>> hist(trace_jacobi ? nj : 0);
         for (int sweep = 0; sweep < MAX_SWEEPS && active > 0; ++sweep) {
             check_cancel();
             wqc::launch_fill_int(d_rot, 0, nj, stream);
@@ -682,35 +714,52 @@ struct wqc_tn_state {
                 klaunch[3] += rounds;
             }
             stats.jacobi_sweeps += active;
-            if (d_off) {
-                std::vector<unsigned long long> off(nj);
-                HIPC(hipMemcpyAsync(off.data(), d_off, nj * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
-                sync();
+            HIPC(hipMemcpyAsync(offv.data(), d_off, nj * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+            HIPC(hipMemcpyAsync(hread, d_rot, nj * sizeof(int), hipMemcpyDeviceToHost, stream));
+            sync();
+            const int* rot = reinterpret_cast<const int*>(hread);
+            if (trace_jacobi) {
                 std::fprintf(stderr, "jacobi sweep %d active %d:", sweep + 1, active);
                 int shown = 0;
                 for (int j = 0; j < nj && shown < 4; ++j) {
                     if (done[j] || jobs[j].n < max_n) continue;
                     double v;
-                    std::memcpy(&v, &off[j], sizeof v);
+                    std::memcpy(&v, &offv[j], sizeof v);
                     std::fprintf(stderr, " [%dx%d] %.2e", jobs[j].n, jobs[j].m, v);
                     ++shown;
                 }
                 std::fprintf(stderr, "\n");
             }
-            HIPC(hipMemcpyAsync(hread, d_rot, nj * sizeof(int), hipMemcpyDeviceToHost, stream));
-            sync();
-            const int* rot = reinterpret_cast<const int*>(hread);
             bool changed = false;
             for (int j = 0; j < nj; ++j)
                 if (!done[j]) last_sweeps[j]++;
-            for (int j = 0; j < nj; ++j)
-                if (!done[j] && !rot[j]) {
+            for (int j = 0; j < nj; ++j) {
+                if (done[j]) continue;
+                double v;
+                std::memcpy(&v, &offv[j], sizeof v);
+                const bool stalled = rot[j] && v <= stall_off && prev_off[j] > 0.0 && v >= 0.5 * prev_off[j];
+                prev_off[j] = v;
+                if (trace_jacobi) hist[j].push_back(rot[j] ? v : 0.0);
+                if (!rot[j] || stalled) {
                     done[j] = 1;
                     active--;
                     changed = true;
+                    if (stalled) stats.jacobi_stalled++;
                 }
+            }
             if (changed && active > 0)
                 HIPC(hipMemcpyAsync(d_done, upload_ints(done), nj * sizeof(int), hipMemcpyDeviceToDevice, stream));
+        }
+        if (trace_jacobi) {
+            std::vector<int> order(nj);
+            for (int j = 0; j < nj; ++j) order[j] = j;
+            std::sort(order.begin(), order.end(), [&](int a, int b) { return last_sweeps[a] > last_sweeps[b]; });
+            for (int k = 0; k < std::min(nj, 3); ++k) {
+                const int j = order[k];
+                std::fprintf(stderr, "jacobi job [%dx%d] sweeps %d:", jobs[j].n, jobs[j].m, last_sweeps[j]);
+                for (double v : hist[j]) std::fprintf(stderr, " %.1e", v);
+                std::fprintf(stderr, "\n");
+            }
         }
         std::vector<long> offs = sig_off;
         long* doff = upload(offs);
@@ -797,7 +846,7 @@ struct wqc_tn_state {
             trace_buf.ensure((size_t)nj * sizeof(unsigned long long));
             d_off = reinterpret_cast<unsigned long long*>(trace_buf.p);
         }
-        for (int sweep = 0; sweep < MAX_SWEEPS_LO && active > 0; ++sweep) {
+        for (int sweep = 0; sweep < max_sweeps_lo && active > 0; ++sweep) {
             check_cancel();
             wqc::launch_fill_int(d_rot, 0, nj, stream);
             if (d_off) wqc::launch_fill_int(reinterpret_cast<int*>(d_off), 0, 2 * nj, stream);
@@ -939,6 +988,7 @@ struct wqc_tn_state {
         // workspace offsets (bytes)
         size_t o_theta, o_thetaT, o_J, o_sig;
         size_t o_Y0, o_Z, o_Y, o_V, o_T, o_Z2, o_B, o_JB, o_sigB, o_W, o_W2;
+        size_t o_lqU = 0, o_lqUT = 0, o_lqUTh = 0, o_lqR = 0, o_lqW = 0;  // LQ preconditioning
         OmegaEntry* om;
         size_t chi_kept;
         bool fallback;
@@ -980,6 +1030,13 @@ struct wqc_tn_state {
             j.o_theta = take(rows * cols * C16);
             j.o_J = take(n_ex * n_ex * C16);
             j.o_sig = take(std::max(rows, cols) * sizeof(double));
+            if (use_lq && n_ex >= (size_t)LQ_MIN_N && std::max(rows, cols) <= (size_t)wqc::LQ_MAX_COLS) {
+                j.o_lqU = take(rows * cols * C16);
+                j.o_lqUT = take(rows * cols * C16);
+                j.o_lqUTh = take(rows * cols * C16);
+                j.o_lqR = take(n_ex * n_ex * C16);
+                j.o_lqW = take(n_ex * wqc::LQ_NB * C16);
+            }
             if (j.rsvd) {
                 j.om = &omega_for(dev, rows, cols, j.ell);
                 const size_t r = std::max(1, j.om->r);
@@ -1067,7 +1124,7 @@ struct wqc_tn_state {
             }
     }
 
-    void gemm(std::vector<GemmJob>& g) {
+    void gemm(std::vector<GemmJob>& g, int family = 2) {
         if (g.empty()) return;
         int mm = 1, mn = 1;
         for (auto& x : g) {
@@ -1076,7 +1133,7 @@ struct wqc_tn_state {
         }
         wqc::launch_gemm(upload(g), (int)g.size(), mm, mn, stream);
         check_launch();
-        klaunch[2]++;
+        klaunch[family]++;
     }
 
     static GemmJob gj(const double2* A, int lda, int opA, const double2* B, int ldb, int opB, double2* C,
@@ -1104,6 +1161,11 @@ struct wqc_tn_state {
     struct SplitBatch {
         std::vector<int> ids;
         std::vector<JacobiJob> jj;
+        // LQ-preconditioned jobs: jj[k].X is R; orig[k] (row stride orig_ld[k]) held X and,
+        // after the Jacobi, holds M = [J 0] Q^H.
+        std::vector<char> pre;
+        std::vector<double2*> orig;
+        std::vector<int> orig_ld;
         std::vector<long> offs;
         double* d_fro = nullptr;
         double* d_part = nullptr;
@@ -1237,10 +1299,112 @@ struct wqc_tn_state {
         check_launch();
     }
 
+    // Householder LQ of every eligible Jacobi job (X = L Q, panels of LQ_NB rows, trailing
+    // rows updated by two GEMMs per panel); the job then rotates the rows of R = L^H.
+    void lq_precondition(std::vector<JobInfo>& J, const std::vector<SplitBatch*>& batches, char* base,
+                         std::vector<wqc::LqJob>& lq, std::vector<double2*>& lqw) {
+        auto P = [&](size_t o) { return reinterpret_cast<double2*>(base + o); };
+        lq.clear();
+        lqw.clear();
+        int max_n = 0;
+        for (auto* b : batches) {
+            const size_t nb = b->ids.size();
+            b->pre.assign(nb, 0);
+            b->orig.assign(nb, nullptr);
+            b->orig_ld.assign(nb, 0);
+            for (size_t i = 0; i < nb; ++i) {
+                JobInfo& j = J[b->ids[i]];
+                JacobiJob& x = b->jj[i];
+                if (!j.o_lqU || x.n < LQ_MIN_N || x.m > wqc::LQ_MAX_COLS || x.n > x.m) continue;
+                wqc::LqJob q;
+                q.X = x.X;
+                q.Ubc = P(j.o_lqU);
+                q.UTc = P(j.o_lqUT);
+                q.UThc = P(j.o_lqUTh);
+                q.R = P(j.o_lqR);
+                q.J = x.J;
+                q.n = x.n;
+                q.m = x.m;
+                q.ldx = x.ldx;
+                q.pad = 0;
+                lq.push_back(q);
+                lqw.push_back(P(j.o_lqW));
+                b->pre[i] = 1;
+                b->orig[i] = x.X;
+                b->orig_ld[i] = x.ldx;
+                x.X = q.R;
+                x.m = x.n;
+                x.ldx = x.n;
+                max_n = std::max(max_n, q.n);
+            }
+        }
+        if (lq.empty()) return;
+        Timer tm(this, 6);
+        const int nl = (int)lq.size();
+        wqc::LqJob* dl = upload(lq);
+        const int NB = wqc::LQ_NB;
+        for (int j0 = 0; j0 < max_n; j0 += NB) {
+            int max_cols = 1;
+            for (const auto& q : lq)
+                if (q.n > j0) max_cols = std::max(max_cols, q.m - j0);
+            wqc::launch_lq_panel(dl, nl, j0, max_cols, stream);
+            check_launch();
+            klaunch[6]++;
+            std::vector<GemmJob> g1, g2;
+            for (int i = 0; i < nl; ++i) {
+                const wqc::LqJob& q = lq[i];
+                const int nbp = std::min(NB, q.n - j0);
+                const int nrest = q.n - j0 - nbp;
+                if (nbp <= 0 || nrest <= 0) continue;
+                const int K = q.m - j0;
+                double2* Xr = q.X + (size_t)(j0 + nbp) * q.ldx + j0;
+                g1.push_back(gj(Xr, q.ldx, 0, q.UTc + (size_t)j0 * q.m + j0, q.m, 1, lqw[i], NB, nrest, nbp, K, 1, 0));
+                g2.push_back(gj(lqw[i], NB, 0, q.Ubc + (size_t)j0 * q.m + j0, q.m, 0, Xr, q.ldx, nrest, K, nbp, -1, 1));
+            }
+            gemm(g1, 6);
+            gemm(g2, 6);
+        }
+        wqc::launch_lq_extract(dl, nl, max_n, stream);
+        check_launch();
+    }
+
+    // After the Jacobi of LQ-preconditioned jobs: X <- M = [J 0] Q_p^H ... Q_1^H.
+    void lq_form_m(const std::vector<wqc::LqJob>& lq, const std::vector<double2*>& lqw) {
+        if (lq.empty()) return;
+        Timer tm(this, 6);
+        const int nl = (int)lq.size();
+        wqc::LqJob* dl = upload(lq);
+        long mx = 1;
+        int max_n = 0;
+        for (const auto& q : lq) {
+            mx = std::max(mx, (long)q.n * q.m);
+            max_n = std::max(max_n, q.n);
+        }
+        wqc::launch_lq_init_m(dl, nl, mx, stream);
+        check_launch();
+        const int NB = wqc::LQ_NB;
+        for (int j0 = ((max_n - 1) / NB) * NB; j0 >= 0; j0 -= NB) {
+            std::vector<GemmJob> g1, g2;
+            for (int i = 0; i < nl; ++i) {
+                const wqc::LqJob& q = lq[i];
+                const int nbp = std::min(NB, q.n - j0);
+                if (nbp <= 0) continue;
+                const int K = q.m - j0;
+                g1.push_back(gj(q.X + j0, q.ldx, 0, q.UThc + (size_t)j0 * q.m + j0, q.m, 1, lqw[i], NB, q.n, nbp, K, 1, 0));
+                g2.push_back(gj(lqw[i], NB, 0, q.Ubc + (size_t)j0 * q.m + j0, q.m, 0, q.X + j0, q.ldx, q.n, K, nbp, -1, 1));
+            }
+            gemm(g1, 6);
+            gemm(g2, 6);
+        }
+    }
+
     // One Jacobi over the jobs of all batches (range-finder B and exact theta' of a layer
     // share the launches), then sigma and |A|_F^2 of every job read back with one sync.
     void jacobi_splits(std::vector<JobInfo>& J, const std::vector<SplitBatch*>& batches, char* base, int* d_flags) {
         auto PD = [&](size_t o) { return reinterpret_cast<double*>(base + o); };
+        std::vector<wqc::LqJob> lq;
+        std::vector<double2*> lqw;
+        lq_precondition(J, batches, base, lq, lqw);
         std::vector<JacobiJob> all;
         std::vector<long> offs;
         for (auto* b : batches) {
@@ -1249,6 +1413,15 @@ struct wqc_tn_state {
         }
         if (all.empty()) return;
         jacobi_any(all, PD(0), offs, d_flags);
+        lq_form_m(lq, lqw);
+        if (trace_jacobi) {
+            std::fprintf(stderr, "jacobi splits (site:rows x cols:sweeps):");
+            size_t t = 0;
+            for (auto* b : batches)
+                for (size_t i = 0; i < b->ids.size(); ++i, ++t)
+                    std::fprintf(stderr, " %d:%dx%d:%d", J[b->ids[i]].p, b->jj[i].n, b->jj[i].m, last_sweeps[t]);
+            std::fprintf(stderr, "\n");
+        }
         size_t k = 0, cur = 0;
         for (auto* b : batches) {
             b->hoff.assign(b->ids.size(), 0);
@@ -1305,6 +1478,11 @@ struct wqc_tn_state {
                                              " matrix failed its check" + buf);
             }
             const size_t c = retained_chi(sorted, chi, j.rows, j.cols);
+            if (trace_jacobi)
+                for (int i = 0; i < nn; ++i)
+                    if (sorted[i] > 0.2 * SINGULAR_CUTOFF && sorted[i] < 5 * SINGULAR_CUTOFF)
+                        std::fprintf(stderr, "near cutoff: site %d %dx%d kept %zu sigma[%d] = %.6e\n", j.p, j.rows,
+                                     j.cols, c, i, sorted[i]);
             bool dropped = false;
             double drop = 0.0;
             for (int i = (int)c; i < nn; ++i) {
@@ -1322,8 +1500,9 @@ struct wqc_tn_state {
             ensure_site_cap(j.p, (size_t)j.dl * 2 * c);
             ensure_site_cap(j.p + 1, c * 2 * (size_t)j.dr);
             const bool tall = j.rows > j.cols;
+            const bool pre = !b.pre.empty() && b.pre[k];
             fj[k].X = jj[k].X;
-            fj[k].J = jj[k].J;
+            fj[k].J = pre ? b.orig[k] : jj[k].J;
             fj[k].left = sites[j.p].ptr;
             fj[k].right = sites[j.p + 1].ptr;
             fj[k].perm = upload(perms[k]);
@@ -1332,8 +1511,8 @@ struct wqc_tn_state {
             fj[k].cols = j.cols;
             fj[k].chi = (int)c;
             fj[k].ldx = jj[k].ldx;
-            fj[k].ldj = nn;
-            fj[k].mode = tall ? 1 : 0;
+            fj[k].ldj = pre ? b.orig_ld[k] : nn;
+            fj[k].mode = (tall != pre) ? 1 : 0;
             mxl = std::max(mxl, (long)j.rows * (long)c);
             mxr = std::max(mxr, (long)c * j.cols);
         }
@@ -1405,8 +1584,9 @@ struct wqc_tn_state {
             ensure_site_cap(j.p, (size_t)j.dl * 2 * c);
             ensure_site_cap(j.p + 1, c * 2 * (size_t)j.dr);
             FactorJob f;
-            f.X = P(j.o_B);
-            f.J = P(j.o_JB);
+            const bool pre = !b.pre.empty() && b.pre[k];
+            f.X = jj[k].X;
+            f.J = pre ? b.orig[k] : jj[k].J;
             f.left = P(j.o_W);
             f.right = sites[j.p + 1].ptr;
             f.perm = upload(perms[k]);
@@ -1414,9 +1594,9 @@ struct wqc_tn_state {
             f.rows = j.rows;
             f.cols = j.cols;
             f.chi = (int)c;
-            f.ldx = j.cols;
-            f.ldj = (int)j.ell;
-            f.mode = 2;
+            f.ldx = jj[k].ldx;
+            f.ldj = pre ? b.orig_ld[k] : (int)j.ell;
+            f.mode = pre ? 3 : 2;
             f.pad0 = f.pad1 = 0;
             fj.push_back(f);
             kept.push_back(ids[k]);
@@ -2780,7 +2960,7 @@ int wqc_tn_get_stats(const wqc_tn_state* st, wqc_tn_stats* out) {
 
 int wqc_tn_kernel_times(const wqc_tn_state* st, double* ms, size_t len) {
     if (!st || !ms) return 0;
-    size_t k = std::min<size_t>(len, 6);
+    size_t k = std::min<size_t>(len, 7);
     for (size_t i = 0; i < k; ++i) ms[i] = st->kms[i];
     return (int)k;
 }
@@ -2793,7 +2973,7 @@ int wqc_tn_enable_timing(wqc_tn_state* st, int enable) {
 
 int wqc_tn_kernel_launches(const wqc_tn_state* st, uint64_t* count, size_t len) {
     if (!st || !count) return 0;
-    size_t k = std::min<size_t>(len, 6);
+    size_t k = std::min<size_t>(len, 7);
     for (size_t i = 0; i < k; ++i) count[i] = st->klaunch[i];
     return (int)k;
 }

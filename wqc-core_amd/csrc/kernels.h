@@ -66,7 +66,10 @@ struct GemmJob {
 //                                           right[k,c] = J[p_k, c] sqrt(s)
 //   mode 2 (range finder): right[k,c] = X[p_k, c] / sqrt(s);
 //                          left (as W, ell x chi): W[t,k] = conj(J[p_k, t]) sqrt(s)
-// p_k = perm[k], s = sigma[p_k]; s == 0 writes zeros.
+//   mode 3 (range finder, LQ-preconditioned): left (as W, ldx x chi) as in mode 1,
+//                          right as in mode 1
+// p_k = perm[k], s = sigma[p_k]; s == 0 writes zeros. After an LQ-preconditioned Jacobi
+// the roles swap: X holds the rotated rows of R (norms s) and J the rows of [J 0] Q^H.
 struct FactorJob {
     const double2* X;
     const double2* J;
@@ -76,6 +79,29 @@ struct FactorJob {
     const double* sigma;
     int rows, cols, chi, ldx, ldj, mode, pad0, pad1;
 };
+
+// Householder LQ of a Jacobi working matrix X (n x m, row stride ldx, n <= m) in panels of
+// LQ_NB rows: X Q_1 Q_2 ... Q_p = [L 0], L lower triangular (n x n), Q_i = I - U_i T_i U_i^H
+// the compact WY form of panel i's reflectors (LAPACK zlarfg / zlarft forward-columnwise
+// conventions). k_lq_panel factors one panel in place (L's rows stay in X) and writes, in
+// the panel's rows of three n x m row-major buffers (row stride m, columns >= the panel
+// start): Ubc = U^H (unit diagonal, zeros to its left), UTc = (U T)^H and UThc = (U T^H)^H.
+// The rows below the panel take X_rest <- X_rest - (X_rest UTc^H) Ubc (two GEMMs). After
+// the last panel R = L^H (launch_lq_extract) is the Jacobi matrix; with J R = Y (rows
+// orthogonal), X = Y^H M where M = [J 0] Q_p^H ... Q_1^H has orthonormal rows
+// (launch_lq_init_m writes [J 0] into X, then per panel in reverse M <- M - (M UThc^H) Ubc).
+struct LqJob {
+    double2* X;
+    double2* Ubc;
+    double2* UTc;
+    double2* UThc;
+    double2* R;        // n x n
+    const double2* J;  // n x n
+    int n, m, ldx, pad;
+};
+constexpr int LQ_NB = 16;
+constexpr int LQ_NT = 512;
+constexpr int LQ_MAX_COLS = 2 * LQ_NT;
 
 // Rank-thresholded Householder QR of the independent columns of Y (range finder).
 // Y (rows x r, row stride r) is overwritten. Column k sits at position pos[k] of the
@@ -130,6 +156,10 @@ void launch_gemm(const GemmJob* jobs, int njobs, int max_m, int max_n, hipStream
 void launch_factor(const FactorJob* jobs, int njobs, long max_left, long max_right,
                    hipStream_t s);
 void launch_householder(const HouseJob* jobs, int njobs, hipStream_t s);
+// Panel starting at row j0 of every job with n > j0; max_cols = the largest m - j0.
+void launch_lq_panel(const LqJob* jobs, int njobs, int j0, int max_cols, hipStream_t s);
+void launch_lq_extract(const LqJob* jobs, int njobs, int max_n, hipStream_t s);
+void launch_lq_init_m(const LqJob* jobs, int njobs, long max_elems, hipStream_t s);
 constexpr int kVecMatChunk = 16;  // rows per workgroup of the left pass
 void launch_vecmat(const VecMatJob* job, int out_len, int in_len, int right, hipStream_t s);
 void launch_fill_int(int* p, int value, int count, hipStream_t s);

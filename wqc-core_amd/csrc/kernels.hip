@@ -1161,8 +1161,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(const GemmJob* __restrict_
 __global__ __launch_bounds__(256) void k_factor(const FactorJob* __restrict__ jobs) {
     const FactorJob jb = jobs[blockIdx.y];
     const bool left = blockIdx.z == 0;
-    // mode 2 writes W (ell x chi) into `left`; its row count is the Jacobi dimension.
-    const long nrows = left ? (jb.mode == 2 ? (long)jb.ldj : (long)jb.rows) : (long)jb.chi;
+    // modes 2 and 3 write W (ell x chi) into `left`; its row count is the Jacobi dimension.
+    const long nrows = left ? (jb.mode == 2 ? (long)jb.ldj : jb.mode == 3 ? (long)jb.ldx : (long)jb.rows)
+                            : (long)jb.chi;
+    const bool swapped = jb.mode == 1 || jb.mode == 3;
     const long ncols = left ? (long)jb.chi : (long)jb.cols;
     const long total = nrows * ncols;
     for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
@@ -1172,7 +1174,7 @@ __global__ __launch_bounds__(256) void k_factor(const FactorJob* __restrict__ jo
             const int p = jb.perm[j];
             const double s = jb.sigma[p];
             if (s > 0.0) {
-                if (jb.mode == 1)
+                if (swapped)
                     v = cscale(cconj(jb.X[(long)p * jb.ldx + i]), 1.0 / sqrt(s));
                 else
                     v = cscale(cconj(jb.J[(long)p * jb.ldj + i]), sqrt(s));
@@ -1182,13 +1184,198 @@ __global__ __launch_bounds__(256) void k_factor(const FactorJob* __restrict__ jo
             const int p = jb.perm[i];
             const double s = jb.sigma[p];
             if (s > 0.0) {
-                if (jb.mode == 1)
+                if (swapped)
                     v = cscale(jb.J[(long)p * jb.ldj + j], sqrt(s));
                 else
                     v = cscale(jb.X[(long)p * jb.ldx + j], 1.0 / sqrt(s));
             }
             jb.right[e] = v;
         }
+    }
+}
+
+// ------------------------------------------------------------------ Householder LQ
+// One workgroup (8 waves) per job factors rows j0 .. j0+nbp-1 of X (see LqJob). Thread tid
+// holds columns j0 + tid + LQ_NT k (k < CPT) of the panel rows in registers. For panel
+// row t (diagonal column j = j0 + t), zlarfg on the column vector conj(x[t, j:]) gives
+// beta (real), tau and u (u_j = 1) with x[t, j:] (I - tau u u^H) = beta e_0; the panel rows
+// below take the same reflector, and T gains column t, T[0:t, t] = -tau T[0:t, 0:t] U^H u.
+// Once row t is done its slots right of the diagonal hold u. Sums are fixed-order (wave
+// shuffles, then the wave partials in order), so the factorization is bit-reproducible.
+template <int V>
+__device__ __forceinline__ void lq_block_sum(double (&v)[V], double (*red)[V], double* fin, int tid) {
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int i = 0; i < V; ++i) v[i] = wave_sum(v[i]);
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < V; ++i) red[wave][i] = v[i];
+    }
+    __syncthreads();
+    if (tid < V) {
+        double acc = 0.0;
+        for (int w = 0; w < LQ_NT / 64; ++w) acc += red[w][tid];
+        fin[tid] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < V; ++i) v[i] = fin[i];
+}
+
+// Shared state of one panel factorization.
+struct LqShared {
+    double red1[LQ_NT / 64][1];
+    double fin1[1];
+    double redv[LQ_NT / 64][2 * LQ_NB];
+    double finv[2 * LQ_NB];
+    double2 alpha;
+    double2 T[LQ_NB][LQ_NB];
+    double beta[LQ_NB];
+    double2 z[LQ_NB];
+};
+
+// Step T of the panel (row T, diagonal column j0 + T); a template so that every register
+// row index is a compile-time constant.
+template <int CPT, int T>
+__device__ __forceinline__ void lq_step(double2 (&x)[LQ_NB][CPT], LqShared& sh, int j0, int m, int nbp, int tid) {
+    if constexpr (T < LQ_NB) {
+        if (T < nbp) {
+            const double2 one = make_double2(1.0, 0.0);
+            const int j = j0 + T;
+            double nrm[1] = {0.0};
+#pragma unroll
+            for (int k = 0; k < CPT; ++k) {
+                const int c = j0 + tid + k * LQ_NT;
+                if (c > j && c < m) nrm[0] = fma(x[T][k].x, x[T][k].x, fma(x[T][k].y, x[T][k].y, nrm[0]));
+                if (c == j) sh.alpha = x[T][k];
+            }
+            lq_block_sum<1>(nrm, sh.red1, sh.fin1, tid);
+            const double2 ac = cconj(sh.alpha);
+            double2 tau = cz(), scale = cz();
+            double beta = ac.x;
+            if (nrm[0] > 0.0 || ac.y != 0.0) {
+                beta = -copysign(sqrt(fma(ac.x, ac.x, fma(ac.y, ac.y, nrm[0]))), ac.x);
+                tau = make_double2((beta - ac.x) / beta, -ac.y / beta);
+                const double dx = ac.x - beta, dy = ac.y;
+                const double dd = 1.0 / fma(dx, dx, dy * dy);
+                scale = make_double2(dx * dd, -dy * dd);
+            }
+#pragma unroll
+            for (int k = 0; k < CPT; ++k) {
+                const int c = j0 + tid + k * LQ_NT;
+                if (c > j && c < m) x[T][k] = cmul(cconj(x[T][k]), scale);
+            }
+            // acc[v]: w_v = x[v, j:] u (v > T) and g_v = u_v^H u (v < T).
+            double acc[2 * LQ_NB];
+#pragma unroll
+            for (int v = 0; v < 2 * LQ_NB; ++v) acc[v] = 0.0;
+#pragma unroll
+            for (int k = 0; k < CPT; ++k) {
+                const int c = j0 + tid + k * LQ_NT;
+                if (c < j || c >= m) continue;
+                const double2 u = c == j ? one : x[T][k];
+#pragma unroll
+                for (int v = 0; v < LQ_NB; ++v) {
+                    if (v == T) continue;
+                    const double2 p = v > T ? cmul(x[v][k], u) : cmul(cconj(x[v][k]), u);
+                    acc[2 * v] += p.x;
+                    acc[2 * v + 1] += p.y;
+                }
+            }
+            lq_block_sum<2 * LQ_NB>(acc, sh.redv, sh.finv, tid);
+#pragma unroll
+            for (int v = T + 1; v < LQ_NB; ++v) {
+                const double2 coef = cmul(tau, make_double2(acc[2 * v], acc[2 * v + 1]));
+#pragma unroll
+                for (int k = 0; k < CPT; ++k) {
+                    const int c = j0 + tid + k * LQ_NT;
+                    if (c < j || c >= m) continue;
+                    const double2 u = c == j ? one : x[T][k];
+                    x[v][k] = csub(x[v][k], cmul(coef, cconj(u)));
+                }
+            }
+            if (tid == 0) {
+                sh.beta[T] = beta;
+#pragma unroll
+                for (int i = 0; i < T; ++i)
+                    sh.z[i] = cmul(make_double2(-tau.x, -tau.y), make_double2(acc[2 * i], acc[2 * i + 1]));
+                for (int i = 0; i < T; ++i) {
+                    double2 a = cz();
+                    for (int l = i; l < T; ++l) cmac(a, sh.T[i][l], sh.z[l]);
+                    sh.T[i][T] = a;
+                }
+                sh.T[T][T] = tau;
+            }
+        }
+        lq_step<CPT, T + 1>(x, sh, j0, m, nbp, tid);
+    }
+}
+
+template <int CPT>
+__global__ __launch_bounds__(LQ_NT) void k_lq_panel(const LqJob* __restrict__ jobs, int j0) {
+    const LqJob jb = jobs[blockIdx.x];
+    if (j0 >= jb.n || jb.m - j0 > CPT * LQ_NT) return;
+    const int nbp = min(LQ_NB, jb.n - j0);
+    const int tid = threadIdx.x;
+    __shared__ LqShared sh;
+    double2 x[LQ_NB][CPT];
+#pragma unroll
+    for (int r = 0; r < LQ_NB; ++r)
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const int c = j0 + tid + k * LQ_NT;
+            x[r][k] = (r < nbp && c < jb.m) ? jb.X[(long)(j0 + r) * jb.ldx + c] : cz();
+        }
+    if (tid < LQ_NB * LQ_NB) sh.T[tid / LQ_NB][tid % LQ_NB] = cz();
+    lq_step<CPT, 0>(x, sh, j0, jb.m, nbp, tid);
+    __syncthreads();
+    const double2 one = make_double2(1.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int c = j0 + tid + k * LQ_NT;
+        if (c >= jb.m) continue;
+        double2 u[LQ_NB];
+#pragma unroll
+        for (int t = 0; t < LQ_NB; ++t) {
+            const int j = j0 + t;
+            u[t] = (t >= nbp || c < j) ? cz() : (c == j ? one : x[t][k]);
+            if (t < nbp) {
+                jb.X[(long)j * jb.ldx + c] = c < j ? x[t][k] : (c == j ? make_double2(sh.beta[t], 0.0) : cz());
+                jb.Ubc[(long)j * jb.m + c] = cconj(u[t]);
+            }
+        }
+#pragma unroll
+        for (int tp = 0; tp < LQ_NB; ++tp) {
+            if (tp >= nbp) continue;
+            double2 a = cz(), b = cz();
+#pragma unroll
+            for (int t = 0; t < LQ_NB; ++t) {
+                if (t <= tp) cmac(a, u[t], sh.T[t][tp]);
+                if (t >= tp && t < nbp) cmac(b, u[t], cconj(sh.T[tp][t]));
+            }
+            jb.UTc[(long)(j0 + tp) * jb.m + c] = cconj(a);
+            jb.UThc[(long)(j0 + tp) * jb.m + c] = cconj(b);
+        }
+    }
+}
+
+// R = L^H (n x n, row stride n) from the lower triangle of the factored X.
+__global__ __launch_bounds__(256) void k_lq_extract(const LqJob* __restrict__ jobs) {
+    const LqJob jb = jobs[blockIdx.y];
+    const long total = (long)jb.n * jb.n;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+        const int i = (int)(e / jb.n), k = (int)(e - (long)i * jb.n);
+        jb.R[e] = k >= i ? cconj(jb.X[(long)k * jb.ldx + i]) : cz();
+    }
+}
+
+// X <- [J 0] (n x m, row stride ldx).
+__global__ __launch_bounds__(256) void k_lq_init_m(const LqJob* __restrict__ jobs) {
+    const LqJob jb = jobs[blockIdx.y];
+    const long total = (long)jb.n * jb.m;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+        const int i = (int)(e / jb.m), c = (int)(e - (long)i * jb.m);
+        jb.X[(long)i * jb.ldx + c] = c < jb.n ? jb.J[(long)i * jb.n + c] : cz();
     }
 }
 
@@ -1611,6 +1798,22 @@ void launch_factor(const FactorJob* jobs, int njobs, long max_left, long max_rig
 
 void launch_householder(const HouseJob* jobs, int njobs, hipStream_t s) {
     hipLaunchKernelGGL(k_householder, dim3(njobs), dim3(256), 0, s, jobs);
+}
+
+void launch_lq_panel(const LqJob* jobs, int njobs, int j0, int max_cols, hipStream_t s) {
+    if (max_cols <= LQ_NT)
+        hipLaunchKernelGGL(k_lq_panel<1>, dim3(njobs), dim3(LQ_NT), 0, s, jobs, j0);
+    else
+        hipLaunchKernelGGL(k_lq_panel<2>, dim3(njobs), dim3(LQ_NT), 0, s, jobs, j0);
+}
+
+void launch_lq_extract(const LqJob* jobs, int njobs, int max_n, hipStream_t s) {
+    hipLaunchKernelGGL(k_lq_extract, dim3(blocks_for((long)max_n * max_n, 256, 1024), njobs), dim3(256), 0, s,
+                       jobs);
+}
+
+void launch_lq_init_m(const LqJob* jobs, int njobs, long max_elems, hipStream_t s) {
+    hipLaunchKernelGGL(k_lq_init_m, dim3(blocks_for(max_elems, 256, 1024), njobs), dim3(256), 0, s, jobs);
 }
 
 void launch_vecmat(const VecMatJob* job, int out_len, int in_len, int right, hipStream_t s) {

@@ -53,7 +53,8 @@ class Stats(ct.Structure):
                 ("jacobi_sweeps", ct.c_uint64), ("truncated_splits", ct.c_uint64),
                 ("peak_vram_bytes", ct.c_uint64), ("max_discarded_weight", ct.c_double),
                 ("sum_discarded_weight", ct.c_double), ("jacobi_inner_sweeps", ct.c_uint64),
-                ("jacobi_pair_solves", ct.c_uint64), ("jacobi_sweeps_lo", ct.c_uint64)]
+                ("jacobi_pair_solves", ct.c_uint64), ("jacobi_sweeps_lo", ct.c_uint64),
+                ("jacobi_stalled", ct.c_uint64)]
 
 
 class WqcTnError(RuntimeError):
@@ -369,17 +370,17 @@ class MpsState:
     def enable_timing(self, on=True):
         _check(load().wqc_tn_enable_timing(self._h, 1 if on else 0))
 
-    FAMILIES = ("one_qubit", "merge", "range_finder", "jacobi", "factors", "readout")
+    FAMILIES = ("one_qubit", "merge", "range_finder", "jacobi", "factors", "readout", "lq")
 
     def kernel_times(self):
         """Device milliseconds per kernel family (timing must be enabled)."""
-        out = np.zeros(6)
-        k = load().wqc_tn_kernel_times(self._h, _dp(out), 6)
+        out = np.zeros(len(self.FAMILIES))
+        k = load().wqc_tn_kernel_times(self._h, _dp(out), len(self.FAMILIES))
         return dict(zip(self.FAMILIES[:k], out[:k].tolist()))
 
     def kernel_launches(self):
-        out = (ct.c_uint64 * 6)()
-        k = load().wqc_tn_kernel_launches(self._h, out, 6)
+        out = (ct.c_uint64 * len(self.FAMILIES))()
+        k = load().wqc_tn_kernel_launches(self._h, out, len(self.FAMILIES))
         return dict(zip(self.FAMILIES[:k], [int(x) for x in out[:k]]))
 
     def enable_split_log(self, on=True):
