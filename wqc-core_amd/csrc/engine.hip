@@ -988,7 +988,7 @@ struct wqc_tn_state {
         // workspace offsets (bytes)
         size_t o_theta, o_thetaT, o_J, o_sig;
         size_t o_Y0, o_Z, o_Y, o_V, o_T, o_Z2, o_B, o_JB, o_sigB, o_W, o_W2;
-        size_t o_lqU = 0, o_lqUT = 0, o_lqUTh = 0, o_lqR = 0, o_lqW = 0;  // LQ preconditioning
+        size_t o_lqU = 0, o_lqUT = 0, o_lqUTh = 0, o_lqR = 0, o_lqW = 0, o_lqT = 0, o_lqG = 0;  // LQ preconditioning
         OmegaEntry* om;
         size_t chi_kept;
         bool fallback;
@@ -1035,7 +1035,9 @@ struct wqc_tn_state {
                 j.o_lqUT = take(rows * cols * C16);
                 j.o_lqUTh = take(rows * cols * C16);
                 j.o_lqR = take(n_ex * n_ex * C16);
-                j.o_lqW = take(n_ex * wqc::LQ_NB * C16);
+                j.o_lqW = take(n_ex * wqc::LQ_SB * C16);
+                j.o_lqT = take((size_t)wqc::LQ_SB * wqc::LQ_SB * C16);
+                j.o_lqG = take((size_t)wqc::LQ_SB * wqc::LQ_SB * C16);
             }
             if (j.rsvd) {
                 j.om = &omega_for(dev, rows, cols, j.ell);
@@ -1323,6 +1325,8 @@ struct wqc_tn_state {
                 q.UThc = P(j.o_lqUTh);
                 q.R = P(j.o_lqR);
                 q.J = x.J;
+                q.T64 = P(j.o_lqT);
+                q.Gu = P(j.o_lqG);
                 q.n = x.n;
                 q.m = x.m;
                 q.ldx = x.ldx;
@@ -1342,25 +1346,54 @@ struct wqc_tn_state {
         Timer tm(this, 6);
         const int nl = (int)lq.size();
         wqc::LqJob* dl = upload(lq);
-        const int NB = wqc::LQ_NB;
-        for (int j0 = 0; j0 < max_n; j0 += NB) {
-            int max_cols = 1;
-            for (const auto& q : lq)
-                if (q.n > j0) max_cols = std::max(max_cols, q.m - j0);
-            wqc::launch_lq_panel(dl, nl, j0, max_cols, stream);
-            check_launch();
-            klaunch[6]++;
-            std::vector<GemmJob> g1, g2;
+        const int NB = wqc::LQ_NB, SB = wqc::LQ_SB;
+        for (int b0 = 0; b0 < max_n; b0 += SB) {
+            // Panels of the block; each updates the block's later rows.
+            for (int j0 = b0; j0 < std::min(b0 + SB, max_n); j0 += NB) {
+                int max_cols = 1;
+                for (const auto& q : lq)
+                    if (q.n > j0) max_cols = std::max(max_cols, q.m - b0);
+                wqc::launch_lq_panel(dl, nl, b0, j0, max_cols, stream);
+                check_launch();
+                klaunch[6]++;
+                std::vector<GemmJob> g1, g2;
+                for (int i = 0; i < nl; ++i) {
+                    const wqc::LqJob& q = lq[i];
+                    const int nbp = std::min(NB, q.n - j0);
+                    const int nrest = std::min(b0 + SB, q.n) - j0 - nbp;
+                    if (nbp <= 0 || nrest <= 0) continue;
+                    const int K = q.m - j0;
+                    double2* Xr = q.X + (size_t)(j0 + nbp) * q.ldx + j0;
+                    g1.push_back(gj(Xr, q.ldx, 0, q.UTc + (size_t)j0 * q.m + j0, q.m, 1, lqw[i], SB, nrest, nbp, K, 1, 0));
+                    g2.push_back(gj(lqw[i], SB, 0, q.Ubc + (size_t)j0 * q.m + j0, q.m, 0, Xr, q.ldx, nrest, K, nbp, -1, 1));
+                }
+                gemm(g1, 6);
+                gemm(g2, 6);
+            }
+            // The block's T: Gu = U^H U, then the off-diagonal panels of T.
+            std::vector<GemmJob> gg, gt, gh, g1, g2;
             for (int i = 0; i < nl; ++i) {
                 const wqc::LqJob& q = lq[i];
-                const int nbp = std::min(NB, q.n - j0);
-                const int nrest = q.n - j0 - nbp;
-                if (nbp <= 0 || nrest <= 0) continue;
-                const int K = q.m - j0;
-                double2* Xr = q.X + (size_t)(j0 + nbp) * q.ldx + j0;
-                g1.push_back(gj(Xr, q.ldx, 0, q.UTc + (size_t)j0 * q.m + j0, q.m, 1, lqw[i], NB, nrest, nbp, K, 1, 0));
-                g2.push_back(gj(lqw[i], NB, 0, q.Ubc + (size_t)j0 * q.m + j0, q.m, 0, Xr, q.ldx, nrest, K, nbp, -1, 1));
+                const int nbs = std::min(SB, q.n - b0);
+                if (nbs <= 0) continue;
+                const int K = q.m - b0;
+                const double2* Ub = q.Ubc + (size_t)b0 * q.m + b0;
+                gg.push_back(gj(Ub, q.m, 0, Ub, q.m, 1, const_cast<double2*>(q.Gu), SB, nbs, nbs, K, 1, 0));
+                gt.push_back(gj(q.T64, SB, 1, Ub, q.m, 0, q.UTc + (size_t)b0 * q.m + b0, q.m, nbs, K, nbs, 1, 0));
+                gh.push_back(gj(q.T64, SB, 0, Ub, q.m, 0, q.UThc + (size_t)b0 * q.m + b0, q.m, nbs, K, nbs, 1, 0));
+                const int nrest = q.n - b0 - nbs;
+                if (nrest <= 0) continue;
+                double2* Xr = q.X + (size_t)(b0 + nbs) * q.ldx + b0;
+                g1.push_back(gj(Xr, q.ldx, 0, q.UTc + (size_t)b0 * q.m + b0, q.m, 1, lqw[i], SB, nrest, nbs, K, 1, 0));
+                g2.push_back(gj(lqw[i], SB, 0, Ub, q.m, 0, Xr, q.ldx, nrest, K, nbs, -1, 1));
             }
+            gemm(gg, 6);
+            wqc::launch_lq_tblock(dl, nl, b0, stream);
+            check_launch();
+            klaunch[6]++;
+            gemm(gt, 6);
+            gemm(gh, 6);
+            // Rows below the block.
             gemm(g1, 6);
             gemm(g2, 6);
         }
@@ -1382,16 +1415,16 @@ struct wqc_tn_state {
         }
         wqc::launch_lq_init_m(dl, nl, mx, stream);
         check_launch();
-        const int NB = wqc::LQ_NB;
-        for (int j0 = ((max_n - 1) / NB) * NB; j0 >= 0; j0 -= NB) {
+        const int SB = wqc::LQ_SB;
+        for (int b0 = ((max_n - 1) / SB) * SB; b0 >= 0; b0 -= SB) {
             std::vector<GemmJob> g1, g2;
             for (int i = 0; i < nl; ++i) {
                 const wqc::LqJob& q = lq[i];
-                const int nbp = std::min(NB, q.n - j0);
-                if (nbp <= 0) continue;
-                const int K = q.m - j0;
-                g1.push_back(gj(q.X + j0, q.ldx, 0, q.UThc + (size_t)j0 * q.m + j0, q.m, 1, lqw[i], NB, q.n, nbp, K, 1, 0));
-                g2.push_back(gj(lqw[i], NB, 0, q.Ubc + (size_t)j0 * q.m + j0, q.m, 0, q.X + j0, q.ldx, q.n, K, nbp, -1, 1));
+                const int nbs = std::min(SB, q.n - b0);
+                if (nbs <= 0) continue;
+                const int K = q.m - b0;
+                g1.push_back(gj(q.X + b0, q.ldx, 0, q.UThc + (size_t)b0 * q.m + b0, q.m, 1, lqw[i], SB, q.n, nbs, K, 1, 0));
+                g2.push_back(gj(lqw[i], SB, 0, q.Ubc + (size_t)b0 * q.m + b0, q.m, 0, q.X + b0, q.ldx, q.n, K, nbs, -1, 1));
             }
             gemm(g1, 6);
             gemm(g2, 6);

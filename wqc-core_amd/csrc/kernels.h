@@ -80,26 +80,32 @@ struct FactorJob {
     int rows, cols, chi, ldx, ldj, mode, pad0, pad1;
 };
 
-// Householder LQ of a Jacobi working matrix X (n x m, row stride ldx, n <= m) in panels of
-// LQ_NB rows: X Q_1 Q_2 ... Q_p = [L 0], L lower triangular (n x n), Q_i = I - U_i T_i U_i^H
-// the compact WY form of panel i's reflectors (LAPACK zlarfg / zlarft forward-columnwise
-// conventions). k_lq_panel factors one panel in place (L's rows stay in X) and writes, in
-// the panel's rows of three n x m row-major buffers (row stride m, columns >= the panel
-// start): Ubc = U^H (unit diagonal, zeros to its left), UTc = (U T)^H and UThc = (U T^H)^H.
-// The rows below the panel take X_rest <- X_rest - (X_rest UTc^H) Ubc (two GEMMs). After
-// the last panel R = L^H (launch_lq_extract) is the Jacobi matrix; with J R = Y (rows
-// orthogonal), X = Y^H M where M = [J 0] Q_p^H ... Q_1^H has orthonormal rows
-// (launch_lq_init_m writes [J 0] into X, then per panel in reverse M <- M - (M UThc^H) Ubc).
+// Householder LQ of a Jacobi working matrix X (n x m, row stride ldx, n <= m):
+// X Q_1 Q_2 ... Q_p = [L 0], L lower triangular (n x n), each Q_i = I - U_i T_i U_i^H the
+// compact WY form of a block of LQ_SB rows' reflectors (LAPACK zlarfg / zlarft forward-
+// columnwise conventions). A block is factored as LQ_SB / LQ_NB register-resident panels:
+// k_lq_panel leaves L's rows in X, writes the panel's rows of Ubc = U^H (unit diagonal,
+// zeros to its left, from the block's first column on) and of UTc = (U T_panel)^H, and its
+// T_panel into the diagonal of the block's T (T64, LQ_SB x LQ_SB); the block's later rows take
+// X_rows -= (X_rows UTc^H) Ubc (two GEMMs). Then Gu = Ubc Ubc^H (GEMM), k_lq_tblock completes
+// T64 (T[0:j, j] = -tau_j T[0:j, 0:j] Gu[0:j, j]), and the block's UTc = T64^H Ubc and
+// UThc = T64 Ubc (GEMMs) serve the rows below, X_rest -= (X_rest UTc^H) Ubc. After the last
+// block R = L^H (launch_lq_extract) is the Jacobi matrix; with J R = Y (rows orthogonal),
+// X = Y^H M where M = [J 0] Q_p^H ... Q_1^H has orthonormal rows (launch_lq_init_m writes
+// [J 0] into X, then per block in reverse M <- M - (M UThc^H) Ubc).
 struct LqJob {
     double2* X;
-    double2* Ubc;
-    double2* UTc;
-    double2* UThc;
+    double2* Ubc;      // n x m
+    double2* UTc;      // n x m
+    double2* UThc;     // n x m
     double2* R;        // n x n
     const double2* J;  // n x n
+    double2* T64;      // LQ_SB x LQ_SB (the current block's T)
+    const double2* Gu; // LQ_SB x LQ_SB (the current block's U^H U)
     int n, m, ldx, pad;
 };
 constexpr int LQ_NB = 16;
+constexpr int LQ_SB = 64;
 constexpr int LQ_NT = 512;
 constexpr int LQ_MAX_COLS = 2 * LQ_NT;
 
@@ -156,8 +162,11 @@ void launch_gemm(const GemmJob* jobs, int njobs, int max_m, int max_n, hipStream
 void launch_factor(const FactorJob* jobs, int njobs, long max_left, long max_right,
                    hipStream_t s);
 void launch_householder(const HouseJob* jobs, int njobs, hipStream_t s);
-// Panel starting at row j0 of every job with n > j0; max_cols = the largest m - j0.
-void launch_lq_panel(const LqJob* jobs, int njobs, int j0, int max_cols, hipStream_t s);
+// Panel starting at row j0 of the block starting at row b0, for every job with n > j0;
+// max_cols = the largest m - b0.
+void launch_lq_panel(const LqJob* jobs, int njobs, int b0, int j0, int max_cols, hipStream_t s);
+// T of the block starting at row b0 (rows b0 .. b0 + min(LQ_SB, n - b0) - 1).
+void launch_lq_tblock(const LqJob* jobs, int njobs, int b0, hipStream_t s);
 void launch_lq_extract(const LqJob* jobs, int njobs, int max_n, hipStream_t s);
 void launch_lq_init_m(const LqJob* jobs, int njobs, long max_elems, hipStream_t s);
 constexpr int kVecMatChunk = 16;  // rows per workgroup of the left pass

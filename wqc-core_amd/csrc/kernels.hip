@@ -1196,12 +1196,13 @@ __global__ __launch_bounds__(256) void k_factor(const FactorJob* __restrict__ jo
 
 // ------------------------------------------------------------------ Householder LQ
 // One workgroup (8 waves) per job factors rows j0 .. j0+nbp-1 of X (see LqJob). Thread tid
-// holds columns j0 + tid + LQ_NT k (k < CPT) of the panel rows in registers. For panel
-// row t (diagonal column j = j0 + t), zlarfg on the column vector conj(x[t, j:]) gives
-// beta (real), tau and u (u_j = 1) with x[t, j:] (I - tau u u^H) = beta e_0; the panel rows
-// below take the same reflector, and T gains column t, T[0:t, t] = -tau T[0:t, 0:t] U^H u.
-// Once row t is done its slots right of the diagonal hold u. Sums are fixed-order (wave
-// shuffles, then the wave partials in order), so the factorization is bit-reproducible.
+// holds columns b0 + tid + LQ_NT k (k < CPT) of the panel rows in registers (b0 = the
+// block's first row, so the block's Ubc rows are written from column b0 on). For panel row
+// t (diagonal column j = j0 + t), zlarfg on the column vector conj(x[t, j:]) gives beta
+// (real), tau and u (u_j = 1) with x[t, j:] (I - tau u u^H) = beta e_0; the panel rows below
+// take the same reflector, and T gains column t, T[0:t, t] = -tau T[0:t, 0:t] U^H u. Once row
+// t is done its slots right of the diagonal hold u. Sums are fixed-order (wave shuffles,
+// then the wave partials in order), so the factorization is bit-reproducible.
 template <int V>
 __device__ __forceinline__ void lq_block_sum(double (&v)[V], double (*red)[V], double* fin, int tid) {
     const int lane = tid & 63, wave = tid >> 6;
@@ -1237,7 +1238,7 @@ struct LqShared {
 // Step T of the panel (row T, diagonal column j0 + T); a template so that every register
 // row index is a compile-time constant.
 template <int CPT, int T>
-__device__ __forceinline__ void lq_step(double2 (&x)[LQ_NB][CPT], LqShared& sh, int j0, int m, int nbp, int tid) {
+__device__ __forceinline__ void lq_step(double2 (&x)[LQ_NB][CPT], LqShared& sh, int b0, int j0, int m, int nbp, int tid) {
     if constexpr (T < LQ_NB) {
         if (T < nbp) {
             const double2 one = make_double2(1.0, 0.0);
@@ -1245,7 +1246,7 @@ __device__ __forceinline__ void lq_step(double2 (&x)[LQ_NB][CPT], LqShared& sh, 
             double nrm[1] = {0.0};
 #pragma unroll
             for (int k = 0; k < CPT; ++k) {
-                const int c = j0 + tid + k * LQ_NT;
+                const int c = b0 + tid + k * LQ_NT;
                 if (c > j && c < m) nrm[0] = fma(x[T][k].x, x[T][k].x, fma(x[T][k].y, x[T][k].y, nrm[0]));
                 if (c == j) sh.alpha = x[T][k];
             }
@@ -1262,7 +1263,7 @@ __device__ __forceinline__ void lq_step(double2 (&x)[LQ_NB][CPT], LqShared& sh, 
             }
 #pragma unroll
             for (int k = 0; k < CPT; ++k) {
-                const int c = j0 + tid + k * LQ_NT;
+                const int c = b0 + tid + k * LQ_NT;
                 if (c > j && c < m) x[T][k] = cmul(cconj(x[T][k]), scale);
             }
             // acc[v]: w_v = x[v, j:] u (v > T) and g_v = u_v^H u (v < T).
@@ -1271,7 +1272,7 @@ __device__ __forceinline__ void lq_step(double2 (&x)[LQ_NB][CPT], LqShared& sh, 
             for (int v = 0; v < 2 * LQ_NB; ++v) acc[v] = 0.0;
 #pragma unroll
             for (int k = 0; k < CPT; ++k) {
-                const int c = j0 + tid + k * LQ_NT;
+                const int c = b0 + tid + k * LQ_NT;
                 if (c < j || c >= m) continue;
                 const double2 u = c == j ? one : x[T][k];
 #pragma unroll
@@ -1288,7 +1289,7 @@ __device__ __forceinline__ void lq_step(double2 (&x)[LQ_NB][CPT], LqShared& sh, 
                 const double2 coef = cmul(tau, make_double2(acc[2 * v], acc[2 * v + 1]));
 #pragma unroll
                 for (int k = 0; k < CPT; ++k) {
-                    const int c = j0 + tid + k * LQ_NT;
+                    const int c = b0 + tid + k * LQ_NT;
                     if (c < j || c >= m) continue;
                     const double2 u = c == j ? one : x[T][k];
                     x[v][k] = csub(x[v][k], cmul(coef, cconj(u)));
@@ -1307,14 +1308,14 @@ __device__ __forceinline__ void lq_step(double2 (&x)[LQ_NB][CPT], LqShared& sh, 
                 sh.T[T][T] = tau;
             }
         }
-        lq_step<CPT, T + 1>(x, sh, j0, m, nbp, tid);
+        lq_step<CPT, T + 1>(x, sh, b0, j0, m, nbp, tid);
     }
 }
 
 template <int CPT>
-__global__ __launch_bounds__(LQ_NT) void k_lq_panel(const LqJob* __restrict__ jobs, int j0) {
+__global__ __launch_bounds__(LQ_NT) void k_lq_panel(const LqJob* __restrict__ jobs, int b0, int j0) {
     const LqJob jb = jobs[blockIdx.x];
-    if (j0 >= jb.n || jb.m - j0 > CPT * LQ_NT) return;
+    if (j0 >= jb.n || jb.m - b0 > CPT * LQ_NT) return;
     const int nbp = min(LQ_NB, jb.n - j0);
     const int tid = threadIdx.x;
     __shared__ LqShared sh;
@@ -1323,16 +1324,16 @@ __global__ __launch_bounds__(LQ_NT) void k_lq_panel(const LqJob* __restrict__ jo
     for (int r = 0; r < LQ_NB; ++r)
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
-            const int c = j0 + tid + k * LQ_NT;
+            const int c = b0 + tid + k * LQ_NT;
             x[r][k] = (r < nbp && c < jb.m) ? jb.X[(long)(j0 + r) * jb.ldx + c] : cz();
         }
     if (tid < LQ_NB * LQ_NB) sh.T[tid / LQ_NB][tid % LQ_NB] = cz();
-    lq_step<CPT, 0>(x, sh, j0, jb.m, nbp, tid);
+    lq_step<CPT, 0>(x, sh, b0, j0, jb.m, nbp, tid);
     __syncthreads();
     const double2 one = make_double2(1.0, 0.0);
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
-        const int c = j0 + tid + k * LQ_NT;
+        const int c = b0 + tid + k * LQ_NT;
         if (c >= jb.m) continue;
         double2 u[LQ_NB];
 #pragma unroll
@@ -1347,16 +1348,47 @@ __global__ __launch_bounds__(LQ_NT) void k_lq_panel(const LqJob* __restrict__ jo
 #pragma unroll
         for (int tp = 0; tp < LQ_NB; ++tp) {
             if (tp >= nbp) continue;
-            double2 a = cz(), b = cz();
+            double2 a = cz();
 #pragma unroll
-            for (int t = 0; t < LQ_NB; ++t) {
+            for (int t = 0; t < LQ_NB; ++t)
                 if (t <= tp) cmac(a, u[t], sh.T[t][tp]);
-                if (t >= tp && t < nbp) cmac(b, u[t], cconj(sh.T[tp][t]));
-            }
             jb.UTc[(long)(j0 + tp) * jb.m + c] = cconj(a);
-            jb.UThc[(long)(j0 + tp) * jb.m + c] = cconj(b);
         }
     }
+    // The panel's T is the diagonal block of the block's T.
+    if (tid < LQ_NB * LQ_NB) {
+        const int i = tid / LQ_NB, k = tid % LQ_NB;
+        if (i < nbp && k < nbp) jb.T64[(j0 - b0 + i) * LQ_SB + (j0 - b0 + k)] = sh.T[i][k];
+    }
+}
+
+// Completes the block's T from its diagonal panel blocks and Gu = U^H U:
+// T[0:jb, j] = -tau_j T[0:jb, 0:j] Gu[0:j, j] for the columns j of the later panels
+// (jb = the first row of j's panel), column by column; the strictly lower part is zero.
+__global__ __launch_bounds__(256) void k_lq_tblock(const LqJob* __restrict__ jobs, int b0) {
+    const LqJob jb = jobs[blockIdx.x];
+    const int nbs = min(LQ_SB, jb.n - b0);
+    if (nbs <= 0) return;
+    __shared__ double2 T[LQ_SB][LQ_SB];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < LQ_SB * LQ_SB; e += 256) {
+        const int i = e / LQ_SB, k = e % LQ_SB;
+        const bool diag_block = i / LQ_NB == k / LQ_NB && k >= i;
+        T[i][k] = (i < nbs && k < nbs && diag_block) ? jb.T64[e] : cz();
+    }
+    __syncthreads();
+    for (int j = LQ_NB; j < nbs; ++j) {
+        const int j0b = (j / LQ_NB) * LQ_NB;
+        double2 acc = cz();
+        if (tid < j0b) {
+            for (int l = tid; l < j; ++l) cmac(acc, T[tid][l], jb.Gu[l * LQ_SB + j]);
+        }
+        const double2 tau = T[j][j];
+        __syncthreads();
+        if (tid < j0b) T[tid][j] = cmul(make_double2(-tau.x, -tau.y), acc);
+        __syncthreads();
+    }
+    for (int e = tid; e < LQ_SB * LQ_SB; e += 256) jb.T64[e] = T[e / LQ_SB][e % LQ_SB];
 }
 
 // R = L^H (n x n, row stride n) from the lower triangle of the factored X.
@@ -1800,11 +1832,15 @@ void launch_householder(const HouseJob* jobs, int njobs, hipStream_t s) {
     hipLaunchKernelGGL(k_householder, dim3(njobs), dim3(256), 0, s, jobs);
 }
 
-void launch_lq_panel(const LqJob* jobs, int njobs, int j0, int max_cols, hipStream_t s) {
+void launch_lq_panel(const LqJob* jobs, int njobs, int b0, int j0, int max_cols, hipStream_t s) {
     if (max_cols <= LQ_NT)
-        hipLaunchKernelGGL(k_lq_panel<1>, dim3(njobs), dim3(LQ_NT), 0, s, jobs, j0);
+        hipLaunchKernelGGL(k_lq_panel<1>, dim3(njobs), dim3(LQ_NT), 0, s, jobs, b0, j0);
     else
-        hipLaunchKernelGGL(k_lq_panel<2>, dim3(njobs), dim3(LQ_NT), 0, s, jobs, j0);
+        hipLaunchKernelGGL(k_lq_panel<2>, dim3(njobs), dim3(LQ_NT), 0, s, jobs, b0, j0);
+}
+
+void launch_lq_tblock(const LqJob* jobs, int njobs, int b0, hipStream_t s) {
+    hipLaunchKernelGGL(k_lq_tblock, dim3(njobs), dim3(256), 0, s, jobs, b0);
 }
 
 void launch_lq_extract(const LqJob* jobs, int njobs, int max_n, hipStream_t s) {
