@@ -300,6 +300,20 @@ __device__ __forceinline__ Rot64 rot_params64(double a, double d, double2 c, dou
     return r;
 }
 
+// Rotation k of a step is computed by lanes k, k + 16, ... of every wave (as the rotation
+// of their column blocks); the row-block rotation of a lane is fetched from lane bk of its
+// wave instead of being derived a second time. m10 = (-m01.x, m01.y) by construction.
+__device__ __forceinline__ Rot64 rot_from_lane(const Rot64& own, int src) {
+    Rot64 r;
+    r.act = __shfl(own.act, src, 64);
+    r.cs = __shfl(own.cs, src, 64);
+    r.tg = __shfl(own.tg, src, 64);
+    r.m01.x = __shfl(own.m01.x, src, 64);
+    r.m01.y = __shfl(own.m01.y, src, 64);
+    r.m10 = make_double2(-r.m01.x, r.m01.y);
+    return r;
+}
+
 __device__ __forceinline__ d4v mfma_f64(double a, double b, d4v c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
@@ -492,8 +506,8 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacobiJob* __rest
             int p1, q1, p2, q2;
             circle_pair(step, bk, p1, q1);
             circle_pair(step, bk2, p2, q2);
-            const Rot64 r1 = rot_params64(Gc[p1][p1].x, Gc[q1][q1].x, Gc[p1][q1], tol, nu2);
             const Rot64 r2 = rot_params64(Gc[p2][p2].x, Gc[q2][q2].x, Gc[p2][q2], tol, nu2);
+            const Rot64 r1 = rot_from_lane(r2, bk & 15);
             double2 h00 = Gc[p1][p2], h01 = Gc[p1][q2], h10 = Gc[q1][p2], h11 = Gc[q1][q2];
             double2 t00 = Tc[p1][p2], t01 = Tc[p1][q2], t10 = Tc[q1][p2], t11 = Tc[q1][q2];
             if (r1.act) {
@@ -854,9 +868,9 @@ __global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __re
             circle_pair(step, bk, p1, q1);
             circle_pair(step, bk2, p2, q2);
             // Parameters in FP64 from the FP32 Gram entries (no underflow of |G_pq|^2).
-            const float2 c1 = Gc[p1][q1], c2 = Gc[p2][q2];
-            const Rot64 r1 = rot_params64(Gc[p1][p1].x, Gc[q1][q1].x, make_double2(c1.x, c1.y), tol, low);
+            const float2 c2 = Gc[p2][q2];
             const Rot64 r2 = rot_params64(Gc[p2][p2].x, Gc[q2][q2].x, make_double2(c2.x, c2.y), tol, low);
+            const Rot64 r1 = rot_from_lane(r2, bk & 15);
             float2 h00 = Gc[p1][p2], h01 = Gc[p1][q2], h10 = Gc[q1][p2], h11 = Gc[q1][q2];
             float2 t00 = Tc[p1][p2], t01 = Tc[p1][q2], t10 = Tc[q1][p2], t11 = Tc[q1][q2];
             if (r1.act) {
