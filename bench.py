@@ -7,8 +7,8 @@ one pass of the whole circuit from |0...0> through the C ABI (wqc_tn_apply_circu
 the final <0...0|psi>; inputs (the gate list) are on the host, every site tensor lives in
 HBM for the whole step.
 
-Printed JSON line: value = merges/s of the mixed form (FP32 early Jacobi sweeps, FP64
-finish); the plain FP64 form's rate, their ratio, the relative amplitude error between the
+Printed JSON line: value = merges/s of the mixed form (Householder-LQ-preconditioned Jacobi:
+up to 6 FP32 sweeps, FP64 finish); the plain FP64 form's rate, their ratio, the relative amplitude error between the
 forms (after the range-finder-free first 24 layers and at the end), a roofline record for
 the dominant kernel and a CPU baseline ride along. `--gpus N` with N > 1 exits: this build runs on one GPU.
 """
@@ -248,7 +248,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"rotation+CZ brickwork {n}q depth {args.depth} chi={chi}",
                    "qubits": n, "depth": args.depth, "chi": chi, "units": units,
-                   "form": "mixed (FP32 early Jacobi sweeps, FP64 finish)" if head == "mixed" else "plain FP64"},
+                   "form": "mixed (LQ-preconditioned Jacobi, FP32 early sweeps, FP64 finish)" if head == "mixed" else "plain FP64"},
         "forms": form_report,
         "roofline": roofline,
     }
