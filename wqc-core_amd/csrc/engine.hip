@@ -450,7 +450,7 @@ struct wqc_tn_state {
 
     // ------------------------------------------------------------ helpers
     void note_peak() {
-        uint64_t now = site_bytes + ws.bytes + ws32.bytes + ews.bytes + tab.bytes;
+        uint64_t now = site_bytes + ws.bytes + ws32.bytes + ews.bytes + tab.bytes + splitk_buf.bytes;
         if (now > stats.peak_vram_bytes) stats.peak_vram_bytes = now;
     }
 
@@ -508,6 +508,7 @@ struct wqc_tn_state {
     bool trace_jacobi = false;
     double stall_off = STALL_OFF;
     bool use_lq = true;
+    bool use_splitk = true;  // WQC_TN_SPLITK=0 turns split-K GEMMs off
     DevBuf trace_buf;
 
     void init(wqc_tn_device* d, size_t qn, size_t mb, int prec) {
@@ -524,6 +525,7 @@ struct wqc_tn_state {
         // decisions); WQC_TN_LQ=0/1 overrides either.
         use_lq = prec == WQC_TN_PRECISION_MIXED_FP32;
         if (const char* e = std::getenv("WQC_TN_LQ")) use_lq = std::atoi(e) != 0;
+        if (const char* e = std::getenv("WQC_TN_SPLITK")) use_splitk = std::atoi(e) != 0;
         if (const char* e = std::getenv("WQC_TN_JACOBI_INNER_LO")) {
             const int v = std::atoi(e);
             if (v >= 1 && v <= 32) inner_cap_lo = v;
@@ -1126,14 +1128,33 @@ struct wqc_tn_state {
             }
     }
 
+    // Batched GEMM. Products with few output tiles and a long K (the LQ's panel and block
+    // products) are split over K into up to 8 slices, summed in slice order by a second kernel.
+    DevBuf splitk_buf;
     void gemm(std::vector<GemmJob>& g, int family = 2) {
         if (g.empty()) return;
-        int mm = 1, mn = 1;
+        int mm = 1, mn = 1, mk = 1;
         for (auto& x : g) {
             mm = std::max(mm, x.M);
             mn = std::max(mn, x.N);
+            mk = std::max(mk, x.K);
         }
-        wqc::launch_gemm(upload(g), (int)g.size(), mm, mn, stream);
+        const long tiles = (long)((mm + 63) / 64) * ((mn + 63) / 64);
+        const int splits = (use_splitk && tiles <= 16 && mk >= 256) ? std::min(8, mk / 128) : 1;
+        if (splits > 1) {
+            size_t need = 0;
+            for (auto& x : g) need += (size_t)splits * x.M * x.N;
+            splitk_buf.ensure(need * sizeof(double2));
+            note_peak();
+            double2* p = reinterpret_cast<double2*>(splitk_buf.p);
+            for (auto& x : g) {
+                x.part = p;
+                p += (size_t)splits * x.M * x.N;
+            }
+            wqc::launch_gemm_splitk(upload(g), (int)g.size(), mm, mn, splits, stream);
+        } else {
+            wqc::launch_gemm(upload(g), (int)g.size(), mm, mn, stream);
+        }
         check_launch();
         klaunch[family]++;
     }
@@ -1154,6 +1175,7 @@ struct wqc_tn_state {
         g.opB = opB;
         g.alpha = make_double2(alpha, 0);
         g.beta = make_double2(beta, 0);
+        g.part = nullptr;
         return g;
     }
 

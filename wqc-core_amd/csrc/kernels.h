@@ -49,6 +49,10 @@ struct JacobiLoJob {
 // Batched complex GEMM: C = alpha * op(A) * op(B) + beta * C, op in {N, C} (C = conjugate
 // transpose). M x N result, inner dimension K. Leading dimensions are row strides of the
 // stored matrices.
+// With split-K (launch_gemm_splitk) every job's K range is cut into `splits` slices, slice s
+// writes its raw product into part + s * M * N (row-major M x N), and a second kernel sums
+// the slices in order and applies alpha and beta: more workgroups for the skinny, long-K
+// products of the LQ, with a fixed summation order.
 struct GemmJob {
     const double2* A;
     const double2* B;
@@ -57,6 +61,7 @@ struct GemmJob {
     int lda, ldb, ldc;
     int opA, opB;  // 0 = N, 1 = C
     double2 alpha, beta;
+    double2* part;  // split-K partials (splits * M * N), or null
 };
 
 // Factor writes of the split.
@@ -159,6 +164,7 @@ void launch_to_f64(const float2* const* src, double2* const* dst, const long* co
 void launch_row_norms(const JacobiJob* jobs, int njobs, int max_n, double* sigma,
                       const long* sigma_off, hipStream_t s);
 void launch_gemm(const GemmJob* jobs, int njobs, int max_m, int max_n, hipStream_t s);
+void launch_gemm_splitk(const GemmJob* jobs, int njobs, int max_m, int max_n, int splits, hipStream_t s);
 void launch_factor(const FactorJob* jobs, int njobs, long max_left, long max_right,
                    hipStream_t s);
 void launch_householder(const HouseJob* jobs, int njobs, hipStream_t s);

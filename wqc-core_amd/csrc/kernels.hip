@@ -1073,10 +1073,18 @@ __global__ __launch_bounds__(256) void k_row_norms(const JacobiJob* __restrict__
 // while staging). Edges are zero-padded.
 constexpr int MG = 64, MGK = 16;
 
-__global__ __launch_bounds__(256, 2) void k_gemm_mfma(const GemmJob* __restrict__ jobs) {
-    const GemmJob jb = jobs[blockIdx.z];
+template <bool PARTIAL>
+__global__ __launch_bounds__(256, 2) void k_gemm_mfma(const GemmJob* __restrict__ jobs, int splits) {
+    const GemmJob jb = jobs[PARTIAL ? blockIdx.z / splits : blockIdx.z];
     const int m0 = blockIdx.y * MG, n0 = blockIdx.x * MG;
     if (m0 >= jb.M || n0 >= jb.N) return;
+    int kbeg = 0, kend = jb.K;
+    if (PARTIAL) {
+        const int slice = blockIdx.z % splits;
+        const int ks = (((jb.K + splits - 1) / splits) + MGK - 1) / MGK * MGK;
+        kbeg = slice * ks;
+        kend = min(jb.K, kbeg + ks);
+    }
     __shared__ double Ar[MG][MGK + 1], Ai[MG][MGK + 1], Br[MGK][MG + 1], Bi[MGK][MG + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
@@ -1090,7 +1098,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(const GemmJob* __restrict_
             e2[a][b] = e1[a][b];
             e3[a][b] = e1[a][b];
         }
-    for (int k0 = 0; k0 < jb.K; k0 += MGK) {
+    for (int k0 = kbeg; k0 < kend; k0 += MGK) {
 #pragma unroll
         for (int u = 0; u < (MG * MGK) / 256; ++u) {
             const int e = tid + 256 * u;
@@ -1100,12 +1108,12 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(const GemmJob* __restrict_
                 mm = e / MGK;
                 kk = e % MGK;
                 const int m = m0 + mm, k = k0 + kk;
-                v = (m < jb.M && k < jb.K) ? jb.A[(long)m * jb.lda + k] : cz();
+                v = (m < jb.M && k < kend) ? jb.A[(long)m * jb.lda + k] : cz();
             } else {
                 kk = e / MG;
                 mm = e % MG;
                 const int m = m0 + mm, k = k0 + kk;
-                v = (m < jb.M && k < jb.K) ? cconj(jb.A[(long)k * jb.lda + m]) : cz();
+                v = (m < jb.M && k < kend) ? cconj(jb.A[(long)k * jb.lda + m]) : cz();
             }
             Ar[mm][kk] = v.x;
             Ai[mm][kk] = v.y;
@@ -1119,12 +1127,12 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(const GemmJob* __restrict_
                 kk = e / MG;
                 nn = e % MG;
                 const int n = n0 + nn, k = k0 + kk;
-                v = (n < jb.N && k < jb.K) ? jb.B[(long)k * jb.ldb + n] : cz();
+                v = (n < jb.N && k < kend) ? jb.B[(long)k * jb.ldb + n] : cz();
             } else {
                 nn = e / MGK;
                 kk = e % MGK;
                 const int n = n0 + nn, k = k0 + kk;
-                v = (n < jb.N && k < jb.K) ? cconj(jb.B[(long)n * jb.ldb + k]) : cz();
+                v = (n < jb.N && k < kend) ? cconj(jb.B[(long)n * jb.ldb + k]) : cz();
             }
             Br[kk][nn] = v.x;
             Bi[kk][nn] = v.y;
@@ -1164,11 +1172,30 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(const GemmJob* __restrict_
                 const int m = m0 + wm + 16 * a + 4 * j + lk, n = n0 + wn + 16 * b + li;
                 if (m >= jb.M || n >= jb.N) continue;
                 const double re = e1[a][b][j] - e2[a][b][j], im = e3[a][b][j] - e1[a][b][j] - e2[a][b][j];
+                if (PARTIAL) {
+                    jb.part[((long)(blockIdx.z % splits) * jb.M + m) * jb.N + n] = make_double2(re, im);
+                    continue;
+                }
                 double2* c = jb.C + (long)m * jb.ldc + n;
                 double2 v = cmul(jb.alpha, make_double2(re, im));
                 if (jb.beta.x != 0.0 || jb.beta.y != 0.0) cmac(v, jb.beta, *c);
                 *c = v;
             }
+}
+
+// Sums the split-K partials in slice order, then C = alpha * sum + beta * C.
+__global__ __launch_bounds__(256) void k_gemm_splitk_reduce(const GemmJob* __restrict__ jobs, int splits) {
+    const GemmJob jb = jobs[blockIdx.y];
+    const long total = (long)jb.M * jb.N;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+        double2 acc = cz();
+        for (int sl = 0; sl < splits; ++sl) acc = cadd(acc, jb.part[(long)sl * total + e]);
+        const int m = (int)(e / jb.N), n = (int)(e - (long)m * jb.N);
+        double2* c = jb.C + (long)m * jb.ldc + n;
+        double2 v = cmul(jb.alpha, acc);
+        if (jb.beta.x != 0.0 || jb.beta.y != 0.0) cmac(v, jb.beta, *c);
+        *c = v;
+    }
 }
 
 // ------------------------------------------------------------------ factors
@@ -1376,14 +1403,16 @@ __global__ __launch_bounds__(LQ_NT) void k_lq_panel(const LqJob* __restrict__ jo
     }
 }
 
-// Completes the block's T from its diagonal panel blocks and Gu = U^H U:
-// T[0:jb, j] = -tau_j T[0:jb, 0:j] Gu[0:j, j] for the columns j of the later panels
-// (jb = the first row of j's panel), column by column; the strictly lower part is zero.
+// Completes the block's T from its diagonal panel blocks and Gu = U^H U by merging compact-WY
+// factors pairwise: for adjacent groups A, B of panels, T_AB = -T_A (U_A^H U_B) T_B (the
+// upper-right block of T for Q_A Q_B); panels (0,1) and (2,3) first, then (01, 23). The
+// strictly lower part is zero.
 __global__ __launch_bounds__(256) void k_lq_tblock(const LqJob* __restrict__ jobs, int b0) {
     const LqJob jb = jobs[blockIdx.x];
     const int nbs = min(LQ_SB, jb.n - b0);
     if (nbs <= 0) return;
-    __shared__ double2 T[LQ_SB][LQ_SB];
+    __shared__ double2 T[LQ_SB][LQ_SB + 1];
+    __shared__ double2 W[LQ_SB / 2][LQ_SB / 2 + 1];
     const int tid = threadIdx.x;
     for (int e = tid; e < LQ_SB * LQ_SB; e += 256) {
         const int i = e / LQ_SB, k = e % LQ_SB;
@@ -1391,16 +1420,26 @@ __global__ __launch_bounds__(256) void k_lq_tblock(const LqJob* __restrict__ job
         T[i][k] = (i < nbs && k < nbs && diag_block) ? jb.T64[e] : cz();
     }
     __syncthreads();
-    for (int j = LQ_NB; j < nbs; ++j) {
-        const int j0b = (j / LQ_NB) * LQ_NB;
-        double2 acc = cz();
-        if (tid < j0b) {
-            for (int l = tid; l < j; ++l) cmac(acc, T[tid][l], jb.Gu[l * LQ_SB + j]);
+    for (int w = LQ_NB; w < LQ_SB; w *= 2) {
+        // Groups [a, a + w) and [a + w, a + 2w) for a = 0, 2w, ...
+        // W = Gu[A, B] T_B (w x w per pair), then T[A, B] = -T_A W.
+        for (int a = 0; a + w < nbs; a += 2 * w) {
+            const int bw = min(w, nbs - a - w);
+            for (int e = tid; e < w * bw; e += 256) {
+                const int i = e / bw, k = e % bw;
+                double2 acc = cz();
+                for (int l = 0; l <= k; ++l) cmac(acc, jb.Gu[(a + i) * LQ_SB + a + w + l], T[a + w + l][a + w + k]);
+                W[i][k] = acc;
+            }
+            __syncthreads();
+            for (int e = tid; e < w * bw; e += 256) {
+                const int i = e / bw, k = e % bw;
+                double2 acc = cz();
+                for (int l = i; l < w; ++l) cmac(acc, T[a + i][a + l], W[l][k]);
+                T[a + i][a + w + k] = make_double2(-acc.x, -acc.y);
+            }
+            __syncthreads();
         }
-        const double2 tau = T[j][j];
-        __syncthreads();
-        if (tid < j0b) T[tid][j] = cmul(make_double2(-tau.x, -tau.y), acc);
-        __syncthreads();
     }
     for (int e = tid; e < LQ_SB * LQ_SB; e += 256) jb.T64[e] = T[e / LQ_SB][e % LQ_SB];
 }
@@ -1833,7 +1872,14 @@ void launch_env_close(const CloseJob* jobs, int njobs, hipStream_t s) {
 
 void launch_gemm(const GemmJob* jobs, int njobs, int max_m, int max_n, hipStream_t s) {
     dim3 grid((max_n + MG - 1) / MG, (max_m + MG - 1) / MG, njobs);
-    hipLaunchKernelGGL(k_gemm_mfma, grid, dim3(256), 0, s, jobs);
+    hipLaunchKernelGGL(k_gemm_mfma<false>, grid, dim3(256), 0, s, jobs, 1);
+}
+
+void launch_gemm_splitk(const GemmJob* jobs, int njobs, int max_m, int max_n, int splits, hipStream_t s) {
+    dim3 grid((max_n + MG - 1) / MG, (max_m + MG - 1) / MG, njobs * splits);
+    hipLaunchKernelGGL(k_gemm_mfma<true>, grid, dim3(256), 0, s, jobs, splits);
+    hipLaunchKernelGGL(k_gemm_splitk_reduce, dim3(blocks_for((long)max_m * max_n, 256, 256), njobs), dim3(256), 0,
+                       s, jobs, splits);
 }
 
 void launch_factor(const FactorJob* jobs, int njobs, long max_left, long max_right,
