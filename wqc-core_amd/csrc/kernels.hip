@@ -267,6 +267,13 @@ __device__ __forceinline__ void circle_pair(int step, int k, int& p, int& q) {
     }
 }
 
+// Rotation pair k of step `step` (0 .. JB-1) of a cross-only sweep: row k of the first
+// block with row JB + (k + step) mod JB of the second, every cross pair once per JB steps.
+__device__ __forceinline__ void cross_pair(int step, int k, int& p, int& q) {
+    p = k;
+    q = JB + ((k + step) & (JB - 1));
+}
+
 // Complex 2x2 Jacobi rotation M = [[cs, m01], [m10, cs]] that zeroes G_pq of the Hermitian
 // block [[a, c], [conj(c), d]]: M G M^H = diag(a - tg, d + tg). act = 0 when the pair is
 // orthogonal to tol (relative) or one of its rows is below nu2.
@@ -350,6 +357,7 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacobiJob* __rest
     __shared__ __align__(16) double s_aux[4 * JP * (JP + 1)];
     __shared__ int s_row[JP];
     __shared__ int s_any;
+    __shared__ int s_intra;
     __shared__ int s_dst[JP];
     __shared__ unsigned long long s_off;
     double (*Gp)[6][4][64] = reinterpret_cast<double (*)[6][4][64]>(s_aux);
@@ -364,6 +372,7 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacobiJob* __rest
     }
     if (tid == 0) {
         s_any = 0;
+        s_intra = 0;
         s_off = 0;
     }
     __syncthreads();
@@ -456,7 +465,7 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacobiJob* __rest
     }
     __syncthreads();
     {
-        bool off = false;
+        bool off = false, intra = false;
         double worst = 0.0;
         for (int e = tid; e < JP * JP; e += 256) {
             const int i = e / JP, j = e % JP;
@@ -465,12 +474,16 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacobiJob* __rest
                 const double g2 = fma(c.x, c.x, c.y * c.y);
                 const double di = G[i][i].x, dj = G[j][j].x;
                 if (g2 > 0.0 && fmin(di, dj) > nu2) {
-                    if (g2 > tol * tol * (di * dj)) off = true;
+                    if (g2 > tol * tol * (di * dj)) {
+                        off = true;
+                        if ((i < JB) == (j < JB)) intra = true;
+                    }
                     if (offmax) worst = fmax(worst, sqrt(g2 / (di * dj)));
                 }
             }
         }
         if (off) s_any = 1;
+        if (intra) s_intra = 1;
         // Non-negative doubles order as their bit patterns.
         if (offmax && worst > 0.0) atomicMax(&s_off, (unsigned long long)__double_as_longlong(worst));
     }
@@ -478,6 +491,7 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacobiJob* __rest
     if (offmax && tid == 0 && s_off) atomicMax(&offmax[job], s_off);
     // Every wave reads the flag before thread 0 clears it.
     const int need = s_any;
+    const int intra0 = s_intra;
     __syncthreads();
     if (!need) return;
     if (tid == 0) {
@@ -502,10 +516,20 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacobiJob* __rest
     const int bk = tid >> 4, bk2 = tid & 15;
     for (int sweep = 0; sweep < inner_cap; ++sweep) {
         ++sweeps_done;
-        for (int step = 0; step < JP - 1; ++step) {
+        // The pair's two blocks come out of their previous rounds mutually orthogonal inside;
+        // when the check found no block-internal pair above tolerance, the first sweep
+        // rotates only the JB x JB cross pairs (JB steps instead of 2 JB - 1).
+        const bool cross = sweep == 0 && !intra0;
+        const int nsteps = cross ? JB : JP - 1;
+        for (int step = 0; step < nsteps; ++step) {
             int p1, q1, p2, q2;
-            circle_pair(step, bk, p1, q1);
-            circle_pair(step, bk2, p2, q2);
+            if (cross) {
+                cross_pair(step, bk, p1, q1);
+                cross_pair(step, bk2, p2, q2);
+            } else {
+                circle_pair(step, bk, p1, q1);
+                circle_pair(step, bk2, p2, q2);
+            }
             const Rot64 r2 = rot_params64(Gc[p2][p2].x, Gc[q2][q2].x, Gc[p2][q2], tol, nu2);
             const Rot64 r1 = rot_from_lane(r2, bk & 15);
             double2 h00 = Gc[p1][p2], h01 = Gc[p1][q2], h10 = Gc[q1][p2], h11 = Gc[q1][q2];
@@ -708,6 +732,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __re
     __shared__ __align__(16) float s_aux[4 * JP * (JP + 1)];
     __shared__ int s_row[JP];
     __shared__ int s_any;
+    __shared__ int s_intra;
     __shared__ float s_low;
     __shared__ int s_dst[JP];
     float (*Gp)[6][4][64] = reinterpret_cast<float (*)[6][4][64]>(s_aux);
@@ -720,7 +745,10 @@ __global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __re
         const int r = b * JB + (tid & (JB - 1));
         s_row[tid] = (b < nb && r < jb.n) ? r : -1;
     }
-    if (tid == 0) s_any = 0;
+    if (tid == 0) {
+        s_any = 0;
+        s_intra = 0;
+    }
     __syncthreads();
 
     {
@@ -818,7 +846,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __re
     __syncthreads();
     const float low = s_low;
     {
-        bool off = false;
+        bool off = false, intra = false;
         double worst = 0.0;
         for (int e = tid; e < JP * JP; e += 256) {
             const int i = e / JP, j = e % JP;
@@ -828,16 +856,21 @@ __global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __re
                 const float di = G[i][i].x, dj = G[j][j].x;
                 if (gg > 0.f && fminf(di, dj) > low) {
                     const float den = sqrtf(fmaxf(di, 0.f) * fmaxf(dj, 0.f));
-                    if (gg > tol * den) off = true;
+                    if (gg > tol * den) {
+                        off = true;
+                        if ((i < JB) == (j < JB)) intra = true;
+                    }
                     if (offmax) worst = fmax(worst, (double)(gg / den));
                 }
             }
         }
         if (off) s_any = 1;
+        if (intra) s_intra = 1;
         if (offmax && worst > 0.0) atomicMax(&offmax[job], (unsigned long long)__double_as_longlong(worst));
     }
     __syncthreads();
     const int need = s_any;
+    const int intra0 = s_intra;
     __syncthreads();
     if (!need) return;
     if (tid == 0) {
@@ -863,10 +896,20 @@ __global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __re
     auto f2 = [](double2 a) { return make_float2((float)a.x, (float)a.y); };
     for (int sweep = 0; sweep < inner_cap; ++sweep) {
         ++sweeps_done;
-        for (int step = 0; step < JP - 1; ++step) {
+        // The pair's two blocks come out of their previous rounds mutually orthogonal inside;
+        // when the check found no block-internal pair above tolerance, the first sweep
+        // rotates only the JB x JB cross pairs (JB steps instead of 2 JB - 1).
+        const bool cross = sweep == 0 && !intra0;
+        const int nsteps = cross ? JB : JP - 1;
+        for (int step = 0; step < nsteps; ++step) {
             int p1, q1, p2, q2;
-            circle_pair(step, bk, p1, q1);
-            circle_pair(step, bk2, p2, q2);
+            if (cross) {
+                cross_pair(step, bk, p1, q1);
+                cross_pair(step, bk2, p2, q2);
+            } else {
+                circle_pair(step, bk, p1, q1);
+                circle_pair(step, bk2, p2, q2);
+            }
             // Parameters in FP64 from the FP32 Gram entries (no underflow of |G_pq|^2).
             const float2 c2 = Gc[p2][q2];
             const Rot64 r2 = rot_params64(Gc[p2][p2].x, Gc[q2][q2].x, make_double2(c2.x, c2.y), tol, low);
