@@ -1307,6 +1307,61 @@ __device__ __forceinline__ void lq_block_sum(double (&v)[V], double (*red)[V], d
     for (int i = 0; i < V; ++i) v[i] = fin[i];
 }
 
+// Block sum of 2 * LQ_NB = 32 values per thread by a reduce-scatter inside each wave (stage k
+// exchanges half of the remaining values with lane ^ (32 >> k): 31 value shuffles instead of
+// 32 full butterflies), then the wave partials summed in wave order. Every thread returns
+// all 32 sums. Fixed order, so bit-reproducible.
+__device__ __forceinline__ void lq_block_sum32(double (&v)[2 * LQ_NB], double (*red)[2 * LQ_NB], double* fin,
+                                               int tid) {
+    static_assert(2 * LQ_NB == 32, "reduce-scatter is written for 32 values");
+    const int lane = tid & 63, wave = tid >> 6;
+    double a[16], b[8], c[4], d[2];
+    const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8, h2 = lane & 4, h1 = lane & 2;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const double send = h5 ? v[i] : v[16 + i];
+        const double keep = h5 ? v[16 + i] : v[i];
+        a[i] = keep + __shfl_xor(send, 32, 64);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const double send = h4 ? a[i] : a[8 + i];
+        const double keep = h4 ? a[8 + i] : a[i];
+        b[i] = keep + __shfl_xor(send, 16, 64);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const double send = h3 ? b[i] : b[4 + i];
+        const double keep = h3 ? b[4 + i] : b[i];
+        c[i] = keep + __shfl_xor(send, 8, 64);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const double send = h2 ? c[i] : c[2 + i];
+        const double keep = h2 ? c[2 + i] : c[i];
+        d[i] = keep + __shfl_xor(send, 4, 64);
+    }
+    double e;
+    {
+        const double send = h1 ? d[0] : d[1];
+        const double keep = h1 ? d[1] : d[0];
+        e = keep + __shfl_xor(send, 2, 64);
+    }
+    e += __shfl_xor(e, 1, 64);
+    // Lane bits 5..1 select the value: 16 h5 + 8 h4 + 4 h3 + 2 h2 + h1.
+    const int idx = (h5 ? 16 : 0) + (h4 ? 8 : 0) + (h3 ? 4 : 0) + (h2 ? 2 : 0) + (h1 ? 1 : 0);
+    if ((lane & 1) == 0) red[wave][idx] = e;
+    __syncthreads();
+    if (tid < 2 * LQ_NB) {
+        double acc = 0.0;
+        for (int w = 0; w < LQ_NT / 64; ++w) acc += red[w][tid];
+        fin[tid] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 2 * LQ_NB; ++i) v[i] = fin[i];
+}
+
 // Shared state of one panel factorization.
 struct LqShared {
     double red1[LQ_NT / 64][1];
@@ -1367,7 +1422,7 @@ __device__ __forceinline__ void lq_step(double2 (&x)[LQ_NB][CPT], LqShared& sh, 
                     acc[2 * v + 1] += p.y;
                 }
             }
-            lq_block_sum<2 * LQ_NB>(acc, sh.redv, sh.finv, tid);
+            lq_block_sum32(acc, sh.redv, sh.finv, tid);
 #pragma unroll
             for (int v = T + 1; v < LQ_NB; ++v) {
                 const double2 coef = cmul(tau, make_double2(acc[2 * v], acc[2 * v + 1]));
