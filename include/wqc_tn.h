@@ -182,6 +182,20 @@ void wqc_tn_clear_cancel(wqc_tn_state *st);
 
 /* ---- readouts ---------------------------------------------------------------------- */
 
+/* <bra|ket> of two states of the same register (overlap, src/tn/mps.rs:1421-1474): the
+ * left-to-right environment (ket bond x bra bond) may hold at most 256 entries, else
+ * EXEC_FAILED "mps overlap environment exceeds 256"; INVALID_ARG for different registers. */
+int wqc_tn_overlap(wqc_tn_state *ket, wqc_tn_state *bra, double out[2]);
+
+/* contract_time_split (src/tn/mps.rs:1241-1249 with time_split_boundaries :1358-1392): the ket
+ * is |0...0> through gates[0, split), the bra |0...0> through the adjoints of gates[split, count)
+ * in reverse order (adjoint_gate_sequence :1398-1419), both with reject_truncation at
+ * max_bond_dim; out = <bra|ket> = <0...0| U |0...0>. EXEC_FAILED when split > count, when the
+ * second half holds MEASURE/RESET/IF, or when the overlap environment exceeds 256 entries;
+ * BOND_EXCEEDED when either half needs truncation. */
+int wqc_tn_contract_time_split(wqc_tn_device *dev, size_t qubit_count, const wqc_tn_gate *gates, size_t count,
+                               size_t split, size_t max_bond_dim, int precision, double out[2]);
+
 /* MpsState::amplitude_at_compact_zero (src/tn/mps.rs:262-276): <0...0|psi>, FP64. */
 int wqc_tn_amplitude_at_compact_zero(wqc_tn_state *st, double out[2]);
 

@@ -1830,6 +1830,39 @@ struct wqc_tn_state {
         }
     }
 
+    // ------------------------------------------------------------ overlap
+    // <bra|ket> with this state as the ket (overlap, src/tn/mps.rs:1421-1474): the environment
+    // (ket bond x bra bond) is carried left to right and may hold at most 256 entries.
+    DevBuf ov_buf;
+    void overlap_with(wqc_tn_state& bra, double out[2]) {
+        if (bra.n != n) fail(WQC_TN_INVALID_ARG, "mismatched register");
+        flush();
+        bra.flush();
+        bra.sync();
+        ov_buf.ensure(2 * wqc::OVERLAP_ENV_MAX * sizeof(double2));
+        double2* cur = reinterpret_cast<double2*>(ov_buf.p);
+        double2* nxt = cur + wqc::OVERLAP_ENV_MAX;
+        HIPC(hipMemcpyAsync(cur, &kOne, sizeof(double2), hipMemcpyHostToDevice, stream));
+        int kl = 1, bl = 1;
+        for (int i = 0; i < n; ++i) {
+            const Site& K = sites[i];
+            const Site& B = bra.sites[i];
+            if (K.l != kl || B.l != bl) fail(WQC_TN_EXEC_FAILED, "mps site shape does not match the overlap");
+            if ((long)K.r * B.r > wqc::OVERLAP_ENV_MAX) fail(WQC_TN_EXEC_FAILED, "mps overlap environment exceeds 256");
+            wqc::launch_overlap_site(cur, K.ptr, B.ptr, nxt, kl, K.r, bl, B.r, stream);
+            check_launch();
+            std::swap(cur, nxt);
+            kl = K.r;
+            bl = B.r;
+        }
+        if (kl * bl != 1) fail(WQC_TN_EXEC_FAILED, "mps overlap left an open bond");
+        double2 v;
+        HIPC(hipMemcpyAsync(&v, cur, sizeof(double2), hipMemcpyDeviceToHost, stream));
+        sync();
+        out[0] = v.x;
+        out[1] = v.y;
+    }
+
     // ------------------------------------------------------------ readouts
     // Contract a vector through sites [from, to) (left to right) or (to, from] (right to
     // left) with physical weight phys (0, 1 or 2 = sum). Returns the device vector.
@@ -2506,6 +2539,71 @@ int wqc_tn_apply_circuit(wqc_tn_state* st, const wqc_tn_gate* gates, size_t coun
             std::fill(st->touched.begin(), st->touched.end(), 0);
             throw;
         }
+    });
+}
+
+int wqc_tn_overlap(wqc_tn_state* ket, wqc_tn_state* bra, double out[2]) {
+    return guarded([&] {
+        if (!ket || !bra || !out) fail(WQC_TN_INVALID_ARG, "null argument");
+        if (ket->dev != bra->dev) fail(WQC_TN_INVALID_ARG, "states on different devices");
+        HIPC(hipSetDevice(ket->dev->ordinal));
+        ket->overlap_with(*bra, out);
+    });
+}
+
+namespace {
+// adjoint_gate_sequence (src/tn/mps.rs:1398-1419): each gate's adjoint as gates of the same
+// set; S^H = Z S and T^H = Z T T T are applied as S then Z and T, T, T then Z.
+void append_adjoint(const wqc_tn_gate& g, std::vector<wqc_tn_gate>& out) {
+    wqc_tn_gate a = g;
+    switch (g.kind) {
+        case WQC_TN_GATE_RX:
+        case WQC_TN_GATE_RY:
+        case WQC_TN_GATE_RZ:
+            a.angle = -g.angle;
+            out.push_back(a);
+            break;
+        case WQC_TN_GATE_S:
+        case WQC_TN_GATE_T: {
+            const int reps = g.kind == WQC_TN_GATE_S ? 1 : 3;
+            for (int r = 0; r < reps; ++r) out.push_back(g);
+            wqc_tn_gate z = g;
+            z.kind = WQC_TN_GATE_Z;
+            out.push_back(z);
+            break;
+        }
+        case WQC_TN_GATE_MEASURE:
+        case WQC_TN_GATE_RESET:
+        case WQC_TN_GATE_IF:
+            fail(WQC_TN_EXEC_FAILED, "time-split overlap does not apply measurement or reset");
+        default:
+            out.push_back(g);  // H, X, Y, Z, CNOT, CZ, CCNOT are their own adjoints
+    }
+}
+}  // namespace
+
+int wqc_tn_contract_time_split(wqc_tn_device* dev, size_t qubit_count, const wqc_tn_gate* gates, size_t count,
+                               size_t split, size_t max_bond_dim, int precision, double out[2]) {
+    return guarded([&] {
+        if (!dev || !out || (!gates && count)) fail(WQC_TN_INVALID_ARG, "null argument");
+        if (split > count) fail(WQC_TN_EXEC_FAILED, "time-split index past the gate list");
+        auto make = [&]() {
+            wqc_tn_state* p = nullptr;
+            const int rc = wqc_tn_state_new(dev, qubit_count, max_bond_dim, precision, &p);
+            if (rc != WQC_TN_OK) fail(rc, g_last_error);
+            p->reject = true;
+            return std::unique_ptr<wqc_tn_state, void (*)(wqc_tn_state*)>(p, wqc_tn_state_free);
+        };
+        HIPC(hipSetDevice(dev->ordinal));
+        auto ket = make();
+        for (size_t i = 0; i < split; ++i) ket->queue_gate(gates[i]);
+        ket->flush();
+        std::vector<wqc_tn_gate> bra_gates;
+        for (size_t i = count; i-- > split;) append_adjoint(gates[i], bra_gates);
+        auto bra = make();
+        for (const auto& g : bra_gates) bra->queue_gate(g);
+        bra->flush();
+        ket->overlap_with(*bra, out);
     });
 }
 

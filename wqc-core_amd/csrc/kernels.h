@@ -196,6 +196,13 @@ void launch_pauli_site(const double2* A, double2* bx, double2* by, double2* bz, 
                        hipStream_t s);
 void launch_env_close(const CloseJob* jobs, int njobs, hipStream_t s);
 
+// One site of <bra|ket>: En[a', b'] = sum_{a, b, s} E[a, b] K[a, s, a'] conj(B[b, s, b']) with
+// E (kl x bl) and En (kr x br) row-major, K (kl, 2, kr) and B (bl, 2, br) sites; kl * bl and
+// kr * br at most 256 (one workgroup).
+constexpr int OVERLAP_ENV_MAX = 256;
+void launch_overlap_site(const double2* E, const double2* K, const double2* B, double2* En, int kl, int kr,
+                         int bl, int br, hipStream_t s);
+
 // MPO x MPS site: out[(a, w), t, (b, v)] = sum_s W[w][v][t][s] A[a, s, b] for MPO bond indices
 // w in [w0, w0 + wn) of Dl and v in [v0, v0 + vn) of Dr (W is Dl x Dr x 2 x 2, row-major);
 // the combined bond index is mps_index * wn + (w - w0). out is (l*wn) x 2 x (r*vn).

@@ -1783,6 +1783,24 @@ __global__ __launch_bounds__(256) void k_env_close(const CloseJob* __restrict__ 
 }
 
 // ------------------------------------------------------------------ MPO x MPS, operator sites
+__global__ __launch_bounds__(256) void k_overlap_site(const double2* __restrict__ E, const double2* __restrict__ K,
+                                                      const double2* __restrict__ B, double2* __restrict__ En,
+                                                      int kl, int kr, int bl, int br) {
+    __shared__ double2 Es[OVERLAP_ENV_MAX];
+    const int tid = threadIdx.x;
+    if (tid < kl * bl) Es[tid] = E[tid];
+    __syncthreads();
+    if (tid >= kr * br) return;
+    const int ap = tid / br, bp = tid - ap * br;
+    double2 acc = cz();
+    for (int a = 0; a < kl; ++a)
+        for (int b = 0; b < bl; ++b) {
+            const double2 e = Es[a * bl + b];
+            for (int s = 0; s < 2; ++s) cmacc(acc, cmul(e, K[(a * 2 + s) * kr + ap]), B[(b * 2 + s) * br + bp]);
+        }
+    En[tid] = acc;
+}
+
 __global__ __launch_bounds__(256) void k_mpo_site(const double2* __restrict__ A, const double2* __restrict__ W,
                                                   double2* __restrict__ out, int l, int r, int Dl, int Dr,
                                                   int w0, int wn, int v0, int vn) {
@@ -1962,6 +1980,11 @@ void launch_sample_step(const double2* W, const double2* Z, int r, long shots, c
                         hipStream_t s) {
     hipLaunchKernelGGL(k_sample_step, dim3((unsigned)((shots + 3) / 4)), dim3(256), 0, s, W, Z, r,
                        shots, u, ustride, bit, bits, vnext);
+}
+
+void launch_overlap_site(const double2* E, const double2* K, const double2* B, double2* En, int kl, int kr,
+                         int bl, int br, hipStream_t s) {
+    hipLaunchKernelGGL(k_overlap_site, dim3(1), dim3(256), 0, s, E, K, B, En, kl, kr, bl, br);
 }
 
 void launch_env_close(const CloseJob* jobs, int njobs, hipStream_t s) {

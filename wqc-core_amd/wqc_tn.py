@@ -32,7 +32,7 @@ EXPORTED = (
     "wqc_tn_peak_vram_bytes", "wqc_tn_get_stats", "wqc_tn_kernel_times", "wqc_tn_enable_timing",
     "wqc_tn_kernel_launches", "wqc_tn_enable_split_log", "wqc_tn_split_log",
     "wqc_tn_pauli_expectations", "wqc_tn_observable_expectation", "wqc_tn_sample_counts",
-    "wqc_tn_run_slices", "wqc_tn_backward",
+    "wqc_tn_run_slices", "wqc_tn_backward", "wqc_tn_overlap", "wqc_tn_contract_time_split",
 )
 
 
@@ -110,6 +110,8 @@ def load():
                                     ct.POINTER(ct.c_int)]
     L.wqc_tn_sample_counts.argtypes = [vp, i32p, i32p, sz, sz, ct.c_uint64, ct.c_uint64, ct.c_int,
                                        u64p, u64p, sz, ct.POINTER(ct.c_size_t)]
+    L.wqc_tn_overlap.argtypes = [vp, vp, dp]
+    L.wqc_tn_contract_time_split.argtypes = [vp, sz, ct.c_void_p, sz, sz, sz, ct.c_int, dp]
     L.wqc_tn_bond_dims.argtypes = [vp, ct.POINTER(sz), sz]
     L.wqc_tn_download_site.argtypes = [vp, sz, dp, sz, ct.POINTER(sz), ct.POINTER(sz)]
     L.wqc_tn_upload_site.argtypes = [vp, sz, dp, sz, sz]
@@ -192,6 +194,19 @@ def run_slices(slices, precision: int = PRECISION_FP64, workers: int = 8, device
     if rc != OK and all(c == OK for c in codes):
         _check(rc)
     return out, codes
+
+
+def contract_time_split(qubit_count, gates, split, max_bond_dim, precision: int = PRECISION_FP64, device=None):
+    """<0...0| U |0...0> as the overlap of |0...0> through gates[:split] with |0...0> through
+    the adjoints of gates[split:] in reverse, both untruncated at max_bond_dim
+    (contract_time_split, src/tn/mps.rs:1241-1249)."""
+    L = load()
+    dev = device or default_device()
+    garr = gate_array(gates)
+    out = np.zeros(2)
+    _check(L.wqc_tn_contract_time_split(dev._h, qubit_count, ct.cast(garr, ct.c_void_p), len(gates), split,
+                                        max_bond_dim, precision, _dp(out)))
+    return complex(out[0], out[1])
 
 
 def backward(qubit_count, max_bond_dim, gates, terms, checkpoint_every=2, adjoint_bond=0,
@@ -282,6 +297,12 @@ class MpsState:
     def amplitude_at_compact_zero(self) -> complex:
         out = np.zeros(2)
         _check(load().wqc_tn_amplitude_at_compact_zero(self._h, _dp(out)))
+        return complex(out[0], out[1])
+
+    def overlap(self, bra: "MpsState") -> complex:
+        """<bra|self> (overlap, src/tn/mps.rs:1421-1474); environments above 256 entries fail."""
+        out = np.zeros(2)
+        _check(load().wqc_tn_overlap(self._h, bra._h, _dp(out)))
         return complex(out[0], out[1])
 
     def site_amplitudes(self, target: int):
