@@ -203,6 +203,12 @@ int wqc_tn_amplitude_at_compact_zero(wqc_tn_state *st, double out[2]);
 int wqc_tn_site_amplitudes(wqc_tn_state *st, size_t target, double out[4]);
 /* MpsState::control_probability (src/tn/mps.rs:695-698). */
 int wqc_tn_control_probability(wqc_tn_state *st, size_t control, double *out);
+/* site_amplitudes of many targets of one state (src/tn/mps.rs:679-693), for the trace's
+ * per-layer probes (first touch of each target and every control, src/tn/trace.rs:346-420,
+ * 640-657): out[4k..4k+3] = (a0.re, a0.im, a1.re, a1.im) of targets[k]; the control
+ * probability of a control c is |a1|^2 (mps.rs:695-698). One sweep of environments serves
+ * every target. INVALID_ARG for a target outside the register. */
+int wqc_tn_site_amplitudes_batch(wqc_tn_state *st, const int32_t *targets, size_t count, double *out);
 
 /* MpsState::contract_to_statevector (src/tn/mps.rs:215-260): n <= 20, else
  * EXEC_FAILED "full statevector projection requires qubit_count <= 20".

@@ -151,6 +151,37 @@ def test_site_amplitudes_vs_oracle(wt):
         assert abs(g.control_probability(t) - o.control_probability(t)) <= 1e-12
 
 
+def test_site_amplitudes_batch_vs_single_and_oracle(wt):
+    """The trace's per-layer probes in one call: every target (repeats and any order) equals
+    the single-target probe and the oracle; config 2 (40 qubits, chi 128) for all 40 targets
+    against the single-target calls."""
+    import time
+
+    n = 12
+    gates = C.brickwork(n, 7, 123)
+    g, o = run_both(wt, n, 32, gates)
+    targets = [5, 0, n - 1, 5, 3, 11, 7]
+    got = g.site_amplitudes_batch(targets)
+    for t, (a0, a1) in zip(targets, got):
+        oa = o.site_amplitudes(t)
+        for x, y in zip((a0, a1), oa):
+            assert abs(x - y) <= 1e-12 * max(1.0, abs(y)), (t, x, y)
+    n, chi, gates = C.config2()
+    st = wt.MpsState(n, chi)
+    st.apply_circuit(gates)
+    t0 = time.time()
+    single = [st.site_amplitudes(t) for t in range(n)]
+    t1 = time.time()
+    batch = st.site_amplitudes_batch(list(range(n)))
+    t2 = time.time()
+    say("config2 probes: 40 single calls %.1f ms, one batch %.1f ms" % (1e3 * (t1 - t0), 1e3 * (t2 - t1)))
+    for (a0, a1), (b0, b1) in zip(single, batch):
+        assert abs(a0 - b0) <= 1e-12 * max(1.0, abs(a0)) and abs(a1 - b1) <= 1e-12 * max(1.0, abs(a1))
+    with pytest.raises(wt.WqcTnError):
+        st.site_amplitudes_batch([n])
+    st.free()
+
+
 def test_reject_truncation_and_errors(wt):
     st = wt.MpsState(8, 2)
     st.set_reject_truncation(True)

@@ -1990,6 +1990,79 @@ struct wqc_tn_state {
         out[3] = a1.y;
     }
 
+    // site_amplitudes for many targets of one state (the trace's first-touch probes and control
+    // probabilities of a layer, src/tn/trace.rs:346-420): the all-ones left environments of
+    // every prefix and right environments of every suffix in one sweep each, then one launch
+    // for all targets; out[4k..4k+3] = (a0, a1) of targets[k].
+    void site_amplitudes_batch(const int32_t* targets, size_t count, double* out) {
+        flush();
+        sync();
+        tables_reset();
+        Timer tm(this, 5);
+        std::vector<size_t> loff(n + 1), roff(n + 2);
+        std::vector<int> llen(n + 1), rlen(n + 2);
+        size_t total = 0;
+        llen[0] = 1;
+        loff[0] = total;
+        total += 1;
+        for (int q = 0; q < n; ++q) {
+            llen[q + 1] = sites[q].r;
+            loff[q + 1] = total;
+            total += sites[q].r;
+        }
+        rlen[n] = 1;
+        roff[n] = total;
+        total += 1;
+        for (int q = n - 1; q >= 0; --q) {
+            rlen[q] = sites[q].l;
+            roff[q] = total;
+            total += sites[q].l;
+        }
+        size_t mx = 2;
+        for (auto& st : sites) mx = std::max(mx, (size_t)std::max(st.l, st.r) + 2);
+        const size_t part = (mx / wqc::kVecMatChunk + 2) * mx;
+        env_buf.ensure((total + part + count * 2) * sizeof(double2) + 4096);
+        note_peak();
+        double2* E = reinterpret_cast<double2*>(env_buf.p);
+        vm_part = E + total;
+        double2* amps = vm_part + part;
+        HIPC(hipMemcpyAsync(E + loff[0], &kOne, sizeof(double2), hipMemcpyHostToDevice, stream));
+        HIPC(hipMemcpyAsync(E + roff[n], &kOne, sizeof(double2), hipMemcpyHostToDevice, stream));
+        std::vector<VecMatJob> jobs;
+        for (int q = 0; q < n; ++q)
+            jobs.push_back(VecMatJob{sites[q].ptr, E + loff[q], E + loff[q + 1], sites[q].l, sites[q].r, llen[q], 2, 0, 0,
+                                     vm_part});
+        for (int q = n - 1; q >= 0; --q)
+            jobs.push_back(VecMatJob{sites[q].ptr, E + roff[q + 1], E + roff[q], sites[q].l, sites[q].r, rlen[q + 1], 2, 1,
+                                     0, vm_part});
+        VecMatJob* dj = upload(jobs);
+        for (int q = 0; q < n; ++q)
+            wqc::launch_vecmat(dj + q, sites[q].r, std::min(sites[q].l, llen[q]), 0, stream);
+        for (int k = 0; k < n; ++k) {
+            const int q = n - 1 - k;
+            wqc::launch_vecmat(dj + n + k, sites[q].l, std::min(sites[q].r, rlen[q + 1]), 1, stream);
+        }
+        klaunch[5] += 2 * n + 1;
+        std::vector<wqc::SiteAmpJob> sj(count);
+        for (size_t k = 0; k < count; ++k) {
+            const int t = targets[k];
+            sj[k] = wqc::SiteAmpJob{E + loff[t], sites[t].ptr, E + roff[t + 1], amps + 2 * k, llen[t], sites[t].l,
+                                    sites[t].r, rlen[t + 1]};
+        }
+        wqc::launch_site_amps(upload(sj), (int)count, stream);
+        check_launch();
+        std::vector<double2> h(2 * count);
+        HIPC(hipMemcpyAsync(h.data(), amps, 2 * count * sizeof(double2), hipMemcpyDeviceToHost, stream));
+        sync();
+        for (size_t k = 0; k < count; ++k) {
+            out[4 * k] = h[2 * k].x;
+            out[4 * k + 1] = h[2 * k].y;
+            out[4 * k + 2] = h[2 * k + 1].x;
+            out[4 * k + 3] = h[2 * k + 1].y;
+        }
+    }
+    DevBuf env_buf;
+
     // Left norm environments L_q (sites < q) and conjugated right norm environments Rb_q
     // (sites >= q), q = 0..n, each l_q x l_q at env_off[q]; L may be null. FL and HR are
     // scratch of 2 maxb^2 elements. L_{q+1} = A_stack^H (L_q A_flat) and
@@ -2704,6 +2777,15 @@ int wqc_tn_site_amplitudes(wqc_tn_state* st, size_t target, double out[4]) {
     return guarded([&] {
         if (!st || !out || target >= (size_t)st->n) fail(WQC_TN_INVALID_ARG, "bad target");
         st->site_amplitudes((int)target, out);
+    });
+}
+
+int wqc_tn_site_amplitudes_batch(wqc_tn_state* st, const int32_t* targets, size_t count, double* out) {
+    return guarded([&] {
+        if (!st || (count && (!targets || !out))) fail(WQC_TN_INVALID_ARG, "null argument");
+        for (size_t k = 0; k < count; ++k)
+            if (targets[k] < 0 || targets[k] >= st->n) fail(WQC_TN_INVALID_ARG, "bad target");
+        if (count) st->site_amplitudes_batch(targets, count, out);
     });
 }
 

@@ -142,6 +142,17 @@ struct VecMatJob {
     double2* part;
 };
 
+// Site amplitudes of one target from its left and right environments:
+// out[s] = sum_{a < min(l, llen), b < r} Lv[a] site[a, s, b] (b < rlen ? Rv[b] : 1), s = 0, 1.
+struct SiteAmpJob {
+    const double2* Lv;
+    const double2* site;
+    const double2* Rv;
+    double2* out;  // 2 values
+    int llen, l, r, rlen;
+};
+void launch_site_amps(const SiteAmpJob* jobs, int njobs, hipStream_t s);
+
 void launch_one_qubit(const OneQubitJob* jobs, int njobs, int max_elems, hipStream_t s);
 void launch_merge(const MergeJob* jobs, int njobs, int max_dl, int max_dr, hipStream_t s);
 void launch_sumsq(const double2* const* ptrs, const long* counts, double* partial,

@@ -1783,6 +1783,29 @@ __global__ __launch_bounds__(256) void k_env_close(const CloseJob* __restrict__ 
 }
 
 // ------------------------------------------------------------------ MPO x MPS, operator sites
+// One workgroup per (target, s): the bilinear form Lv . site[:, s, :] . Rv, summed per thread
+// over a strided share of (a, b) and then over the threads in a fixed tree.
+__global__ __launch_bounds__(256) void k_site_amps(const SiteAmpJob* __restrict__ jobs) {
+    const SiteAmpJob jb = jobs[blockIdx.x];
+    const int s = blockIdx.y, tid = threadIdx.x;
+    const int la = min(jb.l, jb.llen);
+    const long total = (long)la * jb.r;
+    double2 acc = cz();
+    for (long e = tid; e < total; e += 256) {
+        const int a = (int)(e / jb.r), b = (int)(e - (long)a * jb.r);
+        const double2 w = b < jb.rlen ? jb.Rv[b] : make_double2(1.0, 0.0);
+        cmac(acc, cmul(jb.Lv[a], jb.site[((long)a * 2 + s) * jb.r + b]), w);
+    }
+    __shared__ double2 red[256];
+    red[tid] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) red[tid] = cadd(red[tid], red[tid + w]);
+        __syncthreads();
+    }
+    if (tid == 0) jb.out[s] = red[0];
+}
+
 __global__ __launch_bounds__(256) void k_overlap_site(const double2* __restrict__ E, const double2* __restrict__ K,
                                                       const double2* __restrict__ B, double2* __restrict__ En,
                                                       int kl, int kr, int bl, int br) {
@@ -1980,6 +2003,10 @@ void launch_sample_step(const double2* W, const double2* Z, int r, long shots, c
                         hipStream_t s) {
     hipLaunchKernelGGL(k_sample_step, dim3((unsigned)((shots + 3) / 4)), dim3(256), 0, s, W, Z, r,
                        shots, u, ustride, bit, bits, vnext);
+}
+
+void launch_site_amps(const SiteAmpJob* jobs, int njobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_site_amps, dim3(njobs, 2), dim3(256), 0, s, jobs);
 }
 
 void launch_overlap_site(const double2* E, const double2* K, const double2* B, double2* En, int kl, int kr,

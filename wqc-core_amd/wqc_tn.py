@@ -33,6 +33,7 @@ EXPORTED = (
     "wqc_tn_kernel_launches", "wqc_tn_enable_split_log", "wqc_tn_split_log",
     "wqc_tn_pauli_expectations", "wqc_tn_observable_expectation", "wqc_tn_sample_counts",
     "wqc_tn_run_slices", "wqc_tn_backward", "wqc_tn_overlap", "wqc_tn_contract_time_split",
+    "wqc_tn_site_amplitudes_batch",
 )
 
 
@@ -111,6 +112,7 @@ def load():
     L.wqc_tn_sample_counts.argtypes = [vp, i32p, i32p, sz, sz, ct.c_uint64, ct.c_uint64, ct.c_int,
                                        u64p, u64p, sz, ct.POINTER(ct.c_size_t)]
     L.wqc_tn_overlap.argtypes = [vp, vp, dp]
+    L.wqc_tn_site_amplitudes_batch.argtypes = [vp, ct.POINTER(ct.c_int32), sz, dp]
     L.wqc_tn_contract_time_split.argtypes = [vp, sz, ct.c_void_p, sz, sz, sz, ct.c_int, dp]
     L.wqc_tn_bond_dims.argtypes = [vp, ct.POINTER(sz), sz]
     L.wqc_tn_download_site.argtypes = [vp, sz, dp, sz, ct.POINTER(sz), ct.POINTER(sz)]
@@ -304,6 +306,15 @@ class MpsState:
         out = np.zeros(2)
         _check(load().wqc_tn_overlap(self._h, bra._h, _dp(out)))
         return complex(out[0], out[1])
+
+    def site_amplitudes_batch(self, targets):
+        """[(a0, a1)] of site_amplitudes for every target, from one sweep of environments."""
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        out = np.zeros(4 * max(1, len(t)))
+        _check(load().wqc_tn_site_amplitudes_batch(self._h, t.ctypes.data_as(ct.POINTER(ct.c_int32)), len(t),
+                                                   _dp(out)))
+        return [(complex(out[4 * k], out[4 * k + 1]), complex(out[4 * k + 2], out[4 * k + 3]))
+                for k in range(len(t))]
 
     def site_amplitudes(self, target: int):
         out = np.zeros(4)
