@@ -270,6 +270,15 @@ int wqc_tn_sample_counts(wqc_tn_state *st, const int32_t *measure_qubits,
                          const int32_t *measure_cbits, size_t n_measures, size_t classical_bits,
                          uint64_t shots, uint64_t seed, int method, uint64_t *keys,
                          uint64_t *counts, size_t cap, size_t *n_out);
+
+/* compute_outcome_probabilities (src/sample.rs:85-131) for qubit_count <= 20: the snapped
+ * Born table of the projected statevector, one (key, probability) per outcome with p > 0 in
+ * key order (bit c of a key = classical bit c); the table build_terminal_distribution_segment
+ * (src/distribution_proof.rs:139-185) and the exact sampler draw from. Same argument checks
+ * and errors as wqc_tn_sample_counts; INVALID_ARG when cap is too small. */
+int wqc_tn_outcome_probabilities(wqc_tn_state *st, const int32_t *measure_qubits, const int32_t *measure_cbits,
+                                 size_t n_measures, size_t classical_bits, uint64_t *keys, double *probs,
+                                 size_t cap, size_t *n_out);
 /* observable_expectation + compute_expectations (src/expectation.rs:170-200): sum_k coeff_k
  * <P_k> with complex coefficients (coeffs = n_terms (re, im) pairs), each component snapped to
  * the 1e-6 grid (snap_complex_result, src/engine.rs:30-40). out = {re, im}. */

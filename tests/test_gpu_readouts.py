@@ -248,3 +248,52 @@ def test_mps_sampler_config2_marginals(wt):
         worst = max(worst, abs(emp - z) / se)
         assert abs(emp - z) <= 5 * se + 1e-3, (q, emp, z)
     say("config2 sampler: worst marginal deviation in standard errors", worst)
+
+
+def test_outcome_probabilities_kat_and_oracle(wt):
+    """compute_outcome_probabilities (src/sample.rs:85-131) on the device statevector: the
+    Bell-state table of src/distribution_proof.rs:242-265 (two entries, 0.707107^2 each after
+    the 1e-6 snap), then random circuits and measure maps against the snapped table of the
+    oracle's own statevector, and the reference's errors."""
+    g = wt.MpsState(2, 4)
+    g.apply_circuit([C.one(C.H, 0), C.two(C.CNOT, 0, 1)])
+    table = g.outcome_probabilities([(0, 0), (1, 1)], 2)
+    assert sorted(table) == ["00", "11"]
+    assert table["00"] == 0.707107 * 0.707107 and table["11"] == 0.707107 * 0.707107
+    from test_oracle import seeded_random_circuit
+
+    rng = C.SplitMix64(0x0B_2026)
+    for case in range(8):
+        n = 2 + rng.below(11)
+        gates = seeded_random_circuit(rng, n, 8 + rng.below(16))
+        g = wt.MpsState(n, 64)
+        g.apply_circuit(gates)
+        o = om.OracleMps(n, 64)
+        o.apply_circuit(gates)
+        sv = o.contract_to_statevector()
+        cbits = 1 + rng.below(n + 2)
+        measures = [(rng.below(n), rng.below(cbits)) for _ in range(1 + rng.below(n))]
+        want = {}
+        for basis, amp in enumerate(sv):
+            re, im = round(amp.real * 1e6) / 1e6, round(amp.imag * 1e6) / 1e6
+            p = re * re + im * im
+            if p == 0.0:
+                continue
+            key = 0
+            for q, c in measures:
+                if (basis >> q) & 1:
+                    key |= 1 << c
+                else:
+                    key &= ~(1 << c)
+            k = format(key, "0%db" % cbits)
+            want[k] = want.get(k, 0.0) + p
+        got = g.outcome_probabilities(measures, cbits)
+        assert sorted(got) == sorted(want), (case, n, measures)
+        for k in want:
+            assert abs(got[k] - want[k]) <= 1e-12, (case, k, got[k], want[k])
+    with pytest.raises(wt.WqcTnError) as e:
+        wt.MpsState(3, 4).outcome_probabilities([(0, 0)], 0)
+    assert e.value.code == wt.EXEC_FAILED and "classical_bit_count must be > 0" in str(e.value)
+    with pytest.raises(wt.WqcTnError) as e:
+        wt.MpsState(24, 4).outcome_probabilities([(0, 0)], 1)
+    assert e.value.code == wt.EXEC_FAILED and "qubit_count <= 20" in str(e.value)

@@ -2098,9 +2098,8 @@ struct wqc_tn_state {
     //         norm environments R and the carried left vector v renormalized by sqrt(p_s);
     //         the uniform for (shot k, qubit q) is draw k*n + q of the same seeded stream.
     //         Shots are exact samples of |psi_MPS|^2 / <psi|psi>.
-    std::vector<std::pair<uint64_t, uint64_t>> sample_counts(const int32_t* mq, const int32_t* mc,
-                                                             size_t nm, size_t cbits, uint64_t shots,
-                                                             uint64_t seed, int method) {
+    // The checks of compute_outcome_probabilities (src/sample.rs:91-109).
+    void check_measures(const int32_t* mq, const int32_t* mc, size_t nm, size_t cbits) const {
         if (cbits == 0) fail(WQC_TN_EXEC_FAILED, "classical_bit_count must be > 0 for sample_counts");
         if (cbits > 64) fail(WQC_TN_INVALID_ARG, "classical_bit_count above 64 is not supported");
         for (size_t k = 0; k < nm; ++k) {
@@ -2111,13 +2110,27 @@ struct wqc_tn_state {
                 fail(WQC_TN_EXEC_FAILED, "classical bit index " + std::to_string(mc[k]) +
                                              " out of bounds (limit: " + std::to_string(cbits) + ")");
         }
+    }
+
+    // compute_outcome_probabilities (src/sample.rs:85-131) of this state: the statevector
+    // projected on the device, the snapped Born table on the host (born.cpp), key order.
+    std::vector<wqc::BornEntry> outcome_probabilities(const int32_t* mq, const int32_t* mc, size_t nm,
+                                                      size_t cbits) {
+        check_measures(mq, mc, nm, cbits);
+        if (n > 20) fail(WQC_TN_EXEC_FAILED, "full statevector projection requires qubit_count <= 20");
+        std::vector<double> sv((size_t)2 << n);
+        statevector(sv.data(), (size_t)1 << n);
+        return wqc::born_table(sv.data(), n, mq, mc, nm, cbits);
+    }
+
+    std::vector<std::pair<uint64_t, uint64_t>> sample_counts(const int32_t* mq, const int32_t* mc,
+                                                             size_t nm, size_t cbits, uint64_t shots,
+                                                             uint64_t seed, int method) {
+        check_measures(mq, mc, nm, cbits);
         const bool exact = method == 0 || (method == 2 && n <= 20);
         if (method < 0 || method > 2) fail(WQC_TN_INVALID_ARG, "unknown sampling method");
         if (exact) {
-            if (n > 20) fail(WQC_TN_EXEC_FAILED, "full statevector projection requires qubit_count <= 20");
-            std::vector<double> sv((size_t)2 << n);
-            statevector(sv.data(), (size_t)1 << n);
-            return wqc::draw_counts(wqc::born_table(sv.data(), n, mq, mc, nm, cbits), shots, seed);
+            return wqc::draw_counts(outcome_probabilities(mq, mc, nm, cbits), shots, seed);
         }
         if (n > 64) fail(WQC_TN_INVALID_ARG, "the MPS sampler supports qubit_count <= 64");
         flush();
@@ -2836,6 +2849,22 @@ int wqc_tn_sample_counts(wqc_tn_state* st, const int32_t* measure_qubits, const 
         for (size_t i = 0; i < res.size(); ++i) {
             keys[i] = res[i].first;
             counts[i] = res[i].second;
+        }
+        *n_out = res.size();
+    });
+}
+
+int wqc_tn_outcome_probabilities(wqc_tn_state* st, const int32_t* measure_qubits, const int32_t* measure_cbits,
+                                 size_t n_measures, size_t classical_bits, uint64_t* keys, double* probs, size_t cap,
+                                 size_t* n_out) {
+    return guarded([&] {
+        if (!st || !n_out || (n_measures && (!measure_qubits || !measure_cbits)) || (cap && (!keys || !probs)))
+            fail(WQC_TN_INVALID_ARG, "null argument");
+        auto res = st->outcome_probabilities(measure_qubits, measure_cbits, n_measures, classical_bits);
+        if (res.size() > cap) fail(WQC_TN_INVALID_ARG, "output capacity too small for the distinct outcomes");
+        for (size_t i = 0; i < res.size(); ++i) {
+            keys[i] = res[i].key;
+            probs[i] = res[i].p;
         }
         *n_out = res.size();
     });

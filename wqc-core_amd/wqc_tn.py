@@ -33,7 +33,7 @@ EXPORTED = (
     "wqc_tn_kernel_launches", "wqc_tn_enable_split_log", "wqc_tn_split_log",
     "wqc_tn_pauli_expectations", "wqc_tn_observable_expectation", "wqc_tn_sample_counts",
     "wqc_tn_run_slices", "wqc_tn_backward", "wqc_tn_overlap", "wqc_tn_contract_time_split",
-    "wqc_tn_site_amplitudes_batch",
+    "wqc_tn_site_amplitudes_batch", "wqc_tn_outcome_probabilities",
 )
 
 
@@ -114,6 +114,7 @@ def load():
     L.wqc_tn_overlap.argtypes = [vp, vp, dp]
     L.wqc_tn_site_amplitudes_batch.argtypes = [vp, ct.POINTER(ct.c_int32), sz, dp]
     L.wqc_tn_contract_time_split.argtypes = [vp, sz, ct.c_void_p, sz, sz, sz, ct.c_int, dp]
+    L.wqc_tn_outcome_probabilities.argtypes = [vp, i32p, i32p, sz, sz, u64p, dp, sz, ct.POINTER(ct.c_size_t)]
     L.wqc_tn_bond_dims.argtypes = [vp, ct.POINTER(sz), sz]
     L.wqc_tn_download_site.argtypes = [vp, sz, dp, sz, ct.POINTER(sz), ct.POINTER(sz)]
     L.wqc_tn_upload_site.argtypes = [vp, sz, dp, sz, sz]
@@ -363,6 +364,21 @@ class MpsState:
         for i in range(nout.value):
             out[format(int(keys[i]), "0%db" % classical_bits)] = int(cnt[i])
         return out
+
+    def outcome_probabilities(self, measures, classical_bits):
+        """{bitstring: probability} of compute_outcome_probabilities (src/sample.rs:85-131), the
+        snapped Born table of the projected statevector (qubit_count <= 20)."""
+        mq = np.array([m[0] for m in measures] or [0], dtype=np.int32)
+        mc = np.array([m[1] for m in measures] or [0], dtype=np.int32)
+        cap = int(max(1, 1 << min(max(classical_bits, 0), self.qubit_count, 20)))
+        keys = np.zeros(cap, dtype=np.uint64)
+        probs = np.zeros(cap)
+        nout = ct.c_size_t(0)
+        i32p, u64p = ct.POINTER(ct.c_int32), ct.POINTER(ct.c_uint64)
+        _check(load().wqc_tn_outcome_probabilities(self._h, mq.ctypes.data_as(i32p), mc.ctypes.data_as(i32p),
+                                                   len(measures), classical_bits, keys.ctypes.data_as(u64p),
+                                                   _dp(probs), cap, ct.byref(nout)))
+        return {format(int(keys[i]), "0%db" % classical_bits): float(probs[i]) for i in range(nout.value)}
 
     def observable_expectation(self, terms):
         """Snapped sum of coeff * <P> over (label, coeff) terms, src/expectation.rs:170-200."""
