@@ -182,6 +182,51 @@ def test_site_amplitudes_batch_vs_single_and_oracle(wt):
     st.free()
 
 
+def test_sweep_replay_hooks_vs_dense(wt):
+    """The sweep-tree replay hooks (src/tn/mps.rs:621-655, driven by tree.rs:704-788):
+    apply_unary_at / apply_tdg_at on a site, apply_swap_at / apply_pair_at / apply_cnot_at on
+    the pair (site, site + 1) with the control on either side, in a random order, against the
+    dense statevector of the same operations (no truncation at chi 64 on 8 qubits)."""
+    import dense_ref as D
+
+    n = 8
+    rng = C.SplitMix64(0x5EE9)
+    st = wt.MpsState(n, 64)
+    psi = np.zeros(2 ** n, dtype=complex)
+    psi[0] = 1.0
+    for _ in range(120):
+        r = rng.below(5)
+        site = rng.below(n - 1) if r >= 2 else rng.below(n)
+        if r == 0:
+            kind = (C.H, C.X, C.Y, C.Z, C.S, C.T, C.RX, C.RY, C.RZ)[rng.below(9)]
+            ang = 2 * math.pi * rng.uniform()
+            st.apply_unary_at(site, C.one(kind, 0, ang))
+            psi = D._apply_1q(psi, n, site, D.unary(kind, ang))
+        elif r == 1:
+            st.apply_tdg_at(site)
+            psi = D._apply_1q(psi, n, site, D.unary(C.T).conj())
+        elif r == 2:
+            st.apply_swap_at(site)
+            for a, b in ((site, site + 1), (site + 1, site), (site, site + 1)):
+                psi = D.apply_gate(psi, n, C.two(C.CNOT, a, b))
+        else:
+            left = rng.below(2) == 1
+            c, t = (site, site + 1) if left else (site + 1, site)
+            if r == 3:
+                kind = (C.CNOT, C.CZ)[rng.below(2)]
+                st.apply_pair_at(site, C.two(kind, 0, 1), ctrl_on_left=left)
+                psi = D.apply_gate(psi, n, C.two(kind, c, t))
+            else:
+                st.apply_cnot_at(site, ctrl_on_left=left)
+                psi = D.apply_gate(psi, n, C.two(C.CNOT, c, t))
+    got = st.contract_to_statevector()
+    err = np.max(np.abs(got - psi))
+    say("sweep replay hooks: max |gpu - dense| =", err)
+    assert err <= 1e-12
+    assert abs(st.amplitude_at_compact_zero() - psi[0]) <= 1e-12
+    st.free()
+
+
 def test_reject_truncation_and_errors(wt):
     st = wt.MpsState(8, 2)
     st.set_reject_truncation(True)
