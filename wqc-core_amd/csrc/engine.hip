@@ -1615,6 +1615,14 @@ struct wqc_tn_state {
             for (double& s : values)
                 if (s < SINGULAR_CUTOFF * 1e-2) s = 0.0;
             const size_t c = retained_chi(values, chi, j.rows, j.cols);
+            if (trace_jacobi && (!svd_ok || c == 0 || needs_f64_fallback(values, c))) {
+                std::fprintf(stderr, "rsvd fallback: site %d ell %zu kept %zu svd_ok %d", j.p, j.ell, c, (int)svd_ok);
+                for (int i = 0; i < nn; ++i)
+                    if (values[i] > 0.0 && std::fabs(values[i] - SINGULAR_CUTOFF) <= CUTOFF_ABS_BAND)
+                        std::fprintf(stderr, " s[%d]=%.3e", i, values[i]);
+                if (c > 0 && c < (size_t)nn) std::fprintf(stderr, " kept %.6e dropped %.6e", values[c - 1], values[c]);
+                std::fprintf(stderr, "\n");
+            }
             if (!svd_ok || c == 0 || needs_f64_fallback(values, c)) {
                 j.fallback = true;
                 stats.rsvd_fallbacks++;
