@@ -405,6 +405,7 @@ struct wqc_tn_state {
     DevBuf ws32;      // mixed-precision rows and the J X product
     DevBuf ews;       // expectation environments and transfer buffers
     DevBuf tab;       // device job tables
+    DevBuf gather;    // sigma and |A|_F^2 of a Jacobi batch, gathered for one readback
     char* htab = nullptr;  // pinned staging for tables
     size_t htab_cap = 0, tab_off = 0;
     double* hread = nullptr;  // pinned readback
@@ -438,6 +439,10 @@ struct wqc_tn_state {
         tab.release();
         jstats_buf.release();
         trace_buf.release();
+        splitk_buf.release();
+        gather.release();
+        ov_buf.release();
+        env_buf.release();
         for (auto& sp : spans) {
             (void)hipEventDestroy(sp.a);
             (void)hipEventDestroy(sp.b);
@@ -450,7 +455,7 @@ struct wqc_tn_state {
 
     // ------------------------------------------------------------ helpers
     void note_peak() {
-        uint64_t now = site_bytes + ws.bytes + ws32.bytes + ews.bytes + tab.bytes + splitk_buf.bytes;
+        uint64_t now = site_bytes + ws.bytes + ws32.bytes + ews.bytes + tab.bytes + splitk_buf.bytes + gather.bytes;
         if (now > stats.peak_vram_bytes) stats.peak_vram_bytes = now;
     }
 
@@ -686,7 +691,6 @@ struct wqc_tn_state {
         // Per job: tolerance JACOBI_TOL sqrt(m), null-row floor NULL_ROWS_SQ / n (in the kernel).
         const double tol = JACOBI_TOL;
         const double nu2 = NULL_ROWS_SQ;
-        int* d_rot = d_flags;
         int* d_done = d_flags + nj;
         unsigned long long* d_jstats = jstats_dev();
         wqc::launch_fill_int(d_done, 0, nj, stream);
@@ -698,15 +702,16 @@ struct wqc_tn_state {
         // (quadratic convergence would have taken it far below) and stops.
         // WQC_TN_JACOBI_TRACE=1 prints it per sweep for the widest jobs, and the history of
         // the jobs that needed the most sweeps.
-        trace_buf.ensure((size_t)nj * sizeof(unsigned long long));
+        // The per-sweep flags live together: offmax (nj u64) then rotated (nj int), cleared by
+        // one fill and read back by one copy.
+        trace_buf.ensure((size_t)nj * (sizeof(unsigned long long) + sizeof(int)));
         unsigned long long* d_off = reinterpret_cast<unsigned long long*>(trace_buf.p);
+        int* d_rot = reinterpret_cast<int*>(d_off + nj);
         std::vector<double> prev_off(nj, 0.0);
-        std::vector<unsigned long long> offv(nj);
         std::vector<std::vector<double>> hist(trace_jacobi ? nj : 0);
         for (int sweep = 0; sweep < MAX_SWEEPS && active > 0; ++sweep) {
             check_cancel();
-            wqc::launch_fill_int(d_rot, 0, nj, stream);
-            if (d_off) wqc::launch_fill_int(reinterpret_cast<int*>(d_off), 0, 2 * nj, stream);
+            wqc::launch_fill_int(reinterpret_cast<int*>(d_off), 0, 3 * nj, stream);
             {
                 Timer tm(this, 3);
                 for (int r = 0; r < rounds; ++r)
@@ -716,10 +721,11 @@ struct wqc_tn_state {
                 klaunch[3] += rounds;
             }
             stats.jacobi_sweeps += active;
-            HIPC(hipMemcpyAsync(offv.data(), d_off, nj * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
-            HIPC(hipMemcpyAsync(hread, d_rot, nj * sizeof(int), hipMemcpyDeviceToHost, stream));
+            HIPC(hipMemcpyAsync(hread, d_off, nj * (sizeof(unsigned long long) + sizeof(int)), hipMemcpyDeviceToHost,
+                                stream));
             sync();
-            const int* rot = reinterpret_cast<const int*>(hread);
+            const unsigned long long* offv = reinterpret_cast<const unsigned long long*>(hread);
+            const int* rot = reinterpret_cast<const int*>(offv + nj);
             if (trace_jacobi) {
                 std::fprintf(stderr, "jacobi sweep %d active %d:", sweep + 1, active);
                 int shown = 0;
@@ -749,8 +755,8 @@ struct wqc_tn_state {
                     if (stalled) stats.jacobi_stalled++;
                 }
             }
-            if (changed && active > 0)
-                HIPC(hipMemcpyAsync(d_done, upload_ints(done), nj * sizeof(int), hipMemcpyDeviceToDevice, stream));
+            // The rounds read the done flags from the uploaded table itself.
+            if (changed && active > 0) d_done = upload_ints(done);
         }
         if (trace_jacobi) {
             std::vector<int> order(nj);
@@ -837,21 +843,21 @@ struct wqc_tn_state {
         const int rounds = nbe - 1, pairs = nbe / 2;
         const float tol = (float)tol_lo;
         const double nu2 = NULL_ROWS_SQ;
-        int* d_rot = d_flags;
         int* d_done = d_flags + nj;
         unsigned long long* d_jstats = jstats_dev();
         wqc::launch_fill_int(d_done, 0, nj, stream);
         std::vector<int> done(nj, 0);
         int active = nj;
-        unsigned long long* d_off = nullptr;
-        if (trace_jacobi) {
-            trace_buf.ensure((size_t)nj * sizeof(unsigned long long));
-            d_off = reinterpret_cast<unsigned long long*>(trace_buf.p);
-        }
+        // offmax (nj u64, collected only when tracing) then rotated (nj int), as in jacobi().
+        trace_buf.ensure((size_t)nj * (sizeof(unsigned long long) + sizeof(int)));
+        unsigned long long* d_off = trace_jacobi ? reinterpret_cast<unsigned long long*>(trace_buf.p) : nullptr;
+        int* d_rot = reinterpret_cast<int*>(reinterpret_cast<unsigned long long*>(trace_buf.p) + nj);
         for (int sweep = 0; sweep < max_sweeps_lo && active > 0; ++sweep) {
             check_cancel();
-            wqc::launch_fill_int(d_rot, 0, nj, stream);
-            if (d_off) wqc::launch_fill_int(reinterpret_cast<int*>(d_off), 0, 2 * nj, stream);
+            if (d_off)
+                wqc::launch_fill_int(reinterpret_cast<int*>(d_off), 0, 3 * nj, stream);
+            else
+                wqc::launch_fill_int(d_rot, 0, nj, stream);
             {
                 Timer tm(this, 3);
                 for (int r = 0; r < rounds; ++r)
@@ -888,8 +894,7 @@ struct wqc_tn_state {
                     active--;
                     changed = true;
                 }
-            if (changed && active > 0)
-                HIPC(hipMemcpyAsync(d_done, upload_ints(done), nj * sizeof(int), hipMemcpyDeviceToDevice, stream));
+            if (changed && active > 0) d_done = upload_ints(done);
         }
         // J = (complex128) J32, made unitary to FP64 precision by two Newton-Schulz steps
         // J <- J (3I - J^H J) / 2 (the FP32 rounds leave it unitary to about 1e-6).
@@ -909,15 +914,21 @@ struct wqc_tn_state {
             check_launch();
             for (int it = 0; it < 2; ++it) {
                 std::vector<GemmJob> g1, g2;
+                std::vector<const double*> cs;
+                std::vector<double*> cd;
+                std::vector<long> cc;
                 for (int k = 0; k < nj; ++k) {
                     const int nn = jobs[k].n;
                     double2* src_j = it == 0 ? jobs[k].J : reinterpret_cast<double2*>(b32 + ojn[k]);
                     double2* dst_j = it == 0 ? reinterpret_cast<double2*>(b32 + ojn[k]) : jobs[k].J;
                     double2* w = reinterpret_cast<double2*>(b32 + ow[k]);
                     g1.push_back(gj(src_j, nn, 1, src_j, nn, 0, w, nn, nn, nn, nn, 1.0, 0.0));
-                    HIPC(hipMemcpyAsync(dst_j, src_j, (size_t)nn * nn * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+                    cs.push_back(reinterpret_cast<const double*>(src_j));
+                    cd.push_back(reinterpret_cast<double*>(dst_j));
+                    cc.push_back(2L * nn * nn);
                     g2.push_back(gj(src_j, nn, 0, w, nn, 0, dst_j, nn, nn, nn, nn, -0.5, 1.5));
                 }
+                copy_batch(cs, cd, cc);
                 gemm(g1);
                 gemm(g2);
             }
@@ -926,12 +937,18 @@ struct wqc_tn_state {
         {
             Timer tm(this, 2);
             std::vector<GemmJob> g;
+            std::vector<const double*> cs;
+            std::vector<double*> cd;
+            std::vector<long> cc;
             for (int k = 0; k < nj; ++k) {
                 double2* tmp = reinterpret_cast<double2*>(b32 + o64[k]);
-                HIPC(hipMemcpyAsync(tmp, jobs[k].X, (size_t)counts[k] * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+                cs.push_back(reinterpret_cast<const double*>(jobs[k].X));
+                cd.push_back(reinterpret_cast<double*>(tmp));
+                cc.push_back(2 * counts[k]);
                 g.push_back(gj(jobs[k].J, jobs[k].n, 0, tmp, jobs[k].m, 0, jobs[k].X, jobs[k].ldx, jobs[k].n,
                                jobs[k].m, jobs[k].n, 1, 0));
             }
+            copy_batch(cs, cd, cc);
             gemm(g);
         }
         jacobi(jobs, d_sigma, sig_off, d_flags);
@@ -948,6 +965,28 @@ struct wqc_tn_state {
     }
 
     int* upload_ints(const std::vector<int>& v) { return upload(v); }
+
+    // Every kept job's row permutation in one upload; fj[q].perm points at perms[q].
+    void set_perms(std::vector<FactorJob>& fj, const std::vector<std::vector<int>>& perms) {
+        std::vector<int> all;
+        std::vector<size_t> at(fj.size());
+        for (size_t q = 0; q < fj.size(); ++q) {
+            at[q] = all.size();
+            all.insert(all.end(), perms[q].begin(), perms[q].end());
+        }
+        int* d = upload(all);
+        for (size_t q = 0; q < fj.size(); ++q) fj[q].perm = d + at[q];
+    }
+
+    // Device-to-device copies of a batch (counts in doubles) as one launch.
+    void copy_batch(const std::vector<const double*>& src, const std::vector<double*>& dst,
+                    const std::vector<long>& counts) {
+        if (src.empty()) return;
+        long mx = 1;
+        for (long c : counts) mx = std::max(mx, c);
+        wqc::launch_copy_batch(upload(src), upload(dst), upload(counts), (int)src.size(), mx, stream);
+        check_launch();
+    }
 
     // Device counters of the Jacobi rounds: [0] inner sweeps, [1] pair eigensolves.
     DevBuf jstats_buf;
@@ -1477,26 +1516,43 @@ struct wqc_tn_state {
                     std::fprintf(stderr, " %d:%dx%d:%d", J[b->ids[i]].p, b->jj[i].n, b->jj[i].m, last_sweeps[t]);
             std::fprintf(stderr, "\n");
         }
+        // sigma of every job and |A|_F^2 of every batch, gathered on the device into one
+        // contiguous block and read back with one copy.
+        size_t total = 0;
+        for (auto* b : batches) {
+            for (size_t i = 0; i < b->ids.size(); ++i) total += b->jj[i].n;
+            total += b->ids.size();
+        }
+        if (total + 1 > hread_cap) fail(WQC_TN_EXEC_FAILED, "readback buffer too small");
+        gather.ensure(std::max<size_t>(total, 1) * sizeof(double));
+        note_peak();
+        double* gd = reinterpret_cast<double*>(gather.p);
+        std::vector<const double*> cs;
+        std::vector<double*> cd;
+        std::vector<long> cc;
         size_t k = 0, cur = 0;
         for (auto* b : batches) {
             b->hoff.assign(b->ids.size(), 0);
             for (size_t i = 0; i < b->ids.size(); ++i, ++k) {
                 J[b->ids[i]].sweeps += last_sweeps[k];
                 J[b->ids[i]].sweeps_lo += last_sweeps_lo[k];
-                if (cur + b->jj[i].n + 1 > hread_cap) fail(WQC_TN_EXEC_FAILED, "readback buffer too small");
                 b->hoff[i] = cur;
-                HIPC(hipMemcpyAsync(hread + cur, PD(0) + b->offs[i], b->jj[i].n * sizeof(double),
-                                    hipMemcpyDeviceToHost, stream));
+                cs.push_back(PD(0) + b->offs[i]);
+                cd.push_back(gd + cur);
+                cc.push_back(b->jj[i].n);
                 cur += b->jj[i].n;
             }
         }
         for (auto* b : batches) {
             if (b->ids.empty()) continue;
-            if (cur + b->ids.size() > hread_cap) fail(WQC_TN_EXEC_FAILED, "readback buffer too small");
             b->hfro = cur;
-            HIPC(hipMemcpyAsync(hread + cur, b->d_fro, b->ids.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+            cs.push_back(b->d_fro);
+            cd.push_back(gd + cur);
+            cc.push_back((long)b->ids.size());
             cur += b->ids.size();
         }
+        copy_batch(cs, cd, cc);
+        if (cur) HIPC(hipMemcpyAsync(hread, gd, cur * sizeof(double), hipMemcpyDeviceToHost, stream));
         sync();
     }
 
@@ -1560,7 +1616,7 @@ struct wqc_tn_state {
             fj[k].J = pre ? b.orig[k] : jj[k].J;
             fj[k].left = sites[j.p].ptr;
             fj[k].right = sites[j.p + 1].ptr;
-            fj[k].perm = upload(perms[k]);
+            fj[k].perm = nullptr;  // set below from the batch's one upload of every perm
             fj[k].sigma = PD(j.o_sig);
             fj[k].rows = j.rows;
             fj[k].cols = j.cols;
@@ -1571,6 +1627,7 @@ struct wqc_tn_state {
             mxl = std::max(mxl, (long)j.rows * (long)c);
             mxr = std::max(mxr, (long)c * j.cols);
         }
+        set_perms(fj, perms);
         {
             Timer tm(this, 4);
             wqc::launch_factor(upload(fj), ne, mxl, mxr, stream);
@@ -1594,6 +1651,7 @@ struct wqc_tn_state {
         const int nr = (int)ids.size();
         std::vector<FactorJob> fj;
         std::vector<int> kept;
+        std::vector<int> kept_k;  // batch index k of each kept job
         std::vector<std::vector<int>> perms(nr);
         long mxl = 1, mxr = 1;
         for (int k = 0; k < nr; ++k) {
@@ -1652,7 +1710,7 @@ struct wqc_tn_state {
             f.J = pre ? b.orig[k] : jj[k].J;
             f.left = P(j.o_W);
             f.right = sites[j.p + 1].ptr;
-            f.perm = upload(perms[k]);
+            f.perm = nullptr;  // set below from the batch's one upload of every perm
             f.sigma = PD(j.o_sigB);
             f.rows = j.rows;
             f.cols = j.cols;
@@ -1663,10 +1721,16 @@ struct wqc_tn_state {
             f.pad0 = f.pad1 = 0;
             fj.push_back(f);
             kept.push_back(ids[k]);
+            kept_k.push_back(k);
             mxl = std::max(mxl, (long)j.ell * (long)c);
             mxr = std::max(mxr, (long)c * j.cols);
         }
         if (fj.empty()) return;
+        {
+            std::vector<std::vector<int>> kp;
+            for (size_t q = 0; q < kept.size(); ++q) kp.push_back(std::move(perms[kept_k[q]]));
+            set_perms(fj, kp);
+        }
         Timer tm(this, 4);
         wqc::launch_factor(upload(fj), (int)fj.size(), mxl, mxr, stream);
         check_launch();

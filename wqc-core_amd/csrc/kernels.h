@@ -172,6 +172,9 @@ void launch_to_f32(const double2* const* src, float2* const* dst, const long* co
                    long max_count, hipStream_t s);
 void launch_to_f64(const float2* const* src, double2* const* dst, const long* counts, int njobs,
                    long max_count, hipStream_t s);
+// dst[j][0 .. counts[j]) = src[j][0 .. counts[j]) in doubles, one launch for the batch.
+void launch_copy_batch(const double* const* src, double* const* dst, const long* counts, int njobs,
+                       long max_count, hipStream_t s);
 void launch_row_norms(const JacobiJob* jobs, int njobs, int max_n, double* sigma,
                       const long* sigma_off, hipStream_t s);
 void launch_gemm(const GemmJob* jobs, int njobs, int max_m, int max_n, hipStream_t s);
