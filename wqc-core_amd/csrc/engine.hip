@@ -29,6 +29,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -983,8 +984,12 @@ struct wqc_tn_state {
                     const std::vector<long>& counts) {
         if (src.empty()) return;
         long mx = 1;
-        for (long c : counts) mx = std::max(mx, c);
-        wqc::launch_copy_batch(upload(src), upload(dst), upload(counts), (int)src.size(), mx, stream);
+        std::vector<wqc::CopyJob> jobs(src.size());
+        for (size_t i = 0; i < src.size(); ++i) {
+            jobs[i] = wqc::CopyJob{src[i], dst[i], counts[i], 0};
+            mx = std::max(mx, counts[i]);
+        }
+        wqc::launch_copy_batch(upload(jobs), (int)jobs.size(), mx, stream);
         check_launch();
     }
 
@@ -1170,32 +1175,82 @@ struct wqc_tn_state {
     // Batched GEMM. Products with few output tiles and a long K (the LQ's panel and block
     // products) are split over K into up to 8 slices, summed in slice order by a second kernel.
     DevBuf splitk_buf;
-    void gemm(std::vector<GemmJob>& g, int family = 2) {
-        if (g.empty()) return;
-        int mm = 1, mn = 1, mk = 1;
+    // One batched product: its jobs sit at off in a job table, with the grid bounds and the
+    // split-K factor. Every split-K call reuses the scratch from its start (the calls run in
+    // stream order, each reduced before the next begins).
+    struct GemmCall {
+        size_t off = 0;
+        int n = 0, mm = 1, mn = 1, splits = 1;
+    };
+    GemmCall size_gemm(const std::vector<GemmJob>& g, size_t& need) const {
+        GemmCall c;
+        int mk = 1;
         for (auto& x : g) {
-            mm = std::max(mm, x.M);
-            mn = std::max(mn, x.N);
+            c.mm = std::max(c.mm, x.M);
+            c.mn = std::max(c.mn, x.N);
             mk = std::max(mk, x.K);
         }
-        const long tiles = (long)((mm + 63) / 64) * ((mn + 63) / 64);
-        const int splits = (use_splitk && tiles <= 16 && mk >= 256) ? std::min(8, mk / 128) : 1;
-        if (splits > 1) {
-            size_t need = 0;
-            for (auto& x : g) need += (size_t)splits * x.M * x.N;
-            splitk_buf.ensure(need * sizeof(double2));
-            note_peak();
-            double2* p = reinterpret_cast<double2*>(splitk_buf.p);
-            for (auto& x : g) {
-                x.part = p;
-                p += (size_t)splits * x.M * x.N;
-            }
-            wqc::launch_gemm_splitk(upload(g), (int)g.size(), mm, mn, splits, stream);
-        } else {
-            wqc::launch_gemm(upload(g), (int)g.size(), mm, mn, stream);
+        c.n = (int)g.size();
+        const long tiles = (long)((c.mm + 63) / 64) * ((c.mn + 63) / 64);
+        c.splits = (use_splitk && tiles <= 16 && mk >= 256) ? std::min(8, mk / 128) : 1;
+        if (c.splits > 1) {
+            size_t nd = 0;
+            for (auto& x : g) nd += (size_t)c.splits * x.M * x.N;
+            need = std::max(need, nd);
         }
+        return c;
+    }
+    void point_parts(std::vector<GemmJob>& g, const GemmCall& c) {
+        if (c.splits <= 1) return;
+        double2* p = reinterpret_cast<double2*>(splitk_buf.p);
+        for (auto& x : g) {
+            x.part = p;
+            p += (size_t)c.splits * x.M * x.N;
+        }
+    }
+    void launch_call(const GemmJob* table, const GemmCall& c, int family) {
+        if (c.n == 0) return;
+        if (c.splits > 1)
+            wqc::launch_gemm_splitk(table + c.off, c.n, c.mm, c.mn, c.splits, stream);
+        else
+            wqc::launch_gemm(table + c.off, c.n, c.mm, c.mn, stream);
         check_launch();
         klaunch[family]++;
+    }
+    void gemm(std::vector<GemmJob>& g, int family = 2) {
+        if (g.empty()) return;
+        size_t need = 0;
+        const GemmCall c = size_gemm(g, need);
+        if (need) {
+            splitk_buf.ensure(need * sizeof(double2));
+            note_peak();
+        }
+        point_parts(g, c);
+        launch_call(upload(g), c, family);
+    }
+    // A fixed sequence of products (and other launches between them, run by `between`):
+    // every call's jobs go up in one table upload instead of one per call.
+    void gemm_sequence(std::vector<std::vector<GemmJob>>& calls, int family,
+                       const std::function<void(size_t)>& between) {
+        size_t need = 0;
+        std::vector<GemmCall> cs(calls.size());
+        for (size_t i = 0; i < calls.size(); ++i) cs[i] = size_gemm(calls[i], need);
+        if (need) {
+            splitk_buf.ensure(need * sizeof(double2));
+            note_peak();
+        }
+        std::vector<GemmJob> all;
+        for (size_t i = 0; i < calls.size(); ++i) {
+            point_parts(calls[i], cs[i]);
+            cs[i].off = all.size();
+            all.insert(all.end(), calls[i].begin(), calls[i].end());
+        }
+        const GemmJob* table = all.empty() ? nullptr : upload(all);
+        for (size_t i = 0; i < calls.size(); ++i) {
+            between(i);
+            launch_call(table, cs[i], family);
+        }
+        between(calls.size());
     }
 
     static GemmJob gj(const double2* A, int lda, int opA, const double2* B, int ldb, int opB, double2* C,
@@ -1408,15 +1463,24 @@ struct wqc_tn_state {
         const int nl = (int)lq.size();
         wqc::LqJob* dl = upload(lq);
         const int NB = wqc::LQ_NB, SB = wqc::LQ_SB;
+        // The whole factorization is a fixed sequence of launches: its products are staged
+        // (one job-table upload) and the panel / T kernels run between them in order.
+        struct Act {
+            int kind, b0, j0, max_cols;  // kind 0: panel at (b0, j0); 1: T of block b0; 2: extract
+        };
+        std::vector<std::vector<GemmJob>> calls;
+        std::vector<std::vector<Act>> pre(1);  // pre[i]: launches before product i
+        auto call = [&](std::vector<GemmJob>& g) {
+            calls.push_back(std::move(g));
+            pre.emplace_back();
+        };
         for (int b0 = 0; b0 < max_n; b0 += SB) {
             // Panels of the block; each updates the block's later rows.
             for (int j0 = b0; j0 < std::min(b0 + SB, max_n); j0 += NB) {
                 int max_cols = 1;
                 for (const auto& q : lq)
                     if (q.n > j0) max_cols = std::max(max_cols, q.m - b0);
-                wqc::launch_lq_panel(dl, nl, b0, j0, max_cols, stream);
-                check_launch();
-                klaunch[6]++;
+                pre.back().push_back(Act{0, b0, j0, max_cols});
                 std::vector<GemmJob> g1, g2;
                 for (int i = 0; i < nl; ++i) {
                     const wqc::LqJob& q = lq[i];
@@ -1428,8 +1492,8 @@ struct wqc_tn_state {
                     g1.push_back(gj(Xr, q.ldx, 0, q.UTc + (size_t)j0 * q.m + j0, q.m, 1, lqw[i], SB, nrest, nbp, K, 1, 0));
                     g2.push_back(gj(lqw[i], SB, 0, q.Ubc + (size_t)j0 * q.m + j0, q.m, 0, Xr, q.ldx, nrest, K, nbp, -1, 1));
                 }
-                gemm(g1, 6);
-                gemm(g2, 6);
+                call(g1);
+                call(g2);
             }
             // The block's T: Gu = U^H U, then the off-diagonal panels of T.
             std::vector<GemmJob> gg, gt, gh, g1, g2;
@@ -1448,18 +1512,27 @@ struct wqc_tn_state {
                 g1.push_back(gj(Xr, q.ldx, 0, q.UTc + (size_t)b0 * q.m + b0, q.m, 1, lqw[i], SB, nrest, nbs, K, 1, 0));
                 g2.push_back(gj(lqw[i], SB, 0, Ub, q.m, 0, Xr, q.ldx, nrest, K, nbs, -1, 1));
             }
-            gemm(gg, 6);
-            wqc::launch_lq_tblock(dl, nl, b0, stream);
-            check_launch();
-            klaunch[6]++;
-            gemm(gt, 6);
-            gemm(gh, 6);
+            call(gg);
+            pre.back().push_back(Act{1, b0, 0, 0});
+            call(gt);
+            call(gh);
             // Rows below the block.
-            gemm(g1, 6);
-            gemm(g2, 6);
+            call(g1);
+            call(g2);
         }
-        wqc::launch_lq_extract(dl, nl, max_n, stream);
-        check_launch();
+        pre.back().push_back(Act{2, 0, 0, 0});
+        gemm_sequence(calls, 6, [&](size_t i) {
+            for (const Act& a : pre[i]) {
+                if (a.kind == 0)
+                    wqc::launch_lq_panel(dl, nl, a.b0, a.j0, a.max_cols, stream);
+                else if (a.kind == 1)
+                    wqc::launch_lq_tblock(dl, nl, a.b0, stream);
+                else
+                    wqc::launch_lq_extract(dl, nl, max_n, stream);
+                check_launch();
+                if (a.kind != 2) klaunch[6]++;
+            }
+        });
     }
 
     // After the Jacobi of LQ-preconditioned jobs: X <- M = [J 0] Q_p^H ... Q_1^H.
@@ -1477,6 +1550,7 @@ struct wqc_tn_state {
         wqc::launch_lq_init_m(dl, nl, mx, stream);
         check_launch();
         const int SB = wqc::LQ_SB;
+        std::vector<std::vector<GemmJob>> calls;
         for (int b0 = ((max_n - 1) / SB) * SB; b0 >= 0; b0 -= SB) {
             std::vector<GemmJob> g1, g2;
             for (int i = 0; i < nl; ++i) {
@@ -1487,9 +1561,10 @@ struct wqc_tn_state {
                 g1.push_back(gj(q.X + b0, q.ldx, 0, q.UThc + (size_t)b0 * q.m + b0, q.m, 1, lqw[i], SB, q.n, nbs, K, 1, 0));
                 g2.push_back(gj(lqw[i], SB, 0, q.Ubc + (size_t)b0 * q.m + b0, q.m, 0, q.X + b0, q.ldx, q.n, K, nbs, -1, 1));
             }
-            gemm(g1, 6);
-            gemm(g2, 6);
+            calls.push_back(std::move(g1));
+            calls.push_back(std::move(g2));
         }
+        gemm_sequence(calls, 6, [](size_t) {});
     }
 
     // One Jacobi over the jobs of all batches (range-finder B and exact theta' of a layer

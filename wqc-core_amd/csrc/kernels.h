@@ -11,6 +11,14 @@
 namespace wqc {
 
 // One-qubit update of one site (l, 2, r) in place: site[a,s',b] = sum_s U[s'][s] site[a,s,b].
+// One device-to-device copy of n doubles.
+struct CopyJob {
+    const double* src;
+    double* dst;
+    long n;
+    long pad;
+};
+
 struct OneQubitJob {
     double2* site;
     int l, r;
@@ -172,9 +180,8 @@ void launch_to_f32(const double2* const* src, float2* const* dst, const long* co
                    long max_count, hipStream_t s);
 void launch_to_f64(const float2* const* src, double2* const* dst, const long* counts, int njobs,
                    long max_count, hipStream_t s);
-// dst[j][0 .. counts[j]) = src[j][0 .. counts[j]) in doubles, one launch for the batch.
-void launch_copy_batch(const double* const* src, double* const* dst, const long* counts, int njobs,
-                       long max_count, hipStream_t s);
+// jobs[j].dst[0 .. n) = jobs[j].src[0 .. n) in doubles, one launch for the batch.
+void launch_copy_batch(const CopyJob* jobs, int njobs, long max_count, hipStream_t s);
 void launch_row_norms(const JacobiJob* jobs, int njobs, int max_n, double* sigma,
                       const long* sigma_off, hipStream_t s);
 void launch_gemm(const GemmJob* jobs, int njobs, int max_m, int max_n, hipStream_t s);

@@ -1087,16 +1087,11 @@ __global__ __launch_bounds__(256) void k_to_f64(const float2* const* __restrict_
     }
 }
 
-// dst[job][e] = src[job][e], e < counts[job] (doubles): the per-job device copies of a batch
+// jobs[j].dst[e] = jobs[j].src[e], e < jobs[j].n (doubles): the per-job device copies of a batch
 // (saved rows, J copies, sigma gathers) as one launch instead of one copy per job.
-__global__ __launch_bounds__(256) void k_copy_batch(const double* const* __restrict__ src,
-                                                    double* const* __restrict__ dst,
-                                                    const long* __restrict__ counts) {
-    const int job = blockIdx.y;
-    const double* s = src[job];
-    double* d = dst[job];
-    const long n = counts[job];
-    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) d[e] = s[e];
+__global__ __launch_bounds__(256) void k_copy_batch(const CopyJob* __restrict__ jobs) {
+    const CopyJob jb = jobs[blockIdx.y];
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < jb.n; e += (long)gridDim.x * 256) jb.dst[e] = jb.src[e];
 }
 
 // sigma[row] = ||X[row, :]||, one wave per row.
@@ -1987,11 +1982,9 @@ void launch_to_f64(const float2* const* src, double2* const* dst, const long* co
                        dst, counts);
 }
 
-void launch_copy_batch(const double* const* src, double* const* dst, const long* counts, int njobs,
-                       long max_count, hipStream_t s) {
+void launch_copy_batch(const CopyJob* jobs, int njobs, long max_count, hipStream_t s) {
     if (njobs <= 0) return;
-    hipLaunchKernelGGL(k_copy_batch, dim3(blocks_for(max_count, 256, 2048), njobs), dim3(256), 0, s, src,
-                       dst, counts);
+    hipLaunchKernelGGL(k_copy_batch, dim3(blocks_for(max_count, 256, 2048), njobs), dim3(256), 0, s, jobs);
 }
 
 void launch_row_norms(const JacobiJob* jobs, int njobs, int max_n, double* sigma,
