@@ -325,7 +325,7 @@ __device__ __forceinline__ d4v mfma_f64(double a, double b, d4v c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-__global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacobiJob* __restrict__ jobs,
+__global__ __launch_bounds__(256, 3) void k_jacobi_round(const JacobiJob* __restrict__ jobs,
                                                       int round, double tol_base, double nu2_base,
                                                       int inner_cap, int* __restrict__ rotated,
                                                       const int* __restrict__ done,
@@ -352,8 +352,9 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacobiJob* __rest
     }
 
     __shared__ double2 G[JP][JP + 1];
-    __shared__ double2 T[JP][JP + 1];
-    // Partial Gram tiles of two waves; afterwards the second G and T buffers of the eigensolve.
+    // Partial Gram tiles of two waves; afterwards T (the accumulated rotation, updated in
+    // place: a step's tiles are disjoint) and the second G buffer of the eigensolve. About
+    // 51 KB of LDS in all, so three workgroups share a CU.
     __shared__ __align__(16) double s_aux[4 * JP * (JP + 1)];
     __shared__ int s_row[JP];
     __shared__ int s_any;
@@ -361,8 +362,8 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacobiJob* __rest
     __shared__ int s_dst[JP];
     __shared__ unsigned long long s_off;
     double (*Gp)[6][4][64] = reinterpret_cast<double (*)[6][4][64]>(s_aux);
-    double2 (*G2)[JP + 1] = reinterpret_cast<double2 (*)[JP + 1]>(s_aux);
-    double2 (*T2)[JP + 1] = reinterpret_cast<double2 (*)[JP + 1]>(s_aux + 2 * JP * (JP + 1));
+    double2 (*T)[JP + 1] = reinterpret_cast<double2 (*)[JP + 1]>(s_aux);
+    double2 (*G2)[JP + 1] = reinterpret_cast<double2 (*)[JP + 1]>(s_aux + 2 * JP * (JP + 1));
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid < JP) {
@@ -506,12 +507,11 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacobiJob* __rest
 
     // ---- 2. eigensolve of G (T accumulates the row rotations). Every thread derives the
     // parameters of its own two rotations from the current buffers, so a step is one
-    // read-update-write pass with G and T ping-ponged between two LDS buffers and a single
-    // barrier.
+    // read-update-write pass with G ping-ponged between two LDS buffers and a single barrier;
+    // T is only read and written at the thread's own tile, so it is updated in place.
     double2 (*Gc)[JP + 1] = G;
-    double2 (*Tc)[JP + 1] = T;
+    double2 (*const Tc)[JP + 1] = T;
     double2 (*Gn)[JP + 1] = G2;
-    double2 (*Tn)[JP + 1] = T2;
     int sweeps_done = 0;
     const int bk = tid >> 4, bk2 = tid & 15;
     for (int sweep = 0; sweep < inner_cap; ++sweep) {
@@ -577,17 +577,14 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacobiJob* __rest
             Gn[p1][q2] = h01;
             Gn[q1][p2] = h10;
             Gn[q1][q2] = h11;
-            Tn[p1][p2] = t00;
-            Tn[p1][q2] = t01;
-            Tn[q1][p2] = t10;
-            Tn[q1][q2] = t11;
+            Tc[p1][p2] = t00;
+            Tc[p1][q2] = t01;
+            Tc[q1][p2] = t10;
+            Tc[q1][q2] = t11;
             __syncthreads();
             double2 (*sw)[JP + 1] = Gc;
             Gc = Gn;
             Gn = sw;
-            sw = Tc;
-            Tc = Tn;
-            Tn = sw;
         }
         // Every wave reads the flag before thread 0 clears it for the next sweep.
         const int again = s_any;
@@ -611,20 +608,11 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacobiJob* __rest
     __syncthreads();
     double2 (*const Tp)[JP + 1] = Gn;
 
-    // ---- 3. P <- T P on MFMA. Lane l supplies T[I*16 + l%16][4kk + l/16] (registers) and
-    // P[4kk + l/16][c0 + l%16]; output element j of lane l is local row I*16 + 4j + l/16.
+    // ---- 3. P <- T P on MFMA. Lane l supplies T[I*16 + l%16][4kk + l/16] (read from LDS per
+    // column chunk, which keeps the kernel within the registers of three workgroups per CU)
+    // and P[4kk + l/16][c0 + l%16]; output element j of lane l is local row I*16 + 4j + l/16.
     {
         const int i = lane & 15, g = lane >> 4;
-        double tr[2][8], ti[2][8], ts[2][8];
-#pragma unroll
-        for (int I = 0; I < 2; ++I)
-#pragma unroll
-            for (int kk = 0; kk < 8; ++kk) {
-                const double2 t = Tp[I * 16 + i][4 * kk + g];
-                tr[I][kk] = t.x;
-                ti[I][kk] = t.y;
-                ts[I][kk] = t.x + t.y;
-            }
         int prow[8];
 #pragma unroll
         for (int kk = 0; kk < 8; ++kk) prow[kk] = s_row[4 * kk + g];
@@ -655,9 +643,10 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacobiJob* __rest
         _Pragma("unroll") for (int I = 0; I < 2; ++I) {                              \
             d4v e1 = {0, 0, 0, 0}, e2 = e1, e3 = e1;                                 \
             _Pragma("unroll") for (int kk = 0; kk < 8; ++kk) {                       \
-                e1 = mfma_f64(tr[I][kk], B[kk].x, e1);                               \
-                e2 = mfma_f64(ti[I][kk], B[kk].y, e2);                               \
-                e3 = mfma_f64(ts[I][kk], bs[kk], e3);                                \
+                const double2 t = Tp[I * 16 + i][4 * kk + g];                        \
+                e1 = mfma_f64(t.x, B[kk].x, e1);                                     \
+                e2 = mfma_f64(t.y, B[kk].y, e2);                                     \
+                e3 = mfma_f64(t.x + t.y, bs[kk], e3);                                \
             }                                                                        \
             if (c < cols) {                                                          \
                 _Pragma("unroll") for (int j = 0; j < 4; ++j) {                      \
