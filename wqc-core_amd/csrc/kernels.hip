@@ -689,7 +689,7 @@ __device__ __forceinline__ f4v mfma_f32(float a, float b, f4v c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-__global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __restrict__ jobs,
+__global__ __launch_bounds__(256, 3) void k_jacobi_round_lo(const JacobiLoJob* __restrict__ jobs,
                                                          int round, float tol_base, double nu2_base,
                                                          float rel_floor, int inner_cap,
                                                          int* __restrict__ rotated,
@@ -977,19 +977,10 @@ __global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __re
 
     // X32 <- T X32 and J32 <- T J32 on FP32 MFMA with 3M products (E1 = Tr Pr, E2 = Ti Pi,
     // E3 = (Tr + Ti)(Pr + Pi); Re = E1 - E2, Im = E3 - E1 - E2). J stays unitary to FP32
-    // precision; the host makes it unitary in FP64 once, after the last low round.
+    // precision; the host makes it unitary in FP64 once, after the last low round. T is read
+    // from LDS per column chunk (three workgroups per CU fit in the registers).
     {
         const int i = lane & 15, g = lane >> 4;
-        float tr[2][8], ti[2][8], ts[2][8];
-#pragma unroll
-        for (int I = 0; I < 2; ++I)
-#pragma unroll
-            for (int kk = 0; kk < 8; ++kk) {
-                const float2 t = Tp[I * 16 + i][4 * kk + g];
-                tr[I][kk] = t.x;
-                ti[I][kk] = t.y;
-                ts[I][kk] = t.x + t.y;
-            }
         int prow[8];
 #pragma unroll
         for (int kk = 0; kk < 8; ++kk) prow[kk] = s_row[4 * kk + g];
@@ -1017,9 +1008,10 @@ __global__ __launch_bounds__(256) void k_jacobi_round_lo(const JacobiLoJob* __re
         _Pragma("unroll") for (int I = 0; I < 2; ++I) {                                            \
             f4v e1 = {0, 0, 0, 0}, e2 = e1, e3 = e1;                                               \
             _Pragma("unroll") for (int kk = 0; kk < 8; ++kk) {                                     \
-                e1 = mfma_f32(tr[I][kk], B[kk].x, e1);                                             \
-                e2 = mfma_f32(ti[I][kk], B[kk].y, e2);                                             \
-                e3 = mfma_f32(ts[I][kk], bs[kk], e3);                                              \
+                const float2 t = Tp[I * 16 + i][4 * kk + g];                                       \
+                e1 = mfma_f32(t.x, B[kk].x, e1);                                                   \
+                e2 = mfma_f32(t.y, B[kk].y, e2);                                                   \
+                e3 = mfma_f32(t.x + t.y, bs[kk], e3);                                              \
             }                                                                                      \
             if (c < cols) {                                                                        \
                 _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                    \
