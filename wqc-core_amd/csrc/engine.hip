@@ -515,6 +515,7 @@ struct wqc_tn_state {
     double stall_off = STALL_OFF;
     bool use_lq = true;
     bool use_splitk = true;  // WQC_TN_SPLITK=0 turns split-K GEMMs off
+    bool use_wide = true;    // WQC_TN_JACOBI_WIDE=0 keeps every FP64 round at 256 threads
     DevBuf trace_buf;
 
     void init(wqc_tn_device* d, size_t qn, size_t mb, int prec) {
@@ -532,6 +533,7 @@ struct wqc_tn_state {
         use_lq = prec == WQC_TN_PRECISION_MIXED_FP32;
         if (const char* e = std::getenv("WQC_TN_LQ")) use_lq = std::atoi(e) != 0;
         if (const char* e = std::getenv("WQC_TN_SPLITK")) use_splitk = std::atoi(e) != 0;
+        if (const char* e = std::getenv("WQC_TN_JACOBI_WIDE")) use_wide = std::atoi(e) != 0;
         if (const char* e = std::getenv("WQC_TN_JACOBI_INNER_LO")) {
             const int v = std::atoi(e);
             if (v >= 1 && v <= 32) inner_cap_lo = v;
@@ -710,14 +712,27 @@ struct wqc_tn_state {
         int* d_rot = reinterpret_cast<int*>(d_off + nj);
         std::vector<double> prev_off(nj, 0.0);
         std::vector<std::vector<double>> hist(trace_jacobi ? nj : 0);
+        // Block pairs per job: a sweep whose unfinished jobs have no more pairs than the device
+        // has CUs runs its rounds on the 512-thread form (one workgroup per CU either way, and
+        // the round lasts as long as one workgroup).
+        std::vector<int> jpairs(nj);
+        for (int j = 0; j < nj; ++j) {
+            const int b = (jobs[j].n + 15) / 16;
+            jpairs[j] = (b + (b & 1)) / 2;
+        }
+        const int cus = dev->prop.multiProcessorCount;
         for (int sweep = 0; sweep < MAX_SWEEPS && active > 0; ++sweep) {
             check_cancel();
             wqc::launch_fill_int(reinterpret_cast<int*>(d_off), 0, 3 * nj, stream);
+            int wgs = 0;
+            for (int j = 0; j < nj; ++j)
+                if (!done[j]) wgs += jpairs[j];
+            const bool wide = use_wide && wgs <= cus;
             {
                 Timer tm(this, 3);
                 for (int r = 0; r < rounds; ++r)
                     wqc::launch_jacobi_round(dj, nj, pairs, r, tol, nu2, inner_cap, d_rot, d_done, d_jstats,
-                                             d_off, stream);
+                                             d_off, wide, stream);
                 check_launch();
                 klaunch[3] += rounds;
             }

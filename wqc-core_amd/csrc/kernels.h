@@ -170,9 +170,10 @@ void launch_dust(double2* const* ptrs, const long* counts, const double* fro2, d
                  int njobs, long max_count, hipStream_t s);
 void launch_identity(double2* const* ptrs, const int* dims, int njobs, int max_dim,
                      hipStream_t s);
+// wide: the 512-thread form, for rounds with at most one block pair per CU.
 void launch_jacobi_round(const JacobiJob* jobs, int njobs, int max_pairs, int round,
                          double tol, double nu2, int inner_cap, int* rotated, const int* done,
-                         unsigned long long* stats, unsigned long long* offmax, hipStream_t s);
+                         unsigned long long* stats, unsigned long long* offmax, bool wide, hipStream_t s);
 void launch_jacobi_round_lo(const JacobiLoJob* jobs, int njobs, int max_pairs, int round,
                             float tol, double nu2, float rel_floor, int inner_cap, int* rotated, const int* done,
                             unsigned long long* stats, unsigned long long* offmax, hipStream_t s);

@@ -325,12 +325,18 @@ __device__ __forceinline__ d4v mfma_f64(double a, double b, d4v c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-__global__ __launch_bounds__(256, 3) void k_jacobi_round(const JacobiJob* __restrict__ jobs,
+// NT = 256: four waves, three workgroups per CU (the bulk of a batch's launches). NT = 512:
+// eight waves share the Gram and the rotation of one block pair, for launches with no more
+// block pairs than CUs, where a round lasts as long as one workgroup; the eigensolve stays on
+// the first four waves.
+template <int NT>
+__global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round(const JacobiJob* __restrict__ jobs,
                                                       int round, double tol_base, double nu2_base,
                                                       int inner_cap, int* __restrict__ rotated,
                                                       const int* __restrict__ done,
                                                       unsigned long long* __restrict__ stats,
                                                       unsigned long long* __restrict__ offmax) {
+    constexpr int NW = NT / 64;
     const int job = blockIdx.y;
     if (done[job]) return;
     const JacobiJob jb = jobs[job];
@@ -355,7 +361,7 @@ __global__ __launch_bounds__(256, 3) void k_jacobi_round(const JacobiJob* __rest
     // Partial Gram tiles of two waves; afterwards T (the accumulated rotation, updated in
     // place: a step's tiles are disjoint) and the second G buffer of the eigensolve. About
     // 51 KB of LDS in all, so three workgroups share a CU.
-    __shared__ __align__(16) double s_aux[4 * JP * (JP + 1)];
+    __shared__ __align__(16) double s_aux[NW == 8 ? 4 * 6 * 4 * 64 : 4 * JP * (JP + 1)];
     __shared__ int s_row[JP];
     __shared__ int s_any;
     __shared__ int s_intra;
@@ -412,11 +418,11 @@ __global__ __launch_bounds__(256, 3) void k_jacobi_round(const JacobiJob* __rest
         int c0 = wave * 16;
         if (c0 < jb.m) WQC_GRAM_LOAD(c0, ua, va)
         while (c0 < jb.m) {
-            const int c1 = c0 + 64;
+            const int c1 = c0 + 16 * NW;
             if (c1 < jb.m) WQC_GRAM_LOAD(c1, ub, vb)
             WQC_GRAM_ACC(ua, va)
             if (c1 >= jb.m) break;
-            const int c2 = c1 + 64;
+            const int c2 = c1 + 16 * NW;
             if (c2 < jb.m) WQC_GRAM_LOAD(c2, ua, va)
             WQC_GRAM_ACC(ub, vb)
             c0 = c2;
@@ -430,7 +436,20 @@ __global__ __launch_bounds__(256, 3) void k_jacobi_round(const JacobiJob* __rest
 #define WQC_GP_ALL(OP) OP(0, a00r) OP(1, a00i) OP(2, a01r) OP(3, a01i) OP(4, a11r) OP(5, a11i)
 #define WQC_GP_STORE(t, v) for (int j = 0; j < 4; ++j) Gp[slot][t][j][lane] = v[j];
 #define WQC_GP_ADD(t, v) for (int j = 0; j < 4; ++j) v[j] += Gp[slot][t][j][lane];
-        if (wave >= 2) {
+        if constexpr (NW == 8) {
+            // (w0 + w4), (w1 + w5), (w2 + w6), (w3 + w7) first.
+            if (wave >= 4) {
+                const int slot = wave - 4;
+                WQC_GP_ALL(WQC_GP_STORE)
+            }
+            __syncthreads();
+            if (wave < 4) {
+                const int slot = wave;
+                WQC_GP_ALL(WQC_GP_ADD)
+            }
+            __syncthreads();
+        }
+        if (wave >= 2 && wave < 4) {
             const int slot = wave - 2;
             WQC_GP_ALL(WQC_GP_STORE)
         }
@@ -468,7 +487,7 @@ __global__ __launch_bounds__(256, 3) void k_jacobi_round(const JacobiJob* __rest
     {
         bool off = false, intra = false;
         double worst = 0.0;
-        for (int e = tid; e < JP * JP; e += 256) {
+        for (int e = tid; e < JP * JP; e += NT) {
             const int i = e / JP, j = e % JP;
             if (i < j) {
                 const double2 c = G[i][j];
@@ -499,7 +518,7 @@ __global__ __launch_bounds__(256, 3) void k_jacobi_round(const JacobiJob* __rest
         rotated[job] = 1;
         s_any = 0;
     }
-    for (int e = tid; e < JP * JP; e += 256) {
+    for (int e = tid; e < JP * JP; e += NT) {
         const int i = e / JP, j = e % JP;
         T[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
     }
@@ -522,65 +541,67 @@ __global__ __launch_bounds__(256, 3) void k_jacobi_round(const JacobiJob* __rest
         const bool cross = sweep == 0 && !intra0;
         const int nsteps = cross ? JB : JP - 1;
         for (int step = 0; step < nsteps; ++step) {
-            int p1, q1, p2, q2;
-            if (cross) {
-                cross_pair(step, bk, p1, q1);
-                cross_pair(step, bk2, p2, q2);
-            } else {
-                circle_pair(step, bk, p1, q1);
-                circle_pair(step, bk2, p2, q2);
+            if (tid < 256) {  // the eigensolve's 16 x 16 tiles of 2 x 2
+                int p1, q1, p2, q2;
+                if (cross) {
+                    cross_pair(step, bk, p1, q1);
+                    cross_pair(step, bk2, p2, q2);
+                } else {
+                    circle_pair(step, bk, p1, q1);
+                    circle_pair(step, bk2, p2, q2);
+                }
+                const Rot64 r2 = rot_params64(Gc[p2][p2].x, Gc[q2][q2].x, Gc[p2][q2], tol, nu2);
+                const Rot64 r1 = rot_from_lane(r2, bk & 15);
+                double2 h00 = Gc[p1][p2], h01 = Gc[p1][q2], h10 = Gc[q1][p2], h11 = Gc[q1][q2];
+                double2 t00 = Tc[p1][p2], t01 = Tc[p1][q2], t10 = Tc[q1][p2], t11 = Tc[q1][q2];
+                if (r1.act) {
+                    const double cs = r1.cs;
+                    const double2 m01 = r1.m01, m10 = r1.m10;
+                    const double2 x00 = cadd(cscale(h00, cs), cmul(m01, h10));
+                    const double2 x01 = cadd(cscale(h01, cs), cmul(m01, h11));
+                    const double2 x10 = cadd(cmul(m10, h00), cscale(h10, cs));
+                    const double2 x11 = cadd(cmul(m10, h01), cscale(h11, cs));
+                    h00 = x00;
+                    h01 = x01;
+                    h10 = x10;
+                    h11 = x11;
+                    const double2 y00 = cadd(cscale(t00, cs), cmul(m01, t10));
+                    const double2 y01 = cadd(cscale(t01, cs), cmul(m01, t11));
+                    const double2 y10 = cadd(cmul(m10, t00), cscale(t10, cs));
+                    const double2 y11 = cadd(cmul(m10, t01), cscale(t11, cs));
+                    t00 = y00;
+                    t01 = y01;
+                    t10 = y10;
+                    t11 = y11;
+                }
+                if (r2.act) {
+                    const double cs = r2.cs;
+                    const double2 c01 = cconj(r2.m01), c10 = cconj(r2.m10);
+                    const double2 n00 = cadd(cscale(h00, cs), cmul(h01, c01));
+                    const double2 n01 = cadd(cmul(h00, c10), cscale(h01, cs));
+                    const double2 n10 = cadd(cscale(h10, cs), cmul(h11, c01));
+                    const double2 n11 = cadd(cmul(h10, c10), cscale(h11, cs));
+                    h00 = n00;
+                    h01 = n01;
+                    h10 = n10;
+                    h11 = n11;
+                }
+                if (bk == bk2 && r1.act) {
+                    h00 = make_double2(Gc[p1][p1].x - r1.tg, 0.0);
+                    h11 = make_double2(Gc[q1][q1].x + r1.tg, 0.0);
+                    h01 = cz();
+                    h10 = cz();
+                    s_any = 1;
+                }
+                Gn[p1][p2] = h00;
+                Gn[p1][q2] = h01;
+                Gn[q1][p2] = h10;
+                Gn[q1][q2] = h11;
+                Tc[p1][p2] = t00;
+                Tc[p1][q2] = t01;
+                Tc[q1][p2] = t10;
+                Tc[q1][q2] = t11;
             }
-            const Rot64 r2 = rot_params64(Gc[p2][p2].x, Gc[q2][q2].x, Gc[p2][q2], tol, nu2);
-            const Rot64 r1 = rot_from_lane(r2, bk & 15);
-            double2 h00 = Gc[p1][p2], h01 = Gc[p1][q2], h10 = Gc[q1][p2], h11 = Gc[q1][q2];
-            double2 t00 = Tc[p1][p2], t01 = Tc[p1][q2], t10 = Tc[q1][p2], t11 = Tc[q1][q2];
-            if (r1.act) {
-                const double cs = r1.cs;
-                const double2 m01 = r1.m01, m10 = r1.m10;
-                const double2 x00 = cadd(cscale(h00, cs), cmul(m01, h10));
-                const double2 x01 = cadd(cscale(h01, cs), cmul(m01, h11));
-                const double2 x10 = cadd(cmul(m10, h00), cscale(h10, cs));
-                const double2 x11 = cadd(cmul(m10, h01), cscale(h11, cs));
-                h00 = x00;
-                h01 = x01;
-                h10 = x10;
-                h11 = x11;
-                const double2 y00 = cadd(cscale(t00, cs), cmul(m01, t10));
-                const double2 y01 = cadd(cscale(t01, cs), cmul(m01, t11));
-                const double2 y10 = cadd(cmul(m10, t00), cscale(t10, cs));
-                const double2 y11 = cadd(cmul(m10, t01), cscale(t11, cs));
-                t00 = y00;
-                t01 = y01;
-                t10 = y10;
-                t11 = y11;
-            }
-            if (r2.act) {
-                const double cs = r2.cs;
-                const double2 c01 = cconj(r2.m01), c10 = cconj(r2.m10);
-                const double2 n00 = cadd(cscale(h00, cs), cmul(h01, c01));
-                const double2 n01 = cadd(cmul(h00, c10), cscale(h01, cs));
-                const double2 n10 = cadd(cscale(h10, cs), cmul(h11, c01));
-                const double2 n11 = cadd(cmul(h10, c10), cscale(h11, cs));
-                h00 = n00;
-                h01 = n01;
-                h10 = n10;
-                h11 = n11;
-            }
-            if (bk == bk2 && r1.act) {
-                h00 = make_double2(Gc[p1][p1].x - r1.tg, 0.0);
-                h11 = make_double2(Gc[q1][q1].x + r1.tg, 0.0);
-                h01 = cz();
-                h10 = cz();
-                s_any = 1;
-            }
-            Gn[p1][p2] = h00;
-            Gn[p1][q2] = h01;
-            Gn[q1][p2] = h10;
-            Gn[q1][q2] = h11;
-            Tc[p1][p2] = t00;
-            Tc[p1][q2] = t01;
-            Tc[q1][p2] = t10;
-            Tc[q1][q2] = t11;
             __syncthreads();
             double2 (*sw)[JP + 1] = Gc;
             Gc = Gn;
@@ -601,7 +622,7 @@ __global__ __launch_bounds__(256, 3) void k_jacobi_round(const JacobiJob* __rest
     // Ordered rotation: row k of T moves to row dst[k], into the free G buffer.
     pair_order(tid, s_row, &Gc[0][0].x, 2 * (JP + 2), s_dst);
     __syncthreads();
-    for (int e = tid; e < JP * JP; e += 256) {
+    for (int e = tid; e < JP * JP; e += NT) {
         const int i = e / JP, j = e % JP;
         Gn[s_dst[i]][j] = Tc[i][j];
     }
@@ -660,11 +681,11 @@ __global__ __launch_bounds__(256, 3) void k_jacobi_round(const JacobiJob* __rest
             int c0 = wave * 16;
             if (c0 < cols) WQC_APPLY_LOAD(c0, ba)
             while (c0 < cols) {
-                const int c1 = c0 + 64;
+                const int c1 = c0 + 16 * NW;
                 if (c1 < cols) WQC_APPLY_LOAD(c1, bb)
                 WQC_APPLY_ROT(c0, ba)
                 if (c1 >= cols) break;
-                const int c2 = c1 + 64;
+                const int c2 = c1 + 16 * NW;
                 if (c2 < cols) WQC_APPLY_LOAD(c2, ba)
                 WQC_APPLY_ROT(c1, bb)
                 c0 = c2;
@@ -1940,9 +1961,13 @@ void launch_identity(double2* const* ptrs, const int* dims, int njobs, int max_d
 
 void launch_jacobi_round(const JacobiJob* jobs, int njobs, int max_pairs, int round, double tol,
                          double nu2, int inner_cap, int* rotated, const int* done, unsigned long long* stats,
-                         unsigned long long* offmax, hipStream_t s) {
-    hipLaunchKernelGGL(k_jacobi_round, dim3(max_pairs, njobs), dim3(256), 0, s, jobs, round, tol,
-                       nu2, inner_cap, rotated, done, stats, offmax);
+                         unsigned long long* offmax, bool wide, hipStream_t s) {
+    if (wide)
+        hipLaunchKernelGGL(k_jacobi_round<512>, dim3(max_pairs, njobs), dim3(512), 0, s, jobs, round, tol,
+                           nu2, inner_cap, rotated, done, stats, offmax);
+    else
+        hipLaunchKernelGGL(k_jacobi_round<256>, dim3(max_pairs, njobs), dim3(256), 0, s, jobs, round, tol,
+                           nu2, inner_cap, rotated, done, stats, offmax);
 }
 
 void launch_jacobi_round_lo(const JacobiLoJob* jobs, int njobs, int max_pairs, int round, float tol,
