@@ -265,9 +265,15 @@ void resolve_u4(const Mat4 u, bool ctrl_is_first, double2* u4) {
 }
 
 // ---------------------------------------------------------------- device buffers
+// Grow-only device buffer; freed by release() or when its owner goes away (the owner makes
+// its device current first). Not copyable: one owner per allocation.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
     void ensure(size_t want) {
         if (want <= bytes) return;
         if (p) HIPC(hipFree(p));
