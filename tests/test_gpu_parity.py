@@ -521,3 +521,43 @@ def test_lq_preconditioned_split_vs_oracle(wt, dl, dr, bond, chi, lq_env):
     assert np.allclose(ln, rn, rtol=1e-10)
     assert np.all(np.diff(ln) <= 1e-12 * ln[0])
     g.free()
+
+
+def test_wide_and_bulk_fp64_rounds_agree(wt):
+    """The FP64 Jacobi round has a 512-thread form for sweeps with at most one block pair per
+    CU and a 256-thread form for the rest; they differ only in the order the Gram partials are
+    summed. A layer of three splits (among them a truncating 128 x 128 theta over a bond of
+    200), run with the 512-thread form on (what such a small batch uses) and off
+    (WQC_TN_JACOBI_WIDE=0), must give identical bonds and the same state to 1e-12, and both
+    must match the oracle's state to the parity bar."""
+    import os
+
+    rng = np.random.default_rng(5150)
+    n, chi = 8, 100
+    sites = random_mps_sites(rng, [1, 2, 4, 64, 200, 64, 4, 2, 1], 0.02)
+    layer = [C.two(C.CZ, 0, 1), C.two(C.CNOT, 3, 4), C.two(C.CZ, 5, 6)]
+    states = []
+    for wide in ("1", "0"):
+        os.environ["WQC_TN_JACOBI_WIDE"] = wide
+        try:
+            g = wt.MpsState(n, chi)
+        finally:
+            os.environ.pop("WQC_TN_JACOBI_WIDE", None)
+        for i, s in enumerate(sites):
+            g.set_site(i, s)
+        g.apply_adjacent_disjoint_gates(layer)
+        states.append((g.bond_dims(), g.contract_to_statevector()))
+        g.free()
+    o = om.OracleMps(n, chi)
+    for i, s in enumerate(sites):
+        o.set_site(i, s)
+    o.apply_adjacent_disjoint_gates(layer)
+    want = o.contract_to_statevector()
+    (bw, vw), (bb, vb) = states
+    assert bw == bb == o.bond_dims()
+    scale = np.max(np.abs(want))
+    d_forms = np.max(np.abs(vw - vb)) / scale
+    d_oracle = max(np.max(np.abs(vw - want)), np.max(np.abs(vb - want))) / scale
+    say("wide vs bulk rounds: bonds", bw, "forms differ by %.1e, oracle by %.1e" % (d_forms, d_oracle))
+    assert d_forms <= 1e-12
+    assert d_oracle <= TOL
