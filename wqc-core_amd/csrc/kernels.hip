@@ -1574,7 +1574,7 @@ __global__ __launch_bounds__(256) void k_lq_init_m(const LqJob* __restrict__ job
 // ------------------------------------------------------------------ Householder QR
 // One workgroup per job; see HouseJob. Reflector convention v = x + (x0/|x0|) ||x|| e1,
 // normalized (the reflection axis nalgebra computes), H = I - 2 v v^H.
-__global__ __launch_bounds__(256) void k_householder(const HouseJob* __restrict__ jobs) {
+__global__ __launch_bounds__(1024) void k_householder(const HouseJob* __restrict__ jobs) {
     const HouseJob jb = jobs[blockIdx.x];
     const int rows = jb.rows, r = jb.r;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1587,8 +1587,11 @@ __global__ __launch_bounds__(256) void k_householder(const HouseJob* __restrict_
     __shared__ int s_skip;
     double2* Y = jb.Y;
     double2* V = jb.V;
+    // Sixteen waves; every per-column loop gives a column to one wave, and the one block sum
+    // (the reflector norm) runs on the first four waves, so the arithmetic of each result
+    // does not depend on the number of waves.
     // Original column norms.
-    for (int k = wave; k < r; k += 4) {
+    for (int k = wave; k < r; k += 16) {
         double acc = 0.0;
         for (int i = lane; i < rows; i += 64) {
             const double2 v = Y[(long)i * r + k];
@@ -1597,17 +1600,18 @@ __global__ __launch_bounds__(256) void k_householder(const HouseJob* __restrict_
         acc = wave_sum(acc);
         if (lane == 0) s_cn[k] = sqrt(acc);
     }
-    for (long e = tid; e < (long)rows * r; e += 256) V[e] = cz();
+    for (long e = tid; e < (long)rows * r; e += 1024) V[e] = cz();
     __syncthreads();
     for (int k = 0; k < r; ++k) {
         const int j = jb.pos[k];
         double acc = 0.0;
-        for (int i = j + tid; i < rows; i += 256) {
-            const double2 v = Y[(long)i * r + k];
-            acc = fma(v.x, v.x, fma(v.y, v.y, acc));
-        }
+        if (tid < 256)
+            for (int i = j + tid; i < rows; i += 256) {
+                const double2 v = Y[(long)i * r + k];
+                acc = fma(v.x, v.x, fma(v.y, v.y, acc));
+            }
         acc = wave_sum(acc);
-        if (lane == 0) red[wave] = acc;
+        if (lane == 0 && wave < 4) red[wave] = acc;
         __syncthreads();
         if (tid == 0) {
             const double a2 = ((red[0] + red[1]) + red[2]) + red[3];
@@ -1629,13 +1633,13 @@ __global__ __launch_bounds__(256) void k_householder(const HouseJob* __restrict_
         __syncthreads();
         if (s_skip) continue;
         const double inv = 1.0 / s_vn;
-        for (int i = j + tid; i < rows; i += 256) {
+        for (int i = j + tid; i < rows; i += 1024) {
             const double2 x = (i == j) ? s_x0 : Y[(long)i * r + k];
             V[(long)i * r + k] = cscale(x, inv);
         }
         __syncthreads();
         // Trailing columns: Y[:, c] -= 2 v (v^H Y[:, c]), one wave per column.
-        for (int c = k + 1 + wave; c < r; c += 4) {
+        for (int c = k + 1 + wave; c < r; c += 16) {
             double2 d = cz();
             for (int i = j + lane; i < rows; i += 64) ccmac(d, V[(long)i * r + k], Y[(long)i * r + c]);
             d.x = wave_sum(d.x);
@@ -1652,11 +1656,11 @@ __global__ __launch_bounds__(256) void k_householder(const HouseJob* __restrict_
     }
     // T (upper triangular): T[k][k] = tau_k, T[0:k, k] = -tau_k T[0:k, 0:k] (V[:,0:k]^H v_k).
     double2* T = jb.T;
-    for (long e = tid; e < (long)r * r; e += 256) T[e] = cz();
+    for (long e = tid; e < (long)r * r; e += 1024) T[e] = cz();
     __syncthreads();
     for (int k = 0; k < r; ++k) {
         if (s_tau[k] == 0.0) continue;
-        for (int i = wave; i < k; i += 4) {
+        for (int i = wave; i < k; i += 16) {
             double2 d = cz();
             for (int row = lane; row < rows; row += 64)
                 ccmac(d, V[(long)row * r + i], V[(long)row * r + k]);
@@ -2055,7 +2059,7 @@ void launch_factor(const FactorJob* jobs, int njobs, long max_left, long max_rig
 }
 
 void launch_householder(const HouseJob* jobs, int njobs, hipStream_t s) {
-    hipLaunchKernelGGL(k_householder, dim3(njobs), dim3(256), 0, s, jobs);
+    hipLaunchKernelGGL(k_householder, dim3(njobs), dim3(1024), 0, s, jobs);
 }
 
 void launch_lq_panel(const LqJob* jobs, int njobs, int b0, int j0, int max_cols, hipStream_t s) {
