@@ -1150,46 +1150,44 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(const GemmJob* __restrict_
             e2[a][b] = e1[a][b];
             e3[a][b] = e1[a][b];
         }
+    // The next K tile is fetched into registers while the MFMAs consume the current one from
+    // LDS (the accumulation order over k is unchanged).
+    constexpr int PER = (MG * MGK) / 256;
+    double2 ra[PER], rb[PER];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int e = tid + 256 * u;
+            if (jb.opA == 0) {
+                const int m = m0 + e / MGK, k = k0 + e % MGK;
+                ra[u] = (m < jb.M && k < kend) ? jb.A[(long)m * jb.lda + k] : cz();
+            } else {
+                const int m = m0 + e % MG, k = k0 + e / MG;
+                ra[u] = (m < jb.M && k < kend) ? cconj(jb.A[(long)k * jb.lda + m]) : cz();
+            }
+            if (jb.opB == 0) {
+                const int n = n0 + e % MG, k = k0 + e / MG;
+                rb[u] = (n < jb.N && k < kend) ? jb.B[(long)k * jb.ldb + n] : cz();
+            } else {
+                const int n = n0 + e / MGK, k = k0 + e % MGK;
+                rb[u] = (n < jb.N && k < kend) ? cconj(jb.B[(long)n * jb.ldb + k]) : cz();
+            }
+        }
+    };
+    if (kbeg < kend) fetch(kbeg);
     for (int k0 = kbeg; k0 < kend; k0 += MGK) {
 #pragma unroll
-        for (int u = 0; u < (MG * MGK) / 256; ++u) {
+        for (int u = 0; u < PER; ++u) {
             const int e = tid + 256 * u;
-            double2 v;
-            int mm, kk;
-            if (jb.opA == 0) {
-                mm = e / MGK;
-                kk = e % MGK;
-                const int m = m0 + mm, k = k0 + kk;
-                v = (m < jb.M && k < kend) ? jb.A[(long)m * jb.lda + k] : cz();
-            } else {
-                kk = e / MG;
-                mm = e % MG;
-                const int m = m0 + mm, k = k0 + kk;
-                v = (m < jb.M && k < kend) ? cconj(jb.A[(long)k * jb.lda + m]) : cz();
-            }
-            Ar[mm][kk] = v.x;
-            Ai[mm][kk] = v.y;
-        }
-#pragma unroll
-        for (int u = 0; u < (MG * MGK) / 256; ++u) {
-            const int e = tid + 256 * u;
-            double2 v;
-            int nn, kk;
-            if (jb.opB == 0) {
-                kk = e / MG;
-                nn = e % MG;
-                const int n = n0 + nn, k = k0 + kk;
-                v = (n < jb.N && k < kend) ? jb.B[(long)k * jb.ldb + n] : cz();
-            } else {
-                nn = e / MGK;
-                kk = e % MGK;
-                const int n = n0 + nn, k = k0 + kk;
-                v = (n < jb.N && k < kend) ? cconj(jb.B[(long)n * jb.ldb + k]) : cz();
-            }
-            Br[kk][nn] = v.x;
-            Bi[kk][nn] = v.y;
+            const int am = jb.opA == 0 ? e / MGK : e % MG, ak = jb.opA == 0 ? e % MGK : e / MG;
+            Ar[am][ak] = ra[u].x;
+            Ai[am][ak] = ra[u].y;
+            const int bn = jb.opB == 0 ? e % MG : e / MGK, bk = jb.opB == 0 ? e / MG : e % MGK;
+            Br[bk][bn] = rb[u].x;
+            Bi[bk][bn] = rb[u].y;
         }
         __syncthreads();
+        if (k0 + MGK < kend) fetch(k0 + MGK);
 #pragma unroll
         for (int ks = 0; ks < MGK / 4; ++ks) {
             const int kk = 4 * ks + lk;
