@@ -521,7 +521,7 @@ struct wqc_tn_state {
     double stall_off = STALL_OFF;
     bool use_lq = true;
     bool use_splitk = true;  // WQC_TN_SPLITK=0 turns split-K GEMMs off
-    bool use_wide = true;    // WQC_TN_JACOBI_WIDE=0 keeps every FP64 round at 256 threads
+    bool use_wide = true;    // WQC_TN_JACOBI_WIDE=0 keeps every Jacobi round at 256 threads
     DevBuf trace_buf;
 
     void init(wqc_tn_device* d, size_t qn, size_t mb, int prec) {
@@ -874,17 +874,28 @@ struct wqc_tn_state {
         trace_buf.ensure((size_t)nj * (sizeof(unsigned long long) + sizeof(int)));
         unsigned long long* d_off = trace_jacobi ? reinterpret_cast<unsigned long long*>(trace_buf.p) : nullptr;
         int* d_rot = reinterpret_cast<int*>(reinterpret_cast<unsigned long long*>(trace_buf.p) + nj);
+        // Block pairs per job, for the choice of the 512-thread form (as in jacobi()).
+        std::vector<int> jpairs(nj);
+        for (int j = 0; j < nj; ++j) {
+            const int b = (jobs[j].n + 15) / 16;
+            jpairs[j] = (b + (b & 1)) / 2;
+        }
+        const int cus = dev->prop.multiProcessorCount;
         for (int sweep = 0; sweep < max_sweeps_lo && active > 0; ++sweep) {
             check_cancel();
             if (d_off)
                 wqc::launch_fill_int(reinterpret_cast<int*>(d_off), 0, 3 * nj, stream);
             else
                 wqc::launch_fill_int(d_rot, 0, nj, stream);
+            int wgs = 0;
+            for (int j = 0; j < nj; ++j)
+                if (!done[j]) wgs += jpairs[j];
+            const bool wide = use_wide && wgs <= cus;
             {
                 Timer tm(this, 3);
                 for (int r = 0; r < rounds; ++r)
                     wqc::launch_jacobi_round_lo(dj, nj, pairs, r, tol, nu2, (float)floor_lo, inner_cap_lo, d_rot, d_done,
-                                                d_jstats, d_off, stream);
+                                                d_jstats, d_off, wide, stream);
                 check_launch();
                 klaunch[3] += rounds;
             }

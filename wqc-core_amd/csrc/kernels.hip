@@ -710,13 +710,16 @@ __device__ __forceinline__ f4v mfma_f32(float a, float b, f4v c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-__global__ __launch_bounds__(256, 3) void k_jacobi_round_lo(const JacobiLoJob* __restrict__ jobs,
+// NT = 512 for rounds with at most one block pair per CU, as in k_jacobi_round.
+template <int NT>
+__global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round_lo(const JacobiLoJob* __restrict__ jobs,
                                                          int round, float tol_base, double nu2_base,
                                                          float rel_floor, int inner_cap,
                                                          int* __restrict__ rotated,
                                                          const int* __restrict__ done,
                                                          unsigned long long* __restrict__ stats,
                                                          unsigned long long* __restrict__ offmax) {
+    constexpr int NW = NT / 64;
     const int job = blockIdx.y;
     if (done[job]) return;
     const JacobiLoJob jb = jobs[job];
@@ -739,7 +742,7 @@ __global__ __launch_bounds__(256, 3) void k_jacobi_round_lo(const JacobiLoJob* _
     __shared__ float2 G[JP][JP + 1];
     __shared__ float2 T[JP][JP + 1];
     // Partial Gram tiles of two waves; afterwards the second G and T buffers of the eigensolve.
-    __shared__ __align__(16) float s_aux[4 * JP * (JP + 1)];
+    __shared__ __align__(16) float s_aux[NW == 8 ? 4 * 6 * 4 * 64 : 4 * JP * (JP + 1)];
     __shared__ int s_row[JP];
     __shared__ int s_any;
     __shared__ int s_intra;
@@ -793,11 +796,11 @@ __global__ __launch_bounds__(256, 3) void k_jacobi_round_lo(const JacobiLoJob* _
         int c0 = wave * 16;
         if (c0 < jb.m) WQC_GRAM_LOAD(c0, ua, va)
         while (c0 < jb.m) {
-            const int c1 = c0 + 64;
+            const int c1 = c0 + 16 * NW;
             if (c1 < jb.m) WQC_GRAM_LOAD(c1, ub, vb)
             WQC_GRAM_ACC(ua, va)
             if (c1 >= jb.m) break;
-            const int c2 = c1 + 64;
+            const int c2 = c1 + 16 * NW;
             if (c2 < jb.m) WQC_GRAM_LOAD(c2, ua, va)
             WQC_GRAM_ACC(ub, vb)
             c0 = c2;
@@ -810,7 +813,20 @@ __global__ __launch_bounds__(256, 3) void k_jacobi_round_lo(const JacobiLoJob* _
 #define WQC_GP_ALL(OP) OP(0, a00r) OP(1, a00i) OP(2, a01r) OP(3, a01i) OP(4, a11r) OP(5, a11i)
 #define WQC_GP_STORE(t, v) for (int j = 0; j < 4; ++j) Gp[slot][t][j][lane] = v[j];
 #define WQC_GP_ADD(t, v) for (int j = 0; j < 4; ++j) v[j] += Gp[slot][t][j][lane];
-        if (wave >= 2) {
+        if constexpr (NW == 8) {
+            // (w0 + w4), (w1 + w5), (w2 + w6), (w3 + w7) first.
+            if (wave >= 4) {
+                const int slot = wave - 4;
+                WQC_GP_ALL(WQC_GP_STORE)
+            }
+            __syncthreads();
+            if (wave < 4) {
+                const int slot = wave;
+                WQC_GP_ALL(WQC_GP_ADD)
+            }
+            __syncthreads();
+        }
+        if (wave >= 2 && wave < 4) {
             const int slot = wave - 2;
             WQC_GP_ALL(WQC_GP_STORE)
         }
@@ -858,7 +874,7 @@ __global__ __launch_bounds__(256, 3) void k_jacobi_round_lo(const JacobiLoJob* _
     {
         bool off = false, intra = false;
         double worst = 0.0;
-        for (int e = tid; e < JP * JP; e += 256) {
+        for (int e = tid; e < JP * JP; e += NT) {
             const int i = e / JP, j = e % JP;
             if (i < j) {
                 const float2 c = G[i][j];
@@ -887,7 +903,7 @@ __global__ __launch_bounds__(256, 3) void k_jacobi_round_lo(const JacobiLoJob* _
         rotated[job] = 1;
         s_any = 0;
     }
-    for (int e = tid; e < JP * JP; e += 256) {
+    for (int e = tid; e < JP * JP; e += NT) {
         const int i = e / JP, j = e % JP;
         T[i][j] = make_float2(i == j ? 1.f : 0.f, 0.f);
     }
@@ -912,61 +928,63 @@ __global__ __launch_bounds__(256, 3) void k_jacobi_round_lo(const JacobiLoJob* _
         const bool cross = sweep == 0 && !intra0;
         const int nsteps = cross ? JB : JP - 1;
         for (int step = 0; step < nsteps; ++step) {
-            int p1, q1, p2, q2;
-            if (cross) {
-                cross_pair(step, bk, p1, q1);
-                cross_pair(step, bk2, p2, q2);
-            } else {
-                circle_pair(step, bk, p1, q1);
-                circle_pair(step, bk2, p2, q2);
+            if (tid < 256) {  // the eigensolve's 16 x 16 tiles of 2 x 2
+                int p1, q1, p2, q2;
+                if (cross) {
+                    cross_pair(step, bk, p1, q1);
+                    cross_pair(step, bk2, p2, q2);
+                } else {
+                    circle_pair(step, bk, p1, q1);
+                    circle_pair(step, bk2, p2, q2);
+                }
+                // Parameters in FP64 from the FP32 Gram entries (no underflow of |G_pq|^2).
+                const float2 c2 = Gc[p2][q2];
+                const Rot64 r2 = rot_params64(Gc[p2][p2].x, Gc[q2][q2].x, make_double2(c2.x, c2.y), tol, low);
+                const Rot64 r1 = rot_from_lane(r2, bk & 15);
+                float2 h00 = Gc[p1][p2], h01 = Gc[p1][q2], h10 = Gc[q1][p2], h11 = Gc[q1][q2];
+                float2 t00 = Tc[p1][p2], t01 = Tc[p1][q2], t10 = Tc[q1][p2], t11 = Tc[q1][q2];
+                if (r1.act) {
+                    const float cs = (float)r1.cs;
+                    const float2 m01 = f2(r1.m01), m10 = f2(r1.m10);
+                    const float2 x00 = fa(fs(h00, cs), fm(m01, h10)), x01 = fa(fs(h01, cs), fm(m01, h11));
+                    const float2 x10 = fa(fm(m10, h00), fs(h10, cs)), x11 = fa(fm(m10, h01), fs(h11, cs));
+                    h00 = x00;
+                    h01 = x01;
+                    h10 = x10;
+                    h11 = x11;
+                    const float2 y00 = fa(fs(t00, cs), fm(m01, t10)), y01 = fa(fs(t01, cs), fm(m01, t11));
+                    const float2 y10 = fa(fm(m10, t00), fs(t10, cs)), y11 = fa(fm(m10, t01), fs(t11, cs));
+                    t00 = y00;
+                    t01 = y01;
+                    t10 = y10;
+                    t11 = y11;
+                }
+                if (r2.act) {
+                    const float cs = (float)r2.cs;
+                    const float2 c01 = fc(f2(r2.m01)), c10 = fc(f2(r2.m10));
+                    const float2 n00 = fa(fs(h00, cs), fm(h01, c01)), n01 = fa(fm(h00, c10), fs(h01, cs));
+                    const float2 n10 = fa(fs(h10, cs), fm(h11, c01)), n11 = fa(fm(h10, c10), fs(h11, cs));
+                    h00 = n00;
+                    h01 = n01;
+                    h10 = n10;
+                    h11 = n11;
+                }
+                if (bk == bk2 && r1.act) {
+                    h00 = make_float2((float)(Gc[p1][p1].x - r1.tg), 0.f);
+                    h11 = make_float2((float)(Gc[q1][q1].x + r1.tg), 0.f);
+                    h01 = make_float2(0.f, 0.f);
+                    h10 = h01;
+                    s_any = 1;
+                }
+                Gn[p1][p2] = h00;
+                Gn[p1][q2] = h01;
+                Gn[q1][p2] = h10;
+                Gn[q1][q2] = h11;
+                Tn[p1][p2] = t00;
+                Tn[p1][q2] = t01;
+                Tn[q1][p2] = t10;
+                Tn[q1][q2] = t11;
             }
-            // Parameters in FP64 from the FP32 Gram entries (no underflow of |G_pq|^2).
-            const float2 c2 = Gc[p2][q2];
-            const Rot64 r2 = rot_params64(Gc[p2][p2].x, Gc[q2][q2].x, make_double2(c2.x, c2.y), tol, low);
-            const Rot64 r1 = rot_from_lane(r2, bk & 15);
-            float2 h00 = Gc[p1][p2], h01 = Gc[p1][q2], h10 = Gc[q1][p2], h11 = Gc[q1][q2];
-            float2 t00 = Tc[p1][p2], t01 = Tc[p1][q2], t10 = Tc[q1][p2], t11 = Tc[q1][q2];
-            if (r1.act) {
-                const float cs = (float)r1.cs;
-                const float2 m01 = f2(r1.m01), m10 = f2(r1.m10);
-                const float2 x00 = fa(fs(h00, cs), fm(m01, h10)), x01 = fa(fs(h01, cs), fm(m01, h11));
-                const float2 x10 = fa(fm(m10, h00), fs(h10, cs)), x11 = fa(fm(m10, h01), fs(h11, cs));
-                h00 = x00;
-                h01 = x01;
-                h10 = x10;
-                h11 = x11;
-                const float2 y00 = fa(fs(t00, cs), fm(m01, t10)), y01 = fa(fs(t01, cs), fm(m01, t11));
-                const float2 y10 = fa(fm(m10, t00), fs(t10, cs)), y11 = fa(fm(m10, t01), fs(t11, cs));
-                t00 = y00;
-                t01 = y01;
-                t10 = y10;
-                t11 = y11;
-            }
-            if (r2.act) {
-                const float cs = (float)r2.cs;
-                const float2 c01 = fc(f2(r2.m01)), c10 = fc(f2(r2.m10));
-                const float2 n00 = fa(fs(h00, cs), fm(h01, c01)), n01 = fa(fm(h00, c10), fs(h01, cs));
-                const float2 n10 = fa(fs(h10, cs), fm(h11, c01)), n11 = fa(fm(h10, c10), fs(h11, cs));
-                h00 = n00;
-                h01 = n01;
-                h10 = n10;
-                h11 = n11;
-            }
-            if (bk == bk2 && r1.act) {
-                h00 = make_float2((float)(Gc[p1][p1].x - r1.tg), 0.f);
-                h11 = make_float2((float)(Gc[q1][q1].x + r1.tg), 0.f);
-                h01 = make_float2(0.f, 0.f);
-                h10 = h01;
-                s_any = 1;
-            }
-            Gn[p1][p2] = h00;
-            Gn[p1][q2] = h01;
-            Gn[q1][p2] = h10;
-            Gn[q1][q2] = h11;
-            Tn[p1][p2] = t00;
-            Tn[p1][q2] = t01;
-            Tn[q1][p2] = t10;
-            Tn[q1][q2] = t11;
             __syncthreads();
             float2 (*sw)[JP + 1] = Gc;
             Gc = Gn;
@@ -989,7 +1007,7 @@ __global__ __launch_bounds__(256, 3) void k_jacobi_round_lo(const JacobiLoJob* _
     // buffer.
     pair_order(tid, s_row, &Gc[0][0].x, 2 * (JP + 2), s_dst);
     __syncthreads();
-    for (int e = tid; e < JP * JP; e += 256) {
+    for (int e = tid; e < JP * JP; e += NT) {
         const int i = e / JP, j = e % JP;
         Gn[s_dst[i]][j] = Tc[i][j];
     }
@@ -1046,11 +1064,11 @@ __global__ __launch_bounds__(256, 3) void k_jacobi_round_lo(const JacobiLoJob* _
             int c0 = wave * 16;
             if (c0 < cols) WQC_APPLY_LOAD(c0, ba)
             while (c0 < cols) {
-                const int c1 = c0 + 64;
+                const int c1 = c0 + 16 * NW;
                 if (c1 < cols) WQC_APPLY_LOAD(c1, bb)
                 WQC_APPLY_ROT(c0, ba)
                 if (c1 >= cols) break;
-                const int c2 = c1 + 64;
+                const int c2 = c1 + 16 * NW;
                 if (c2 < cols) WQC_APPLY_LOAD(c2, ba)
                 WQC_APPLY_ROT(c1, bb)
                 c0 = c2;
@@ -1974,8 +1992,13 @@ void launch_jacobi_round(const JacobiJob* jobs, int njobs, int max_pairs, int ro
 
 void launch_jacobi_round_lo(const JacobiLoJob* jobs, int njobs, int max_pairs, int round, float tol,
                             double nu2, float rel_floor, int inner_cap, int* rotated, const int* done, unsigned long long* stats,
-                            unsigned long long* offmax, hipStream_t s) {
-    hipLaunchKernelGGL(k_jacobi_round_lo, dim3(max_pairs, njobs), dim3(256), 0, s, jobs, round, tol, nu2, rel_floor, inner_cap, rotated, done, stats, offmax);
+                            unsigned long long* offmax, bool wide, hipStream_t s) {
+    if (wide)
+        hipLaunchKernelGGL(k_jacobi_round_lo<512>, dim3(max_pairs, njobs), dim3(512), 0, s, jobs, round, tol, nu2,
+                           rel_floor, inner_cap, rotated, done, stats, offmax);
+    else
+        hipLaunchKernelGGL(k_jacobi_round_lo<256>, dim3(max_pairs, njobs), dim3(256), 0, s, jobs, round, tol, nu2,
+                           rel_floor, inner_cap, rotated, done, stats, offmax);
 }
 
 void launch_to_f32(const double2* const* src, float2* const* dst, const long* counts, int njobs,
