@@ -1396,7 +1396,6 @@ struct LqShared {
     double2 alpha;
     double2 T[LQ_NB][LQ_NB];
     double beta[LQ_NB];
-    double2 z[LQ_NB];
 };
 
 // Step T of the panel (row T, diagonal column j0 + T); a template so that every register
@@ -1459,16 +1458,18 @@ __device__ __forceinline__ void lq_step(double2 (&x)[LQ_NB][CPT], LqShared& sh, 
                     x[v][k] = csub(x[v][k], cmul(coef, cconj(u)));
                 }
             }
+            // Column T of the panel's T: row i (one thread each) is sum_{l >= i} T[i][l] z_l with
+            // z_l = -tau g_l, summed in increasing l.
+            if (tid < T) {
+                const double2 mt = make_double2(-tau.x, -tau.y);
+                double2 a = cz();
+#pragma unroll
+                for (int l = 0; l < T; ++l)
+                    if (l >= tid) cmac(a, sh.T[tid][l], cmul(mt, make_double2(acc[2 * l], acc[2 * l + 1])));
+                sh.T[tid][T] = a;
+            }
             if (tid == 0) {
                 sh.beta[T] = beta;
-#pragma unroll
-                for (int i = 0; i < T; ++i)
-                    sh.z[i] = cmul(make_double2(-tau.x, -tau.y), make_double2(acc[2 * i], acc[2 * i + 1]));
-                for (int i = 0; i < T; ++i) {
-                    double2 a = cz();
-                    for (int l = i; l < T; ++l) cmac(a, sh.T[i][l], sh.z[l]);
-                    sh.T[i][T] = a;
-                }
                 sh.T[T][T] = tau;
             }
         }
