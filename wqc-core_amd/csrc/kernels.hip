@@ -1591,6 +1591,27 @@ __global__ __launch_bounds__(256) void k_lq_init_m(const LqJob* __restrict__ job
 // ------------------------------------------------------------------ Householder QR
 // One workgroup per job; see HouseJob. Reflector convention v = x + (x0/|x0|) ||x|| e1,
 // normalized (the reflection axis nalgebra computes), H = I - 2 v v^H.
+// d = sum over rows i = i0, i0 + 64, ... < rows of conj(A[i r + ca]) B[i r + cb], accumulated
+// in increasing i as a plain loop would; the loads of four rows are issued before their
+// products are added, so the column walk does not wait on one load at a time.
+__device__ __forceinline__ double2 col_dot(const double2* __restrict__ A, int ca, const double2* __restrict__ B,
+                                           int cb, int r, int i0, int rows) {
+    double2 d = cz();
+    for (int base = i0; base < rows; base += 4 * 64) {
+        double2 a[4], b[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int i = base + 64 * t;
+            a[t] = i < rows ? A[(long)i * r + ca] : cz();
+            b[t] = i < rows ? B[(long)i * r + cb] : cz();
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            if (base + 64 * t < rows) ccmac(d, a[t], b[t]);
+    }
+    return d;
+}
+
 __global__ __launch_bounds__(1024) void k_householder(const HouseJob* __restrict__ jobs) {
     const HouseJob jb = jobs[blockIdx.x];
     const int rows = jb.rows, r = jb.r;
@@ -1602,8 +1623,8 @@ __global__ __launch_bounds__(1024) void k_householder(const HouseJob* __restrict
     __shared__ double2 s_x0;
     __shared__ double s_alpha, s_vn;
     __shared__ int s_skip;
-    double2* Y = jb.Y;
-    double2* V = jb.V;
+    double2* __restrict__ Y = jb.Y;
+    double2* __restrict__ V = jb.V;
     // Sixteen waves; every per-column loop gives a column to one wave, and the one block sum
     // (the reflector norm) runs on the first four waves, so the arithmetic of each result
     // does not depend on the number of waves.
@@ -1657,16 +1678,23 @@ __global__ __launch_bounds__(1024) void k_householder(const HouseJob* __restrict
         __syncthreads();
         // Trailing columns: Y[:, c] -= 2 v (v^H Y[:, c]), one wave per column.
         for (int c = k + 1 + wave; c < r; c += 16) {
-            double2 d = cz();
-            for (int i = j + lane; i < rows; i += 64) ccmac(d, V[(long)i * r + k], Y[(long)i * r + c]);
+            double2 d = col_dot(V, k, Y, c, r, j + lane, rows);
             d.x = wave_sum(d.x);
             d.y = wave_sum(d.y);
             const double2 f = cscale(d, 2.0);
-            for (int i = j + lane; i < rows; i += 64) {
-                double2 y = Y[(long)i * r + c];
-                const double2 v = V[(long)i * r + k];
-                y = csub(y, cmul(v, f));
-                Y[(long)i * r + c] = y;
+            for (int base = j + lane; base < rows; base += 4 * 64) {
+                double2 y[4], v[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int i = base + 64 * t;
+                    y[t] = i < rows ? Y[(long)i * r + c] : cz();
+                    v[t] = i < rows ? V[(long)i * r + k] : cz();
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int i = base + 64 * t;
+                    if (i < rows) Y[(long)i * r + c] = csub(y[t], cmul(v[t], f));
+                }
             }
         }
         __syncthreads();
@@ -1678,9 +1706,7 @@ __global__ __launch_bounds__(1024) void k_householder(const HouseJob* __restrict
     for (int k = 0; k < r; ++k) {
         if (s_tau[k] == 0.0) continue;
         for (int i = wave; i < k; i += 16) {
-            double2 d = cz();
-            for (int row = lane; row < rows; row += 64)
-                ccmac(d, V[(long)row * r + i], V[(long)row * r + k]);
+            double2 d = col_dot(V, i, V, k, r, lane, rows);
             d.x = wave_sum(d.x);
             d.y = wave_sum(d.y);
             if (lane == 0) s_w[i] = d;
