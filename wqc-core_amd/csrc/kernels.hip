@@ -1591,27 +1591,6 @@ __global__ __launch_bounds__(256) void k_lq_init_m(const LqJob* __restrict__ job
 // ------------------------------------------------------------------ Householder QR
 // One workgroup per job; see HouseJob. Reflector convention v = x + (x0/|x0|) ||x|| e1,
 // normalized (the reflection axis nalgebra computes), H = I - 2 v v^H.
-// d = sum over rows i = i0, i0 + 64, ... < rows of conj(A[i r + ca]) B[i r + cb], accumulated
-// in increasing i as a plain loop would; the loads of four rows are issued before their
-// products are added, so the column walk does not wait on one load at a time.
-__device__ __forceinline__ double2 col_dot(const double2* __restrict__ A, int ca, const double2* __restrict__ B,
-                                           int cb, int r, int i0, int rows) {
-    double2 d = cz();
-    for (int base = i0; base < rows; base += 4 * 64) {
-        double2 a[4], b[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int i = base + 64 * t;
-            a[t] = i < rows ? A[(long)i * r + ca] : cz();
-            b[t] = i < rows ? B[(long)i * r + cb] : cz();
-        }
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-            if (base + 64 * t < rows) ccmac(d, a[t], b[t]);
-    }
-    return d;
-}
-
 __global__ __launch_bounds__(1024) void k_householder(const HouseJob* __restrict__ jobs) {
     const HouseJob jb = jobs[blockIdx.x];
     const int rows = jb.rows, r = jb.r;
@@ -1623,11 +1602,12 @@ __global__ __launch_bounds__(1024) void k_householder(const HouseJob* __restrict
     __shared__ double2 s_x0;
     __shared__ double s_alpha, s_vn;
     __shared__ int s_skip;
+    __shared__ double2 s_part[16][64];  // per-wave partial column sums
+    __shared__ double2 s_f[64];
     double2* __restrict__ Y = jb.Y;
     double2* __restrict__ V = jb.V;
-    // Sixteen waves; every per-column loop gives a column to one wave, and the one block sum
-    // (the reflector norm) runs on the first four waves, so the arithmetic of each result
-    // does not depend on the number of waves.
+    // Sixteen waves. The column norms give a column to one wave and the reflector norm runs on
+    // the first four waves; the trailing update and T run lanes along rows (see there).
     // Original column norms.
     for (int k = wave; k < r; k += 16) {
         double acc = 0.0;
@@ -1676,28 +1656,32 @@ __global__ __launch_bounds__(1024) void k_householder(const HouseJob* __restrict
             V[(long)i * r + k] = cscale(x, inv);
         }
         __syncthreads();
-        // Trailing columns: Y[:, c] -= 2 v (v^H Y[:, c]), one wave per column.
-        for (int c = k + 1 + wave; c < r; c += 16) {
-            double2 d = col_dot(V, k, Y, c, r, j + lane, rows);
-            d.x = wave_sum(d.x);
-            d.y = wave_sum(d.y);
-            const double2 f = cscale(d, 2.0);
-            for (int base = j + lane; base < rows; base += 4 * 64) {
-                double2 y[4], v[4];
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const int i = base + 64 * t;
-                    y[t] = i < rows ? Y[(long)i * r + c] : cz();
-                    v[t] = i < rows ? V[(long)i * r + k] : cz();
-                }
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const int i = base + 64 * t;
-                    if (i < rows) Y[(long)i * r + c] = csub(y[t], cmul(v[t], f));
+        // Trailing columns: Y[:, c] -= 2 v (v^H Y[:, c]). Lanes run along a row (coalesced:
+        // Y and V are row-major with r columns), wave w takes rows j + w, j + w + 16, ...;
+        // the sixteen wave partials of each column are added in wave order.
+        for (int c0 = k + 1; c0 < r; c0 += 64) {
+            const int c = c0 + lane;
+            const bool act = c < r;
+            double2 d = cz();
+            if (act)
+                for (int i = j + wave; i < rows; i += 16) ccmac(d, V[(long)i * r + k], Y[(long)i * r + c]);
+            s_part[wave][lane] = d;
+            __syncthreads();
+            if (wave == 0) {
+                double2 t = cz();
+                for (int w = 0; w < 16; ++w) t = cadd(t, s_part[w][lane]);
+                s_f[lane] = cscale(t, 2.0);
+            }
+            __syncthreads();
+            if (act) {
+                const double2 f = s_f[lane];
+                for (int i = j + wave; i < rows; i += 16) {
+                    const long at = (long)i * r + c;
+                    Y[at] = csub(Y[at], cmul(V[(long)i * r + k], f));
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
     // T (upper triangular): T[k][k] = tau_k, T[0:k, k] = -tau_k T[0:k, 0:k] (V[:,0:k]^H v_k).
     double2* T = jb.T;
@@ -1705,13 +1689,21 @@ __global__ __launch_bounds__(1024) void k_householder(const HouseJob* __restrict
     __syncthreads();
     for (int k = 0; k < r; ++k) {
         if (s_tau[k] == 0.0) continue;
-        for (int i = wave; i < k; i += 16) {
-            double2 d = col_dot(V, i, V, k, r, lane, rows);
-            d.x = wave_sum(d.x);
-            d.y = wave_sum(d.y);
-            if (lane == 0) s_w[i] = d;
+        // s_w[i] = V[:, i]^H v_k for i < k, laid out like the trailing update.
+        for (int i0 = 0; i0 < k; i0 += 64) {
+            const int i = i0 + lane;
+            double2 d = cz();
+            if (i < k)
+                for (int row = wave; row < rows; row += 16) ccmac(d, V[(long)row * r + i], V[(long)row * r + k]);
+            s_part[wave][lane] = d;
+            __syncthreads();
+            if (wave == 0 && i < k) {
+                double2 t = cz();
+                for (int w = 0; w < 16; ++w) t = cadd(t, s_part[w][lane]);
+                s_w[i] = t;
+            }
+            __syncthreads();
         }
-        __syncthreads();
         if (tid < k) {
             double2 acc = cz();
             for (int l = tid; l < k; ++l) cmac(acc, T[(long)tid * r + l], s_w[l]);
