@@ -115,8 +115,9 @@ const double JACOBI_TOL_LO = 1e-4;
 const int MAX_SWEEPS_LO = 30;
 // The low rounds run at most this many sweeps per job: they pay while the off-diagonal mass
 // falls linearly; after that, rows near FP32 resolution keep single jobs rotating without
-// helping the FP64 finish (config 3 with LQ: cap 30 -> 939 merges/s, 6 -> 1387, 3 -> 1363).
-const int LO_SWEEP_CAP = 6;
+// helping the FP64 finish (config 3 with LQ: cap 30 -> 939 merges/s, 6 -> 1387, 3 -> 1363;
+// with the current round kernels 8 -> 1537, 7 -> 1561, 6 -> 1588, 5 -> 1602, 4 -> 1601).
+const int LO_SWEEP_CAP = 5;
 // With LQ preconditioning (the mixed form), Jacobi jobs of at least this many rows (and at
 // most LQ_MAX_COLS columns) are rotated as R = L^H of a Householder LQ, X = L Q: on graded
 // and rank-deficient spectra the rows of R converge in about half the sweeps of the rows of
