@@ -109,6 +109,9 @@ typedef struct wqc_tn_stats {
     uint64_t jacobi_pair_solves;  /* block-pair eigensolves run                             */
     uint64_t jacobi_sweeps_lo;    /* low-precision sweeps of the mixed form                 */
     uint64_t jacobi_stalled;      /* Jacobi jobs stopped at their rounding floor            */
+    uint64_t jacobi_unconverged;  /* Jacobi jobs still rotating at the sweep cap (60): an
+                                     exact split then fails with EXEC_FAILED, a range-finder
+                                     split falls back to the exact split                     */
 } wqc_tn_stats;
 
 /* ---- errors ------------------------------------------------------------------ */
@@ -297,6 +300,11 @@ int wqc_tn_download_site(wqc_tn_state *st, size_t site, double *out, size_t len_
                          size_t *l, size_t *r);
 int wqc_tn_upload_site(wqc_tn_state *st, size_t site, const double *data, size_t l,
                        size_t r);
+/* dst.sites = src.sites.clone() (the `pub sites` field, src/tn/mps.rs:56) without a host round
+ * trip: every site of src copied device to device into dst. Both states must be on one device
+ * and hold the same qubit count (INVALID_ARG otherwise); precision, bond cap and counters of dst
+ * are kept. */
+int wqc_tn_copy_sites(wqc_tn_state *dst, const wqc_tn_state *src);
 
 /* ---- reporting ---------------------------------------------------------------------- */
 

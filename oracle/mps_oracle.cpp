@@ -552,8 +552,11 @@ std::vector<C> adjoint_rm(const std::vector<C>& a, size_t rows, size_t cols) {
 // (a zero residual gives an identity reflector).
 double g_qr_rank_tol = 0.0;
 
-// thin_q (src/tn/mps.rs:1091-1101): Householder QR, Q is rows x ell row-major.
-std::vector<C> thin_q(const std::vector<C>& y, size_t rows, size_t ell) {
+// thin_q (src/tn/mps.rs:1091-1101): Householder QR, Q is rows x ell row-major. When vs_out is
+// given it receives the unit reflector vectors (vs[i] acts on rows i..rows-1; empty where the
+// column took no reflector).
+std::vector<C> thin_q(const std::vector<C>& y, size_t rows, size_t ell,
+                      std::vector<std::vector<C>>* vs_out = nullptr) {
     Mat a(rows, ell);
     for (size_t r = 0; r < rows; ++r)
         for (size_t c = 0; c < ell; ++c) a(r, c) = y[r * ell + c];
@@ -604,6 +607,7 @@ std::vector<C> thin_q(const std::vector<C>& y, size_t rows, size_t ell) {
     std::vector<C> out(rows * ell);
     for (size_t r = 0; r < rows; ++r)
         for (size_t c = 0; c < ell; ++c) out[r * ell + c] = q(r, c);
+    if (vs_out) *vs_out = std::move(vs);
     return out;
 }
 
@@ -1418,6 +1422,29 @@ int oracle_thin_q(const double* y, size_t rows, size_t ell, double* q) {
     std::memcpy(m.data(), y, m.size() * sizeof(C));
     std::vector<C> out = thin_q(m, rows, ell);
     std::memcpy(q, out.data(), out.size() * sizeof(C));
+    return 0;
+}
+
+// thin_q at rank threshold tau with its reflectors exported: q (rows x ell), v (rows x ell
+// row-major, column i = reflector i, zero above row i and for skipped columns), mask[i] = 1
+// where column i took a reflector.
+int oracle_thin_q_reflectors(const double* y, size_t rows, size_t ell, double tau, double* q, double* v,
+                             int* mask) {
+    const double saved = g_qr_rank_tol;
+    g_qr_rank_tol = tau;
+    std::vector<C> m(rows * ell);
+    std::memcpy(m.data(), y, m.size() * sizeof(C));
+    std::vector<std::vector<C>> vs;
+    std::vector<C> out = thin_q(m, rows, ell, &vs);
+    g_qr_rank_tol = saved;
+    std::memcpy(q, out.data(), out.size() * sizeof(C));
+    C* vo = reinterpret_cast<C*>(v);
+    for (size_t i = 0; i < rows * ell; ++i) vo[i] = ZERO;
+    for (size_t i = 0; i < ell; ++i) {
+        mask[i] = (i < vs.size() && !vs[i].empty()) ? 1 : 0;
+        if (!mask[i]) continue;
+        for (size_t r = i; r < rows; ++r) vo[r * ell + i] = vs[i][r - i];
+    }
     return 0;
 }
 

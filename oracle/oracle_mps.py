@@ -76,6 +76,7 @@ def lib():
         L.oracle_mix_seed.argtypes = [sz, sz, sz]
         L.oracle_rademacher.argtypes = [sz, sz, ct.c_uint64, dp]
         L.oracle_thin_q.argtypes = [dp, sz, sz, dp]
+        L.oracle_thin_q_reflectors.argtypes = [dp, sz, sz, ct.c_double, dp, dp, ct.POINTER(ct.c_int)]
         L.oracle_pauli_expectation.argtypes = [dp, sz, ct.c_char_p, dp]
         L.oracle_sample_counts.argtypes = [dp, sz, ct.POINTER(ct.c_int32),
                                            ct.POINTER(ct.c_int32), sz, sz, ct.c_uint64,
@@ -279,6 +280,19 @@ def thin_q(y: np.ndarray):
     q = np.zeros((rows, ell), dtype=np.complex128)
     lib().oracle_thin_q(_dp(m.view(np.float64)), rows, ell, _dp(q.view(np.float64)))
     return q
+
+
+def thin_q_reflectors(y: np.ndarray, tau: float):
+    """thin_q at rank threshold tau: (Q, V, mask), V's column i the unit reflector of column i
+    (zero above row i; all zero where mask[i] == 0, the columns that took no reflector)."""
+    m = np.ascontiguousarray(y, dtype=np.complex128)
+    rows, ell = m.shape
+    q = np.zeros((rows, ell), dtype=np.complex128)
+    v = np.zeros((rows, ell), dtype=np.complex128)
+    mask = (ct.c_int * ell)()
+    lib().oracle_thin_q_reflectors(_dp(m.view(np.float64)), rows, ell, float(tau), _dp(q.view(np.float64)),
+                                   _dp(v.view(np.float64)), mask)
+    return q, v, np.array(list(mask), dtype=bool)
 
 
 def pauli_expectation(sv: np.ndarray, label: str) -> complex:

@@ -53,4 +53,69 @@ def test_null_arguments_are_rejected_without_a_device():
 
 def test_gate_struct_layout_matches_header():
     assert ct.sizeof(wqc_tn.Gate) == 24
-    assert ct.sizeof(wqc_tn.Stats) == 9 * 8 + 2 * 8 + 4 * 8
+    assert ct.sizeof(wqc_tn.Stats) == 9 * 8 + 2 * 8 + 5 * 8
+
+
+# ------------------------------------------------------------------ the Rust binding
+C_TO_RUST = {"uint64_t": "u64", "double": "c_double", "int32_t": "i32", "size_t": "usize",
+             "int": "c_int"}
+
+
+def header_structs():
+    text = open(os.path.join(ROOT, "include", "wqc_tn.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for body, name in re.findall(r"typedef struct \w+ \{(.*?)\}\s*(\w+);", text, flags=re.S):
+        fields = []
+        for decl in body.split(";"):
+            decl = decl.strip()
+            if not decl:
+                continue
+            ctype, fname = decl.rsplit(None, 1)
+            fields.append((fname, C_TO_RUST[ctype]))
+        out[name] = fields
+    return out
+
+
+def rust_block():
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    m = re.search(r"## 2\. FFI declarations.*?```rust\n(.*?)```", text, flags=re.S)
+    assert m, "INTEGRATION.md has no FFI declaration block"
+    return m.group(1)
+
+
+def rust_structs():
+    out = {}
+    for name, body in re.findall(r"pub struct (\w+)\s*\{(.*?)\}", rust_block(), flags=re.S):
+        fields = re.findall(r"pub (\w+):\s*(\w+)", body)
+        if fields:
+            out[name] = fields
+    return out
+
+
+def test_rust_structs_match_header():
+    """Every #[repr(C)] struct of the Rust binding in INTEGRATION.md has the header's fields, in
+    order and with the matching width: a missing field would let wqc_tn_get_stats write past a
+    Rust caller's struct."""
+    h, r = header_structs(), rust_structs()
+    assert {"wqc_tn_stats", "wqc_tn_gate", "wqc_tn_slice"} <= set(h)
+    for name, fields in h.items():
+        assert r.get(name) == fields, (name, r.get(name), fields)
+    assert ct.sizeof(wqc_tn.Stats) == 8 * len(h["wqc_tn_stats"])
+    assert [f for f, _ in wqc_tn.Stats._fields_] == [f for f, _ in h["wqc_tn_stats"]]
+
+
+def test_rust_functions_match_header():
+    """Every entry point the header declares has a Rust declaration with the same parameter
+    count."""
+    text = open(os.path.join(ROOT, "include", "wqc_tn.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    hdr = {}
+    for name, params in re.findall(r"\b(wqc_tn_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", text, flags=re.S):
+        params = params.strip()
+        hdr[name] = 0 if params in ("", "void") else params.count(",") + 1
+    rs = {}
+    for name, params in re.findall(r"pub fn (wqc_tn_[a-z0-9_]+)\s*\(([^)]*)\)", rust_block(), flags=re.S):
+        params = params.strip().rstrip(",")
+        rs[name] = 0 if not params else params.count(",") + 1
+    assert hdr == rs, (set(hdr) ^ set(rs), {k: (hdr[k], rs.get(k)) for k in hdr if hdr[k] != rs.get(k)})

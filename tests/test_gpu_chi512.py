@@ -1,0 +1,154 @@
+"""GPU parity at chi = 512 (the 100-qubit, depth-30 rotation + CZ brickwork), through the C ABI.
+
+* Saturated range-finder units on identical inputs: theta 1024 x 1024, ell = 528, the
+  reference's rank-48 sketch. Two units take their sites from the engine's own state after
+  layer 25 of the workload, one from synthetic random sites; the GPU split and the oracle's
+  rsvd_sites (exact-arithmetic rank rule, src/tn/mps.rs:983-1053) must keep the same bond and
+  agree on the gauge-invariant product left.right to 1e-9 relative.
+* The mixed form against the golden checkpoints: 1e-10 with identical bonds and counters
+  through layer 24 (exact splits only), then per layer on identical inputs (the plain FP64
+  form's state copied into the mixed state before each of layers 25-30): identical bonds and
+  the amplitude within 1e-9 relative of the FP64 form's on the same input.
+"""
+
+import json
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import circuits as C
+from oracle import oracle_mps as om
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def wt():
+    import wqc_tn
+
+    wqc_tn.load()
+    return wqc_tn
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def layers_of(gates, n):
+    out, i = [], 0
+    while i < len(gates):
+        j = i + n
+        while j < len(gates) and gates[j][0] in (C.CZ, C.CNOT):
+            j += 1
+        out.append(gates[i:j])
+        i = j
+    return out
+
+
+def boundary_sites(rng, dl, dr):
+    a = rng.standard_normal((1, 2, dl)) + 1j * rng.standard_normal((1, 2, dl))
+    b = rng.standard_normal((dr, 2, 1)) + 1j * rng.standard_normal((dr, 2, 1))
+    return a / np.linalg.norm(a), b / np.linalg.norm(b)
+
+
+def split_on_both(wt, s1, s2, rng):
+    """A 4-site chain [boundary, s1, s2, boundary] at chi = 512, CZ on sites 1, 2, on the GPU
+    and on the oracle; returns (gpu bond, oracle bond, relative product error, gpu path,
+    oracle path)."""
+    b0, b3 = boundary_sites(rng, s1.shape[0], s2.shape[2])
+    sites = [b0, s1, s2, b3]
+    g = wt.MpsState(4, 512)
+    g.enable_split_log()
+    o = om.OracleMps(4, 512, record=True)
+    for i, s in enumerate(sites):
+        g.set_site(i, s)
+        o.set_site(i, s)
+    gate = C.two(C.CZ, 1, 2)
+    g.apply_gate(gate)
+    o.apply_gate(gate)
+    kg, ko = g.bond_dims()[1], o.bond_dims()[1]
+    err = float("inf")
+    if kg == ko:
+        pg = g.site(1).reshape(-1, kg) @ g.site(2).reshape(kg, -1)
+        po = o.site(1).reshape(-1, ko) @ o.site(2).reshape(ko, -1)
+        err = float(np.max(np.abs(pg - po)) / np.max(np.abs(po)))
+    gpath = g.split_log()[0][3]
+    opath = o.split_log()[0][0]
+    g.free()
+    return kg, ko, err, gpath, opath
+
+
+def test_range_finder_chi512_units_vs_oracle(wt):
+    n, chi, gates = C.config3()
+    L = layers_of(gates, n)
+    st = wt.MpsState(n, chi)
+    for lay in L[:25]:
+        st.apply_circuit(lay)
+    b = [1] + st.bond_dims()
+    # sites p, p+1 with left bond, middle bond and right bond all 512: theta 1024 x 1024
+    sat = [p for p in range(n - 1) if b[p] == b[p + 1] == b[p + 2] == 512]
+    say("saturated positions after layer 25:", sat)
+    assert len(sat) >= 2
+    rng = np.random.default_rng(512)
+    cases = [(st.site(p), st.site(p + 1), f"layer-25 state, site {p}") for p in (sat[0], sat[len(sat) // 2])]
+    st.free()
+    s1 = rng.standard_normal((512, 2, 512)) + 1j * rng.standard_normal((512, 2, 512))
+    s2 = rng.standard_normal((512, 2, 512)) + 1j * rng.standard_normal((512, 2, 512))
+    s1 *= np.exp(-0.004 * np.arange(512))[None, None, :]
+    s2 *= np.exp(-0.004 * np.arange(512))[:, None, None]
+    cases.append((s1 / np.linalg.norm(s1), s2 / np.linalg.norm(s2), "synthetic graded sites"))
+    om.set_qr_rank_tol(1e-10)
+    try:
+        for s1, s2, label in cases:
+            kg, ko, err, gpath, opath = split_on_both(wt, s1, s2, rng)
+            say(label, "bond gpu", kg, "oracle", ko, "paths", gpath, opath, "product err %.2e" % err)
+            assert opath == 1 and gpath == 1, label
+            assert kg == ko, label
+            assert err <= 1e-9, label
+    finally:
+        om.set_qr_rank_tol(0.0)
+
+
+def test_mixed_form_config3(wt):
+    with open(os.path.join(GOLDEN, "config3.json")) as f:
+        want = json.load(f)
+    n, chi, gates = C.config3()
+    L = layers_of(gates, n)
+    mx = wt.MpsState(n, chi, wt.PRECISION_MIXED_FP32)
+    f64 = wt.MpsState(n, chi, wt.PRECISION_FP64)
+    for k, lay in enumerate(L[:24]):
+        mx.apply_circuit(lay)
+        cp = want["checkpoints"].get(str(k + 1))
+        if cp:
+            a, b = mx.amplitude_at_compact_zero(), complex(*cp["amplitude"])
+            st = mx.stats()
+            say("mixed layer", k + 1, "rel err %.2e" % rel(a, b))
+            assert mx.bond_dims() == cp["bonds"], k + 1
+            assert (st["two_qubit_units"], st["rsvd_kept"], st["rsvd_fallbacks"]) == (
+                cp["counters"]["units"], cp["counters"]["rsvd_kept"], cp["counters"]["rsvd_fallbacks"])
+            assert rel(a, b) <= 1e-10, k + 1
+    f64.copy_sites_from(mx)
+    worst = 0.0
+    for k in range(24, len(L)):
+        mx.copy_sites_from(f64)
+        mx.apply_circuit(L[k])
+        f64.apply_circuit(L[k])
+        a, b = mx.amplitude_at_compact_zero(), f64.amplitude_at_compact_zero()
+        e = rel(a, b)
+        worst = max(worst, e)
+        say("layer", k + 1, "identical input: mixed vs FP64 rel err %.2e" % e, "bonds equal", mx.bond_dims() == f64.bond_dims())
+        assert mx.bond_dims() == f64.bond_dims(), k + 1
+        assert e <= 1e-9, k + 1
+    st = mx.stats()
+    say("mixed: rsvd kept", st["rsvd_kept"], "fallbacks", st["rsvd_fallbacks"], "low sweeps", st["jacobi_sweeps_lo"])
+    assert st["jacobi_sweeps_lo"] > 0 and st["rsvd_kept"] > 0

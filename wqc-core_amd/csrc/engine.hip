@@ -451,6 +451,8 @@ struct wqc_tn_state {
         gather.release();
         ov_buf.release();
         env_buf.release();
+        probe_buf.release();
+        probe_p_buf.release();
         for (auto& sp : spans) {
             (void)hipEventDestroy(sp.a);
             (void)hipEventDestroy(sp.b);
@@ -462,8 +464,12 @@ struct wqc_tn_state {
     }
 
     // ------------------------------------------------------------ helpers
+    // Every device buffer the state owns (sites, workspaces, staging, readback and trace
+    // buffers), so peak_vram_bytes is the largest footprint the state has had.
     void note_peak() {
-        uint64_t now = site_bytes + ws.bytes + ws32.bytes + ews.bytes + tab.bytes + splitk_buf.bytes + gather.bytes;
+        uint64_t now = site_bytes + ws.bytes + ws32.bytes + ews.bytes + tab.bytes + splitk_buf.bytes + gather.bytes +
+                       jstats_buf.bytes + trace_buf.bytes + ov_buf.bytes + env_buf.bytes + probe_buf.bytes +
+                       probe_p_buf.bytes;
         if (now > stats.peak_vram_bytes) stats.peak_vram_bytes = now;
     }
 
@@ -683,11 +689,13 @@ struct wqc_tn_state {
     // Rotates the rows of every job's X until no block pair needs a rotation, then
     // writes the row norms into sigma (device, at sig_off[j]). Returns sweeps used.
     std::vector<int> last_sweeps, last_sweeps_lo;  // per job of the latest jacobi call
+    std::vector<char> last_unconverged;             // per job: still rotating at MAX_SWEEPS
 
     void jacobi(std::vector<JacobiJob>& jobs, double* d_sigma, const std::vector<long>& sig_off,
                 int* d_flags) {
         const int nj = (int)jobs.size();
         last_sweeps.assign(nj, 0);
+        last_unconverged.assign(nj, 0);
         if (!nj) return;
         JacobiJob* dj = upload(jobs);
         int max_n = 1, max_m = 1;
@@ -781,6 +789,13 @@ struct wqc_tn_state {
             // The rounds read the done flags from the uploaded table itself.
             if (changed && active > 0) d_done = upload_ints(done);
         }
+        // A job still rotating after MAX_SWEEPS has not converged: its rows are not orthogonal,
+        // so its row norms are not its singular values. The caller refuses its split.
+        for (int j = 0; j < nj; ++j)
+            if (!done[j]) {
+                last_unconverged[j] = 1;
+                stats.jacobi_unconverged++;
+            }
         if (trace_jacobi) {
             std::vector<int> order(nj);
             for (int j = 0; j < nj; ++j) order[j] = j;
@@ -1071,6 +1086,7 @@ struct wqc_tn_state {
         OmegaEntry* om;
         size_t chi_kept;
         bool fallback;
+        bool unconverged = false;  // its Jacobi reached MAX_SWEEPS still rotating
         int sweeps = 0, sweeps_lo = 0;
     };
 
@@ -1322,7 +1338,47 @@ struct wqc_tn_state {
         double* d_part = nullptr;
         std::vector<size_t> hoff;
         size_t hfro = 0;
+        // Column count of each job's matrix before any preconditioning, and where the two
+        // squared probe residuals of job k sit in the readback (hprobe + 2k, +1).
+        std::vector<int> m0;
+        size_t hprobe = 0;
     };
+
+    // The probe half of svd_reproduces (src/tn/svd_guard.rs:261-281): two fixed vectors
+    // x_p[j] = (sin t, cos t), t = (j + 1)(0.754877666 + 0.56984029 p), |x_p[j]| = 1, as the
+    // columns of P (m x 2 row-major). A factorization passes when |A x_p - U S V^H x_p| <=
+    // 1e-9 |A|_F |x_p| + 1e-14 for both, A being the matrix the Jacobi takes (after the dust
+    // rule: theta', its adjoint when tall, or the range finder's B).
+    DevBuf probe_buf, probe_p_buf;
+    int probe_m = 0;
+    double2* probe_matrix(int m) {
+        if (m > probe_m) {
+            const int cap = std::max(m, 1024);
+            std::vector<double2> P((size_t)cap * 2);
+            for (int j = 0; j < cap; ++j)
+                for (int p = 0; p < 2; ++p) {
+                    const double t = (j + 1.0) * (0.754877666 + p * 0.56984029);
+                    P[(size_t)j * 2 + p] = make_double2(std::sin(t), std::cos(t));
+                }
+            sync();
+            probe_p_buf.ensure(P.size() * sizeof(double2));
+            HIPC(hipMemcpy(probe_p_buf.p, P.data(), P.size() * sizeof(double2), hipMemcpyHostToDevice));
+            probe_m = cap;
+            note_peak();
+        }
+        return reinterpret_cast<double2*>(probe_p_buf.p);
+    }
+
+    // Whether job k of a finished batch passes the probe check (|A|_F^2 = fro2).
+    bool probes_pass(const SplitBatch& b, int k, double fro2) const {
+        const double scale = std::sqrt(std::max(fro2, 2.2250738585072014e-308));
+        const double xn = std::sqrt((double)b.m0[k]);
+        for (int p = 0; p < 2; ++p) {
+            const double err = std::sqrt(hread[b.hprobe + 2 * (size_t)k + p]);
+            if (!(err <= SVD_CHECK_TOL * scale * xn + 1e-14)) return false;
+        }
+        return true;
+    }
 
     // Exact split (truncated_svd_split): theta' (or its adjoint when tall) as the Jacobi rows,
     // the dust rule, J = I.
@@ -1604,6 +1660,40 @@ struct wqc_tn_state {
     // share the launches), then sigma and |A|_F^2 of every job read back with one sync.
     void jacobi_splits(std::vector<JobInfo>& J, const std::vector<SplitBatch*>& batches, char* base, int* d_flags) {
         auto PD = [&](size_t o) { return reinterpret_cast<double*>(base + o); };
+        // Probe products D = A P of every job, before any preconditioning or rotation touches A.
+        size_t njobs = 0, nrows = 0;
+        int mmax = 1;
+        for (auto* b : batches) {
+            b->m0.assign(b->ids.size(), 0);
+            for (size_t i = 0; i < b->ids.size(); ++i) {
+                b->m0[i] = b->jj[i].m;
+                mmax = std::max(mmax, b->jj[i].m);
+                nrows += b->jj[i].n;
+                ++njobs;
+            }
+        }
+        if (!njobs) return;
+        double2* Pm = probe_matrix(mmax);
+        probe_buf.ensure(3 * nrows * 2 * sizeof(double2) + 2 * njobs * sizeof(double) + 256);
+        note_peak();
+        double2* pd = reinterpret_cast<double2*>(probe_buf.p);
+        double2* pt = pd + nrows * 2;
+        double2* pr = pt + nrows * 2;
+        double* perr = reinterpret_cast<double*>(pr + nrows * 2);
+        std::vector<size_t> prow;  // first row of each job in the probe buffers
+        {
+            Timer tm(this, 4);
+            std::vector<GemmJob> g;
+            size_t at = 0;
+            for (auto* b : batches)
+                for (size_t i = 0; i < b->ids.size(); ++i) {
+                    const JacobiJob& x = b->jj[i];
+                    prow.push_back(at);
+                    g.push_back(gj(x.X, x.ldx, 0, Pm, 2, 0, pd + 2 * at, 2, x.n, 2, x.m, 1, 0));
+                    at += x.n;
+                }
+            gemm(g, 4);
+        }
         std::vector<wqc::LqJob> lq;
         std::vector<double2*> lqw;
         lq_precondition(J, batches, base, lq, lqw);
@@ -1616,6 +1706,32 @@ struct wqc_tn_state {
         if (all.empty()) return;
         jacobi_any(all, PD(0), offs, d_flags);
         lq_form_m(lq, lqw);
+        // Rebuilt probes U S V^H x = F^H (G x): (F, G) = (J, rotated rows) for a plain job
+        // (A = J^H X_final), (rotated rows of R, M) for an LQ-preconditioned one (A = Y^H M).
+        {
+            Timer tm(this, 4);
+            std::vector<GemmJob> g1, g2;
+            std::vector<wqc::ProbeJob> pj;
+            size_t t = 0;
+            for (auto* b : batches)
+                for (size_t i = 0; i < b->ids.size(); ++i, ++t) {
+                    const JacobiJob& x = b->jj[i];
+                    const bool pre = !b->pre.empty() && b->pre[i];
+                    const double2* F = pre ? x.X : x.J;
+                    const int ldf = pre ? x.ldx : x.n;
+                    const double2* G = pre ? b->orig[i] : x.X;
+                    const int ldg = pre ? b->orig_ld[i] : x.ldx;
+                    double2* tt = pt + 2 * prow[t];
+                    double2* rr = pr + 2 * prow[t];
+                    g1.push_back(gj(G, ldg, 0, Pm, 2, 0, tt, 2, x.n, 2, b->m0[i], 1, 0));
+                    g2.push_back(gj(F, ldf, 1, tt, 2, 0, rr, 2, x.n, 2, x.n, 1, 0));
+                    pj.push_back(wqc::ProbeJob{pd + 2 * prow[t], rr, perr + 2 * t, x.n, 0});
+                }
+            gemm(g1, 4);
+            gemm(g2, 4);
+            wqc::launch_probe_residual(upload(pj), (int)pj.size(), stream);
+            check_launch();
+        }
         if (trace_jacobi) {
             std::fprintf(stderr, "jacobi splits (site:rows x cols:sweeps):");
             size_t t = 0;
@@ -1626,7 +1742,7 @@ struct wqc_tn_state {
         }
         // sigma of every job and |A|_F^2 of every batch, gathered on the device into one
         // contiguous block and read back with one copy.
-        size_t total = 0;
+        size_t total = 2 * njobs;
         for (auto* b : batches) {
             for (size_t i = 0; i < b->ids.size(); ++i) total += b->jj[i].n;
             total += b->ids.size();
@@ -1644,6 +1760,7 @@ struct wqc_tn_state {
             for (size_t i = 0; i < b->ids.size(); ++i, ++k) {
                 J[b->ids[i]].sweeps += last_sweeps[k];
                 J[b->ids[i]].sweeps_lo += last_sweeps_lo[k];
+                J[b->ids[i]].unconverged = last_unconverged[k] != 0;
                 b->hoff[i] = cur;
                 cs.push_back(PD(0) + b->offs[i]);
                 cd.push_back(gd + cur);
@@ -1658,6 +1775,18 @@ struct wqc_tn_state {
             cd.push_back(gd + cur);
             cc.push_back((long)b->ids.size());
             cur += b->ids.size();
+        }
+        // The probe residuals of every job, in batch order.
+        {
+            size_t t0 = 0;
+            for (auto* b : batches) {
+                b->hprobe = cur + 2 * t0;
+                t0 += b->ids.size();
+            }
+            cs.push_back(perr);
+            cd.push_back(gd + cur);
+            cc.push_back((long)(2 * njobs));
+            cur += 2 * njobs;
         }
         copy_batch(cs, cd, cc);
         if (cur) HIPC(hipMemcpyAsync(hread, gd, cur * sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -1688,11 +1817,18 @@ struct wqc_tn_state {
                 sorted[i] = sig[order[i]];
                 s2 += sorted[i] * sorted[i];
             }
-            if (!(std::fabs(s2 - fro2) <= SVD_CHECK_TOL * std::max(fro2, 2.2250738585072014e-308) + 1e-28)) {
+            if (j.unconverged)
+                fail(WQC_TN_EXEC_FAILED, "SVD of a " + std::to_string(j.rows) + "x" + std::to_string(j.cols) +
+                                             " matrix did not converge in " + std::to_string(MAX_SWEEPS) +
+                                             " Jacobi sweeps");
+            const bool norm_ok =
+                std::fabs(s2 - fro2) <= SVD_CHECK_TOL * std::max(fro2, 2.2250738585072014e-308) + 1e-28;
+            if (!norm_ok || !probes_pass(b, k, fro2)) {
                 stats.svd_check_failures++;
                 char buf[256];
-                std::snprintf(buf, sizeof buf, " (sum s^2 = %.17g, |A|_F^2 = %.17g, job %d of %d, n=%d m=%d)", s2, fro2, k,
-                              ne, nn, jj[k].m);
+                std::snprintf(buf, sizeof buf, " (sum s^2 = %.17g, |A|_F^2 = %.17g, probe residuals %.3g %.3g, job %d of %d, n=%d m=%d)",
+                              s2, fro2, std::sqrt(hread[b.hprobe + 2 * (size_t)k]),
+                              std::sqrt(hread[b.hprobe + 2 * (size_t)k + 1]), k, ne, nn, b.m0[k]);
                 fail(WQC_TN_EXEC_FAILED, "SVD of a " + std::to_string(j.rows) + "x" + std::to_string(j.cols) +
                                              " matrix failed its check" + buf);
             }
@@ -1776,7 +1912,9 @@ struct wqc_tn_state {
                 values[i] = sig[order[i]];
                 s2 += values[i] * values[i];
             }
-            const bool svd_ok = std::fabs(s2 - fro2) <= SVD_CHECK_TOL * std::max(fro2, 2.2250738585072014e-308) + 1e-28;
+            const bool svd_ok = !j.unconverged &&
+                                std::fabs(s2 - fro2) <= SVD_CHECK_TOL * std::max(fro2, 2.2250738585072014e-308) + 1e-28 &&
+                                probes_pass(b, k, fro2);
             if (!svd_ok) stats.svd_check_failures++;
             for (double& s : values)
                 if (s < SINGULAR_CUTOFF * 1e-2) s = 0.0;
@@ -3384,6 +3522,28 @@ int wqc_tn_upload_site(wqc_tn_state* st, size_t site, const double* data, size_t
         HIPC(hipMemcpy(st->sites[site].ptr, data, l * 2 * r * sizeof(double2), hipMemcpyHostToDevice));
         st->sites[site].l = (int)l;
         st->sites[site].r = (int)r;
+    });
+}
+
+int wqc_tn_copy_sites(wqc_tn_state* dst, const wqc_tn_state* src) {
+    return guarded([&] {
+        if (!dst || !src) fail(WQC_TN_INVALID_ARG, "null state");
+        if (dst == src) return;
+        if (dst->dev != src->dev) fail(WQC_TN_INVALID_ARG, "states on different devices");
+        if (dst->n != src->n) fail(WQC_TN_INVALID_ARG, "mismatched register");
+        HIPC(hipSetDevice(dst->dev->ordinal));
+        dst->flush();
+        HIPC(hipStreamSynchronize(src->stream));
+        dst->sync();
+        for (int i = 0; i < dst->n; ++i) {
+            const auto& s = src->sites[i];
+            const size_t elems = (size_t)s.l * 2 * s.r;
+            dst->ensure_site_cap(i, elems);
+            HIPC(hipMemcpyAsync(dst->sites[i].ptr, s.ptr, elems * sizeof(double2), hipMemcpyDeviceToDevice, dst->stream));
+            dst->sites[i].l = s.l;
+            dst->sites[i].r = s.r;
+        }
+        dst->sync();
     });
 }
 

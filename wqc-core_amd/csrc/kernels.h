@@ -183,6 +183,15 @@ void launch_to_f64(const float2* const* src, double2* const* dst, const long* co
                    long max_count, hipStream_t s);
 // jobs[j].dst[0 .. n) = jobs[j].src[0 .. n) in doubles, one launch for the batch.
 void launch_copy_batch(const CopyJob* jobs, int njobs, long max_count, hipStream_t s);
+// The factorization check of a split: d and r are n x 2 row-major (the two probe columns of
+// A X and of the rebuilt factors); out[0..1] receives sum_i |d[i, p] - r[i, p]|^2 per column p.
+struct ProbeJob {
+    const double2* d;
+    const double2* r;
+    double* out;
+    int n, pad;
+};
+void launch_probe_residual(const ProbeJob* jobs, int njobs, hipStream_t s);
 void launch_row_norms(const JacobiJob* jobs, int njobs, int max_n, double* sigma,
                       const long* sigma_off, hipStream_t s);
 void launch_gemm(const GemmJob* jobs, int njobs, int max_m, int max_n, hipStream_t s);

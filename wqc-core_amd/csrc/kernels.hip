@@ -1114,6 +1114,31 @@ __global__ __launch_bounds__(256) void k_copy_batch(const CopyJob* __restrict__ 
     for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < jb.n; e += (long)gridDim.x * 256) jb.dst[e] = jb.src[e];
 }
 
+// Probe residuals of the split check: out[p] = sum_i |d[i, p] - r[i, p]|^2 for the two probe
+// columns p of the n x 2 row-major d and r, one workgroup per job, fixed summation order.
+__global__ __launch_bounds__(256) void k_probe_residual(const ProbeJob* __restrict__ jobs) {
+    const ProbeJob jb = jobs[blockIdx.x];
+    double acc[2] = {0.0, 0.0};
+    for (int i = threadIdx.x; i < jb.n; i += 256)
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const double2 e = csub(jb.d[2 * i + p], jb.r[2 * i + p]);
+            acc[p] = fma(e.x, e.x, acc[p]);
+            acc[p] = fma(e.y, e.y, acc[p]);
+        }
+    __shared__ double ws[2][4];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        const double v = wave_sum(acc[p]);
+        if ((threadIdx.x & 63) == 0) ws[p][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int p = threadIdx.x;
+        jb.out[p] = ((ws[p][0] + ws[p][1]) + ws[p][2]) + ws[p][3];
+    }
+}
+
 // sigma[row] = ||X[row, :]||, one wave per row.
 __global__ __launch_bounds__(256) void k_row_norms(const JacobiJob* __restrict__ jobs,
                                                    double* __restrict__ sigma,
@@ -2035,6 +2060,10 @@ void launch_to_f64(const float2* const* src, double2* const* dst, const long* co
 void launch_copy_batch(const CopyJob* jobs, int njobs, long max_count, hipStream_t s) {
     if (njobs <= 0) return;
     hipLaunchKernelGGL(k_copy_batch, dim3(blocks_for(max_count, 256, 2048), njobs), dim3(256), 0, s, jobs);
+}
+
+void launch_probe_residual(const ProbeJob* jobs, int njobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_probe_residual, dim3(njobs), dim3(256), 0, s, jobs);
 }
 
 void launch_row_norms(const JacobiJob* jobs, int njobs, int max_n, double* sigma,

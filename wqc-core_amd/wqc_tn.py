@@ -33,7 +33,7 @@ EXPORTED = (
     "wqc_tn_kernel_launches", "wqc_tn_enable_split_log", "wqc_tn_split_log",
     "wqc_tn_pauli_expectations", "wqc_tn_observable_expectation", "wqc_tn_sample_counts",
     "wqc_tn_run_slices", "wqc_tn_backward", "wqc_tn_overlap", "wqc_tn_contract_time_split",
-    "wqc_tn_site_amplitudes_batch", "wqc_tn_outcome_probabilities",
+    "wqc_tn_site_amplitudes_batch", "wqc_tn_outcome_probabilities", "wqc_tn_copy_sites",
 )
 
 
@@ -55,7 +55,7 @@ class Stats(ct.Structure):
                 ("peak_vram_bytes", ct.c_uint64), ("max_discarded_weight", ct.c_double),
                 ("sum_discarded_weight", ct.c_double), ("jacobi_inner_sweeps", ct.c_uint64),
                 ("jacobi_pair_solves", ct.c_uint64), ("jacobi_sweeps_lo", ct.c_uint64),
-                ("jacobi_stalled", ct.c_uint64)]
+                ("jacobi_stalled", ct.c_uint64), ("jacobi_unconverged", ct.c_uint64)]
 
 
 class WqcTnError(RuntimeError):
@@ -118,6 +118,7 @@ def load():
     L.wqc_tn_bond_dims.argtypes = [vp, ct.POINTER(sz), sz]
     L.wqc_tn_download_site.argtypes = [vp, sz, dp, sz, ct.POINTER(sz), ct.POINTER(sz)]
     L.wqc_tn_upload_site.argtypes = [vp, sz, dp, sz, sz]
+    L.wqc_tn_copy_sites.argtypes = [vp, vp]
     L.wqc_tn_backend_label.restype = ct.c_char_p
     L.wqc_tn_backend_label.argtypes = [vp]
     L.wqc_tn_peak_vram_bytes.restype = ct.c_uint64
@@ -403,6 +404,10 @@ class MpsState:
     def set_site(self, i: int, arr):
         a = np.ascontiguousarray(arr, dtype=np.complex128)
         _check(load().wqc_tn_upload_site(self._h, i, _dp(a.view(np.float64)), a.shape[0], a.shape[2]))
+
+    def copy_sites_from(self, src: "MpsState"):
+        """self.sites = src.sites (device to device)."""
+        _check(load().wqc_tn_copy_sites(self._h, src._h))
 
     def backend_label(self) -> str:
         return load().wqc_tn_backend_label(self._h).decode()
