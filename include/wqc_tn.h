@@ -109,9 +109,10 @@ typedef struct wqc_tn_stats {
     uint64_t jacobi_pair_solves;  /* block-pair eigensolves run                             */
     uint64_t jacobi_sweeps_lo;    /* low-precision sweeps of the mixed form                 */
     uint64_t jacobi_stalled;      /* Jacobi jobs stopped at their rounding floor            */
-    uint64_t jacobi_unconverged;  /* Jacobi jobs still rotating at the sweep cap (60): an
-                                     exact split then fails with EXEC_FAILED, a range-finder
-                                     split falls back to the exact split                     */
+    uint64_t jacobi_unconverged;  /* Jacobi jobs still rotating at the sweep cap (60); kept,
+                                     like the reference's Jacobi at its cap, only when the
+                                     split passes the norm and probe-vector checks of
+                                     svd_reproduces (src/tn/svd_guard.rs:242-283)             */
 } wqc_tn_stats;
 
 /* ---- errors ------------------------------------------------------------------ */

@@ -689,13 +689,11 @@ struct wqc_tn_state {
     // Rotates the rows of every job's X until no block pair needs a rotation, then
     // writes the row norms into sigma (device, at sig_off[j]). Returns sweeps used.
     std::vector<int> last_sweeps, last_sweeps_lo;  // per job of the latest jacobi call
-    std::vector<char> last_unconverged;             // per job: still rotating at MAX_SWEEPS
 
     void jacobi(std::vector<JacobiJob>& jobs, double* d_sigma, const std::vector<long>& sig_off,
                 int* d_flags) {
         const int nj = (int)jobs.size();
         last_sweeps.assign(nj, 0);
-        last_unconverged.assign(nj, 0);
         if (!nj) return;
         JacobiJob* dj = upload(jobs);
         int max_n = 1, max_m = 1;
@@ -789,13 +787,11 @@ struct wqc_tn_state {
             // The rounds read the done flags from the uploaded table itself.
             if (changed && active > 0) d_done = upload_ints(done);
         }
-        // A job still rotating after MAX_SWEEPS has not converged: its rows are not orthogonal,
-        // so its row norms are not its singular values. The caller refuses its split.
+        // A job still rotating after MAX_SWEEPS is counted; as the reference's Jacobi at its sweep
+        // cap (src/tn/svd_guard.rs:165-229, then svd_reproduces), its factorization is kept when
+        // it passes the norm and probe checks, which the caller applies to every split.
         for (int j = 0; j < nj; ++j)
-            if (!done[j]) {
-                last_unconverged[j] = 1;
-                stats.jacobi_unconverged++;
-            }
+            if (!done[j]) stats.jacobi_unconverged++;
         if (trace_jacobi) {
             std::vector<int> order(nj);
             for (int j = 0; j < nj; ++j) order[j] = j;
@@ -1086,7 +1082,6 @@ struct wqc_tn_state {
         OmegaEntry* om;
         size_t chi_kept;
         bool fallback;
-        bool unconverged = false;  // its Jacobi reached MAX_SWEEPS still rotating
         int sweeps = 0, sweeps_lo = 0;
     };
 
@@ -1760,7 +1755,6 @@ struct wqc_tn_state {
             for (size_t i = 0; i < b->ids.size(); ++i, ++k) {
                 J[b->ids[i]].sweeps += last_sweeps[k];
                 J[b->ids[i]].sweeps_lo += last_sweeps_lo[k];
-                J[b->ids[i]].unconverged = last_unconverged[k] != 0;
                 b->hoff[i] = cur;
                 cs.push_back(PD(0) + b->offs[i]);
                 cd.push_back(gd + cur);
@@ -1817,10 +1811,6 @@ struct wqc_tn_state {
                 sorted[i] = sig[order[i]];
                 s2 += sorted[i] * sorted[i];
             }
-            if (j.unconverged)
-                fail(WQC_TN_EXEC_FAILED, "SVD of a " + std::to_string(j.rows) + "x" + std::to_string(j.cols) +
-                                             " matrix did not converge in " + std::to_string(MAX_SWEEPS) +
-                                             " Jacobi sweeps");
             const bool norm_ok =
                 std::fabs(s2 - fro2) <= SVD_CHECK_TOL * std::max(fro2, 2.2250738585072014e-308) + 1e-28;
             if (!norm_ok || !probes_pass(b, k, fro2)) {
@@ -1912,8 +1902,7 @@ struct wqc_tn_state {
                 values[i] = sig[order[i]];
                 s2 += values[i] * values[i];
             }
-            const bool svd_ok = !j.unconverged &&
-                                std::fabs(s2 - fro2) <= SVD_CHECK_TOL * std::max(fro2, 2.2250738585072014e-308) + 1e-28 &&
+            const bool svd_ok = std::fabs(s2 - fro2) <= SVD_CHECK_TOL * std::max(fro2, 2.2250738585072014e-308) + 1e-28 &&
                                 probes_pass(b, k, fro2);
             if (!svd_ok) stats.svd_check_failures++;
             for (double& s : values)
