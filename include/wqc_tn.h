@@ -203,6 +203,13 @@ int wqc_tn_contract_time_split(wqc_tn_device *dev, size_t qubit_count, const wqc
 /* MpsState::amplitude_at_compact_zero (src/tn/mps.rs:262-276): <0...0|psi>, FP64. */
 int wqc_tn_amplitude_at_compact_zero(wqc_tn_state *st, double out[2]);
 
+/* <0...0|psi> as wqc_tn_amplitude_at_compact_zero (same kernels, same result), allocating
+ * nothing. No counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops). Its vectors live in a fixed 2 MiB device
+ * array per GPU, enough for every bond up to 1024 (a larger bond -> INVALID_ARG). Every call on
+ * one GPU shares that array, so calls that overlap on one GPU race. */
+int wqc_tn_amplitude_at_compact_zero_noalloc(wqc_tn_state *st, double out[2]);
+
 /* MpsState::site_amplitudes (src/tn/mps.rs:679-693): out = {a0.re, a0.im, a1.re, a1.im}. */
 int wqc_tn_site_amplitudes(wqc_tn_state *st, size_t target, double out[4]);
 /* MpsState::control_probability (src/tn/mps.rs:695-698). */

@@ -300,10 +300,19 @@ struct OmegaEntry {
 
 }  // namespace
 
+// ---------------------------------------------------------------- allocation-free scratch
+// Device memory of wqc_tn_amplitude_at_compact_zero_noalloc: its two chain vectors and the left
+// pass's row-chunk partials. Every GPU holds its own copy for the life of the process; host code
+// never uses this name as an address, only what hipGetSymbolAddress returns for the current
+// device (kept per device in wqc_tn_device::scratch). Calls that overlap on one GPU race on it.
+constexpr size_t kScratchElems = size_t(1) << 17;  // 2 MiB of complex128
+static __device__ __align__(256) double2 scratch[kScratchElems];
+
 // ---------------------------------------------------------------- device handle
 struct wqc_tn_device {
     int ordinal = 0;
     hipDeviceProp_t prop{};
+    double2* scratch = nullptr;  // this device's address of the scratch array
     std::mutex mu;
     std::map<std::tuple<size_t, size_t, size_t>, OmegaEntry> omega;
     ~wqc_tn_device() {
@@ -2246,6 +2255,34 @@ struct wqc_tn_state {
         (void)d;
         return {out.x, out.y};
     }
+    // amplitude_zero with the chain vectors and partials in the device's scratch array instead of
+    // the state's workspace: nothing is allocated (no device buffer grows, no event is created).
+    std::pair<double, double> amplitude_zero_noalloc() {
+        if (!pend1.empty() || !pend2.empty()) fail(WQC_TN_EXEC_FAILED, "state has unapplied gates");
+        double2* base = dev->scratch;
+        if (!base) fail(WQC_TN_EXEC_FAILED, "device scratch not set up");
+        size_t mx = 2;
+        for (auto& s : sites) mx = std::max(mx, (size_t)std::max(s.l, s.r) + 2);
+        const size_t need = 2 * mx + (mx / wqc::kVecMatChunk + 2) * mx;
+        if (need > kScratchElems)
+            fail(WQC_TN_INVALID_ARG, "bond dimension " + std::to_string(mx - 2) +
+                                         " too large for the allocation-free readout: it needs " +
+                                         std::to_string(need * sizeof(double2)) + " bytes of scratch, the array holds " +
+                                         std::to_string(kScratchElems * sizeof(double2)) + " (bonds up to 1024 fit)");
+        sync();
+        tables_reset();
+        double2* saved = vm_part;
+        vm_part = base + 2 * mx;
+        int len = 0;
+        double2* v = env_pass(0, n, 0, false, len, base, base + mx);
+        vm_part = saved;
+        klaunch[5] += n;
+        double2 out;
+        HIPC(hipMemcpyAsync(&out, v, sizeof(double2), hipMemcpyDeviceToHost, stream));
+        sync();
+        return {out.x, out.y};
+    }
+
     std::pair<double, double> amplitude_zero_timed() {
         auto r = amplitude_zero();
         resolve_timings();
@@ -2859,6 +2896,11 @@ int wqc_tn_device_open(int ordinal, wqc_tn_device** out) {
         HIPC(hipGetDeviceProperties(&d->prop, ordinal));
         if (std::strncmp(d->prop.gcnArchName, "gfx950", 6) != 0)
             fail(WQC_TN_NO_DEVICE, std::string("device is ") + d->prop.gcnArchName + ", this build targets gfx950");
+        // The scratch array's address on this device (each device has its own copy).
+        void* sp = nullptr;
+        HIPC(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+        if (!sp) fail(WQC_TN_EXEC_FAILED, "scratch array has no device address");
+        d->scratch = static_cast<double2*>(sp);
         *out = d.release();
     });
 }
@@ -3088,6 +3130,16 @@ int wqc_tn_amplitude_at_compact_zero(wqc_tn_state* st, double out[2]) {
     return guarded([&] {
         if (!st || !out) fail(WQC_TN_INVALID_ARG, "null argument");
         auto a = st->amplitude_zero_timed();
+        out[0] = a.first;
+        out[1] = a.second;
+    });
+}
+
+int wqc_tn_amplitude_at_compact_zero_noalloc(wqc_tn_state* st, double out[2]) {
+    return guarded([&] {
+        if (!st || !out) fail(WQC_TN_INVALID_ARG, "null argument");
+        HIPC(hipSetDevice(st->dev->ordinal));
+        auto a = st->amplitude_zero_noalloc();
         out[0] = a.first;
         out[1] = a.second;
     });

@@ -34,6 +34,7 @@ EXPORTED = (
     "wqc_tn_pauli_expectations", "wqc_tn_observable_expectation", "wqc_tn_sample_counts",
     "wqc_tn_run_slices", "wqc_tn_backward", "wqc_tn_overlap", "wqc_tn_contract_time_split",
     "wqc_tn_site_amplitudes_batch", "wqc_tn_outcome_probabilities", "wqc_tn_copy_sites",
+    "wqc_tn_amplitude_at_compact_zero_noalloc",
 )
 
 
@@ -99,6 +100,7 @@ def load():
     L.wqc_tn_request_cancel.argtypes = [vp]
     L.wqc_tn_clear_cancel.argtypes = [vp]
     L.wqc_tn_amplitude_at_compact_zero.argtypes = [vp, dp]
+    L.wqc_tn_amplitude_at_compact_zero_noalloc.argtypes = [vp, dp]
     L.wqc_tn_site_amplitudes.argtypes = [vp, sz, dp]
     L.wqc_tn_control_probability.argtypes = [vp, sz, dp]
     L.wqc_tn_contract_to_statevector.argtypes = [vp, dp, sz]
@@ -301,6 +303,11 @@ class MpsState:
     def amplitude_at_compact_zero(self) -> complex:
         out = np.zeros(2)
         _check(load().wqc_tn_amplitude_at_compact_zero(self._h, _dp(out)))
+        return complex(out[0], out[1])
+
+    def amplitude_at_compact_zero_noalloc(self) -> complex:
+        out = np.zeros(2)
+        _check(load().wqc_tn_amplitude_at_compact_zero_noalloc(self._h, _dp(out)))
         return complex(out[0], out[1])
 
     def overlap(self, bra: "MpsState") -> complex:
