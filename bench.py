@@ -8,9 +8,15 @@ the final <0...0|psi>; inputs (the gate list) are on the host, every site tensor
 HBM for the whole step.
 
 Printed JSON line: value = merges/s of the mixed form (Householder-LQ-preconditioned Jacobi:
-up to 6 FP32 sweeps, FP64 finish); the plain FP64 form's rate, their ratio, the relative amplitude error between the
-forms (after the range-finder-free first 24 layers and at the end), a roofline record for
-the dominant kernel and a CPU baseline ride along. `--gpus N` with N > 1 exits: this build runs on one GPU.
+up to 5 FP32 sweeps, FP64 finish). Riding along: the plain FP64 form's rate and their ratio;
+per form the amplitude error against the complex128 oracle after the range-finder-free first
+24 layers (the golden checkpoint), and for layers 25-30 the error of each layer on identical
+inputs (the FP64 form's state copied into the form's state before the layer: amplitude and
+per-split discarded weight against the FP64 form, whose splits match the oracle's to 1e-13);
+the roofline record of the dominant kernel family (Jacobi) and of the one-qubit kernel (GB/s);
+the GPU rate on a batch of saturated chi = 512 units taken from the workload's own state; and
+the CPU baseline: the FP64 oracle on the same saturated units, one thread and all host cores.
+`--gpus N` with N > 1 exits: this build runs on one GPU.
 """
 
 from __future__ import annotations
@@ -69,6 +75,13 @@ def layers_of(gates, n):
     return out
 
 
+def one_qubit_bytes(bonds_before, n):
+    """HBM bytes of a layer's one-qubit updates, one per site: read and write of every site
+    (l, 2, r) in complex128, 2 * 16 * 2 * l * r."""
+    b = [1] + list(bonds_before)
+    return sum(64.0 * b[q] * b[q + 1] for q in range(n))
+
+
 def run_gpu(wt, n, chi, gates, precision, steps, warmup, prefix_layers=24):
     amps = []
     for _ in range(warmup):
@@ -92,7 +105,9 @@ def run_gpu(wt, n, chi, gates, precision, steps, warmup, prefix_layers=24):
     st.enable_split_log(True)
     t1 = time.perf_counter()
     layers = layers_of(gates, n)
+    info["one_qubit_bytes"] = 0.0
     for L, lay in enumerate(layers):
+        info["one_qubit_bytes"] += one_qubit_bytes(st.bond_dims(), n)
         st.apply_circuit(lay)
         if L + 1 == prefix_layers:
             info["prefix_amplitude"] = st.amplitude_at_compact_zero()
@@ -125,33 +140,134 @@ def rel(a, b):
     return abs(a - b) / abs(b) if b != 0 else float("inf")
 
 
-def cpu_baseline(sample_units: int = 1):
-    """The FP64 oracle on the host cores: saturated chi=512 units (bonds 512 on both sides,
-    theta 1024 x 1024, range finder with ell = 528) on synthetic random sites."""
+def saturated_units(wt, n, chi, gates, count, layers=26):
+    """(left site, right site) of `count` saturated units of the workload's own state after
+    `layers` layers (plain FP64 form): positions whose left, middle and right bonds are all chi,
+    so theta is 2chi x 2chi and the split takes the range finder (ell = chi + 16)."""
+    st = wt.MpsState(n, chi)
+    for lay in layers_of(gates, n)[:layers]:
+        st.apply_circuit(lay)
+    b = [1] + st.bond_dims()
+    sat = [p for p in range(n - 1) if b[p] == b[p + 1] == b[p + 2] == chi]
+    picks = [sat[(k * len(sat)) // count] for k in range(count)] if len(sat) >= count else \
+        [sat[k % len(sat)] for k in range(count)]
+    pairs = [(st.site(p), st.site(p + 1)) for p in picks]
+    st.free()
+    return pairs, sat
+
+
+def unit_chain(pairs, chi):
+    """A register holding the units side by side: boundary (1, 2, chi), A_1, B_1, ..., A_K, B_K,
+    boundary (chi, 2, 1), and the disjoint layer of CZs on (A_k, B_k)."""
     import numpy as np
 
+    rng = np.random.default_rng(2026)
+    b0 = rng.standard_normal((1, 2, chi)) + 1j * rng.standard_normal((1, 2, chi))
+    b1 = rng.standard_normal((chi, 2, 1)) + 1j * rng.standard_normal((chi, 2, 1))
+    sites = [b0 / np.linalg.norm(b0)]
+    for a, b in pairs:
+        sites += [a, b]
+    sites.append(b1 / np.linalg.norm(b1))
+    layer = [C.two(C.CZ, 2 * k + 1, 2 * k + 2) for k in range(len(pairs))]
+    return sites, layer
+
+
+def gpu_saturated_rate(wt, pairs, chi, precision, repeats=3):
+    """Units/s of the GPU on one batch of the saturated units (sites uploaded untimed)."""
+    sites, layer = unit_chain(pairs, chi)
+    best = None
+    for _ in range(repeats):
+        st = wt.MpsState(len(sites), chi, precision)
+        for i, s in enumerate(sites):
+            st.set_site(i, s)
+        t0 = time.perf_counter()
+        st.apply_adjacent_disjoint_gates(layer)
+        dt = time.perf_counter() - t0
+        st.free()
+        best = dt if best is None else min(best, dt)
+    return {"units": len(layer), "seconds": best, "units_per_s": len(layer) / best}
+
+
+def host_info():
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"nproc": aff, "cpu_count": os.cpu_count(), "model": model}
+
+
+def cpu_baseline(pairs, chi, one_thread_units=2, all_cores=False):
+    """The FP64 oracle on the host cores, on the saturated units of the workload's own state
+    (theta 1024 x 1024, the reference's range finder, ell = 528): one thread on
+    `one_thread_units` of them (about 13 s per unit). With all_cores, also one unit per core on
+    all host cores (at most 16, the box's CPU share) in one disjoint layer, the oracle running a
+    layer's jobs in parallel; that leg takes about two minutes (16 units on 16 cores in 126 s:
+    its memory-bound loops scale poorly), so it is off by default."""
     from oracle import oracle_mps as om
 
-    om.set_threads(1)
-    rng = np.random.default_rng(2024)
-    b = 512
-    sites = []
-    for l, r in [(1, b), (b, b), (b, b), (b, 1)]:
-        t = rng.standard_normal((l, 2, r)) + 1j * rng.standard_normal((l, 2, r))
-        t *= np.exp(-0.004 * np.arange(r))[None, None, :]
-        t /= np.linalg.norm(t) / math.sqrt(r)
-        sites.append(t)
-    t0 = time.perf_counter()
-    for _ in range(sample_units):
-        o = om.OracleMps(4, 512)
+    host = host_info()
+    cores = max(1, min(16, host["nproc"]))
+
+    def timed(sub, threads):
+        om.set_threads(threads)
+        sites, layer = unit_chain(sub, chi)
+        o = om.OracleMps(len(sites), chi)
         for i, s in enumerate(sites):
             o.set_site(i, s)
-        o.apply_gate(C.two(C.CZ, 1, 2))
-    dt = time.perf_counter() - t0
-    return {"value": sample_units / dt, "unit": "merges/s", "cores": 1, "kind": "port",
-            "sample": f"{sample_units} saturated chi=512 unit(s): theta 1024x1024, range finder "
-                      f"ell=528, synthetic sites; FP64 oracle, 1 thread ({platform.processor() or 'x86_64'})",
-            "seconds": dt}
+        t0 = time.perf_counter()
+        o.apply_adjacent_disjoint_gates(layer)
+        return len(layer), time.perf_counter() - t0
+
+    u1, t1 = timed(pairs[:one_thread_units], 1)
+    out = {"value": u1 / t1, "unit": "merges/s", "cores": 1, "kind": "port",
+           "sample": f"{u1} saturated chi=512 units of the workload's own state after layer 26 "
+                     f"(theta 1024x1024, range finder ell=528) as one disjoint layer, 1 thread; "
+                     f"FP64 oracle ({host['model']}, nproc {host['nproc']})",
+           "seconds": t1, "host": host}
+    if all_cores:
+        ua, ta = timed(pairs[:cores], cores)
+        out["all_cores"] = {"value": ua / ta, "units": ua, "seconds": ta, "cores": cores}
+    return out
+
+
+def per_layer_errors(wt, n, chi, gates, forms, first=24):
+    """For layers first+1 .. end: every form applies the layer to the plain FP64 form's state
+    (copied device to device), and its amplitude and per-split discarded weights are compared
+    with the FP64 form's on the same input."""
+    layers = layers_of(gates, n)
+    f = wt.MpsState(n, chi, wt.PRECISION_FP64)
+    for lay in layers[:first]:
+        f.apply_circuit(lay)
+    out = {name: [] for name, _ in forms}
+    for k in range(first, len(layers)):
+        got = {}
+        for name, prec in forms:
+            m = wt.MpsState(n, chi, prec)
+            m.copy_sites_from(f)
+            m.enable_split_log(True)
+            m.apply_circuit(layers[k])
+            got[name] = (m.amplitude_at_compact_zero(), m.split_discarded(), m.bond_dims())
+            m.free()
+        f.enable_split_log(False)
+        f.enable_split_log(True)
+        f.apply_circuit(layers[k])
+        af, dwf, bf = f.amplitude_at_compact_zero(), f.split_discarded(), f.bond_dims()
+        for name, _ in forms:
+            a, dw, b = got[name]
+            dmax = max([abs(x - y) for x, y in zip(dw, dwf)] or [0.0])
+            rels = [abs(x - y) / y for x, y in zip(dw, dwf) if y > 1e-14]
+            out[name].append({"layer": k + 1, "rel_amp_error": rel(a, af),
+                              "max_abs_discarded_weight_error": dmax,
+                              "max_rel_discarded_weight_error": max(rels) if rels else 0.0,
+                              "bonds_differing": sum(1 for x, y in zip(b, bf) if x != y)})
+    f.free()
+    return out
 
 
 def main():
@@ -162,7 +278,10 @@ def main():
     ap.add_argument("--depth", type=int, default=30)
     ap.add_argument("--qubits", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-units", type=int, default=1)
+    ap.add_argument("--cpu-units", type=int, default=2, help="saturated units of the one-thread CPU leg")
+    ap.add_argument("--sat-units", type=int, default=16, help="saturated units of the GPU batch")
+    ap.add_argument("--cpu-all-cores", action="store_true",
+                    help="add the all-core CPU leg (one saturated unit per core, about two minutes)")
     ap.add_argument("--form", default="both", choices=["both", "fp64", "mixed"])
     ap.add_argument("--profile-pass", action="store_true",
                     help="run exactly one pass per form (for rocprofv3 counter runs), no JSON line")
@@ -232,14 +351,37 @@ def main():
         "avg_launch_us": 1e3 * jac_ms / max(1, launches.get("jacobi", 1)),
         "dominant_family": dom,
     }
+    golden24 = None
+    gpath = os.path.join(ROOT, "tests", "golden", "config3.json")
+    if n == 100 and args.depth == 30 and os.path.exists(gpath):
+        with open(gpath) as fh:
+            golden24 = complex(*json.load(fh)["checkpoints"]["24"]["amplitude"])
+    prec_of = {"fp64": wt.PRECISION_FP64, "mixed": wt.PRECISION_MIXED_FP32}
+    low_forms = [(k, prec_of[k]) for k in forms if k != "fp64"]
+    print("# per-layer errors on identical inputs", file=sys.stderr, flush=True)
+    per_layer = per_layer_errors(wt, n, chi, gates, low_forms) if low_forms and args.depth > 24 else {}
+    print("# saturated units", file=sys.stderr, flush=True)
+    pairs, sat_positions = saturated_units(wt, n, chi, gates, max(args.sat_units, 16)) if n >= 40 else ([], [])
     form_report = {}
     for name, r in forms.items():
+        kq = r["kernel_ms"].get("one_qubit", 0.0)
+        q_gbs = r["one_qubit_bytes"] / (kq * 1e-3) / 1e9 if kq > 0 else None
         form_report[name] = {
             "merges_per_s": units * args.steps / r["seconds"],
             "amplitude": [r["amplitude"].real, r["amplitude"].imag],
             "prefix24_amplitude": [r["prefix_amplitude"].real, r["prefix_amplitude"].imag],
+            "rel_amp_error_vs_oracle_layer24": rel(r["prefix_amplitude"], golden24) if golden24 else None,
             "alg_tflops_whole_step": sum(unit_flops(*e[:6]) for e in r["split_log"]) / r["timed_pass_seconds"] / 1e12,
+            "one_qubit": {"bound": "hbm", "achieved": q_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": q_gbs / HBM_PEAK_GBS if q_gbs else None,
+                          "bytes_per_pass": r["one_qubit_bytes"], "kernel_ms": kq},
             "kernel_ms": r["kernel_ms"], "launches": r["launches"], "stats": r["stats"]}
+        if name in per_layer:
+            pl = per_layer[name]
+            form_report[name]["identical_input_layers"] = pl
+            form_report[name]["max_rel_amp_error_per_layer"] = max(e["rel_amp_error"] for e in pl)
+        if pairs:
+            form_report[name]["saturated_batch"] = gpu_saturated_rate(wt, pairs[:args.sat_units], chi, prec_of[name])
     out = {
         "metric": "2q gate-merges/s at chi=512 (mixed vs FP64 plain) + rel. amp error vs FP64",
         "value": value, "unit": "merges/s", "n_gpus": 1, "steps": args.steps,
@@ -259,8 +401,9 @@ def main():
             "after_layer_24_exact_splits_only": rel(m["prefix_amplitude"], f["prefix_amplitude"]),
             "final_after_range_finder_layers": rel(m["amplitude"], f["amplitude"]),
         }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_units)
+    if not args.no_cpu_baseline and pairs:
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(pairs, chi, args.cpu_units, args.cpu_all_cores)
     print(json.dumps(out), flush=True)
 
 

@@ -338,6 +338,11 @@ int wqc_tn_kernel_launches(const wqc_tn_state *st, uint64_t *count, size_t len);
  * returns the number recorded. */
 int wqc_tn_enable_split_log(wqc_tn_state *st, int enable);
 size_t wqc_tn_split_log(const wqc_tn_state *st, int32_t *out, size_t cap_entries);
+/* The discarded weight sum_{k >= chi} s_k^2 / sum_k s_k^2 of each logged unit, in split-log order
+ * (the range finder's B spectrum where that path kept its split): the per-bond diagnostic of the
+ * truncation, no counterpart in the reference. Copies up to cap entries, returns the number
+ * recorded. */
+size_t wqc_tn_split_discarded(const wqc_tn_state *st, double *out, size_t cap_entries);
 
 #ifdef __cplusplus
 }

@@ -441,6 +441,7 @@ struct wqc_tn_state {
     std::vector<hipEvent_t> event_pool;
     bool split_log_on = false;
     std::vector<int32_t> split_log;
+    std::vector<double> split_dw;  // discarded weight of each logged unit
     std::vector<PendingOne> pend1;
     std::vector<Unit> pend2;
     std::vector<char> touched;
@@ -601,6 +602,7 @@ struct wqc_tn_state {
         pend1.clear();
         pend2.clear();
         split_log.clear();
+        split_dw.clear();
         stats = wqc_tn_stats{};
         cancel.store(0);
         reject = false;
@@ -1092,6 +1094,7 @@ struct wqc_tn_state {
         size_t chi_kept;
         bool fallback;
         int sweeps = 0, sweeps_lo = 0;
+        double dw = 0.0;  // discarded weight sum_{k >= chi} s_k^2 / sum s_k^2 of the split
     };
 
     void run_units(std::vector<Unit>& units) {
@@ -1222,6 +1225,7 @@ struct wqc_tn_state {
                 const int32_t e[8] = {j.rows, j.cols, j.bond, path, (int32_t)j.ell, (int32_t)j.chi_kept,
                                       j.sweeps, j.sweeps_lo};
                 split_log.insert(split_log.end(), e, e + 8);
+                split_dw.push_back(j.dw);
             }
     }
 
@@ -1846,6 +1850,7 @@ struct wqc_tn_state {
             if (reject && dropped) fail(WQC_TN_BOND_EXCEEDED, "mps bond exceeded chi");
             if (dropped) stats.truncated_splits++;
             const double dw = s2 > 0 ? drop / s2 : 0.0;
+            j.dw = dw;
             stats.max_discarded_weight = std::max(stats.max_discarded_weight, dw);
             stats.sum_discarded_weight += dw;
             stats.exact_splits++;
@@ -1942,6 +1947,7 @@ struct wqc_tn_state {
             }
             if (dropped) stats.truncated_splits++;
             const double dw = tot > 0 ? drop / tot : 0.0;
+            j.dw = dw;
             stats.max_discarded_weight = std::max(stats.max_discarded_weight, dw);
             stats.sum_discarded_weight += dw;
             perms[k].assign(order.begin(), order.begin() + c);
@@ -3626,7 +3632,10 @@ int wqc_tn_kernel_launches(const wqc_tn_state* st, uint64_t* count, size_t len) 
 int wqc_tn_enable_split_log(wqc_tn_state* st, int enable) {
     if (!st) return WQC_TN_INVALID_ARG;
     st->split_log_on = enable != 0;
-    if (!enable) st->split_log.clear();
+    if (!enable) {
+        st->split_log.clear();
+        st->split_dw.clear();
+    }
     return WQC_TN_OK;
 }
 
@@ -3637,6 +3646,13 @@ size_t wqc_tn_split_log(const wqc_tn_state* st, int32_t* out, size_t cap_entries
         const size_t k = std::min(m, cap_entries);
         std::memcpy(out, st->split_log.data(), k * 8 * sizeof(int32_t));
     }
+    return m;
+}
+
+size_t wqc_tn_split_discarded(const wqc_tn_state* st, double* out, size_t cap_entries) {
+    if (!st) return 0;
+    const size_t m = st->split_dw.size();
+    if (out) std::memcpy(out, st->split_dw.data(), std::min(m, cap_entries) * sizeof(double));
     return m;
 }
 

@@ -34,7 +34,7 @@ EXPORTED = (
     "wqc_tn_pauli_expectations", "wqc_tn_observable_expectation", "wqc_tn_sample_counts",
     "wqc_tn_run_slices", "wqc_tn_backward", "wqc_tn_overlap", "wqc_tn_contract_time_split",
     "wqc_tn_site_amplitudes_batch", "wqc_tn_outcome_probabilities", "wqc_tn_copy_sites",
-    "wqc_tn_amplitude_at_compact_zero_noalloc",
+    "wqc_tn_amplitude_at_compact_zero_noalloc", "wqc_tn_split_discarded",
 )
 
 
@@ -132,6 +132,8 @@ def load():
     L.wqc_tn_enable_split_log.argtypes = [vp, i32]
     L.wqc_tn_split_log.restype = sz
     L.wqc_tn_split_log.argtypes = [vp, ct.POINTER(ct.c_int32), sz]
+    L.wqc_tn_split_discarded.restype = sz
+    L.wqc_tn_split_discarded.argtypes = [vp, dp, sz]
     _lib = L
     return L
 
@@ -454,6 +456,13 @@ class MpsState:
         load().wqc_tn_split_log(self._h, buf, m)
         a = np.frombuffer(buf, dtype=np.int32)[: 8 * m].reshape(m, 8)
         return [tuple(int(x) for x in row) for row in a]
+
+    def split_discarded(self):
+        """Discarded weight of each unit in split-log order."""
+        m = load().wqc_tn_split_discarded(self._h, None, 0)
+        out = np.zeros(max(1, m))
+        load().wqc_tn_split_discarded(self._h, _dp(out), m)
+        return out[:m].tolist()
 
 
 def backend_from_env() -> str:
