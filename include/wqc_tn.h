@@ -253,11 +253,15 @@ int wqc_tn_backward(wqc_tn_device *dev, size_t qubit_count, size_t max_bond_dim,
                     size_t adjoint_bond, double *grad, double value[2]);
 
 /* A batch of independent slices, contract_slice (src/tn/contract.rs:10-30) per slice: new
- * register, the slice's gates, <0...0|psi> into out_amps[2i], out_amps[2i+1]. Slices run
- * concurrently on `workers` host threads (<= 0: 8), each driving one state on its own stream,
- * largest estimated cost first. out_status[i] is the slice's status code (its amplitude is NaN
- * when it is not OK); the call returns EXEC_FAILED naming the first failed slice, OK when all
- * succeeded. */
+ * register, the slice's gates, <0...0|psi> into out_amps[2i], out_amps[2i+1].
+ * workers == 0: one batch. Every slice's gates are scheduled into layers of site-disjoint
+ * operations, and step t runs layer t of all slices together: one one-qubit launch and shared
+ * two-site batches (merge, Jacobi rounds, factors over units of many slices of different widths
+ * and bond caps). Should a batch fail, the slices are rerun one by one for per-slice statuses.
+ * workers > 0: that many host threads, each driving one state on its own stream, largest
+ * estimated cost first (workers < 0: 8 threads).
+ * out_status[i] is the slice's status code (its amplitude is NaN when it is not OK); the call
+ * returns EXEC_FAILED naming the first failed slice, OK when all succeeded. */
 int wqc_tn_run_slices(wqc_tn_device *dev, const wqc_tn_slice *slices, size_t n_slices,
                       const wqc_tn_gate *gates, int precision, int workers, double *out_amps,
                       int *out_status);

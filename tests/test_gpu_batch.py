@@ -2,14 +2,20 @@
 {16, 32, 64, 128, 256}, config-1 gate set) through wqc_tn_run_slices.
 
 Bars:
-- every slice runs (status OK, finite amplitude);
-- the concurrent batch is bitwise equal to running the same slices one at a time on a
-  single state (determinism under concurrency);
-- a cost-bounded subsample against the oracle: 1e-10 relative where the oracle took no
-  range-finder split; where it did, identical bond dimensions (the range finder amplifies
-  rounding in any implementation, see test_oracle.py::test_range_finder_amplifies_rounding).
+- the batched form (workers = 0: every layer step of all slices in shared launches) and the
+  threaded form (8 host threads, one state per slice) run every slice (status OK, finite
+  amplitude);
+- both agree with one-at-a-time runs on one state: to 1e-12 relative on every slice whose
+  splits were all exact (the forms differ only in the summation order of a few batched
+  reductions), and with identical bonds everywhere (checked on the one-at-a-time states);
+- against the oracle: 1e-10 relative on range-finder-free slices (at least 16 of them), and
+  identical bonds and split counters on at least four chi = 128 and four chi = 256 slices whose
+  splits take the range finder (the range finder amplifies rounding in any implementation, see
+  test_oracle.py::test_range_finder_amplifies_rounding, so amplitudes there are compared only
+  as printed figures).
 """
 
+import os
 import sys
 import time
 
@@ -35,25 +41,54 @@ def say(*a):
     sys.stdout.flush()
 
 
-def test_config4_batch(wt):
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def one_at_a_time(wt, n, chi, gates):
+    st = wt.MpsState(n, chi)
+    st.enable_split_log(True)
+    st.apply_circuit(gates)
+    a = st.amplitude_at_compact_zero()
+    bonds = st.bond_dims()
+    paths = [e[3] for e in st.split_log()]
+    st.free()
+    return a, bonds, paths
+
+
+def test_config4_batched_and_threaded(wt):
     slices = C.config4()
     assert len(slices) == 256
     t = time.time()
-    amps, codes = wt.run_slices(slices, workers=8)
-    say("config4: 256 slices in %.2f s" % (time.time() - t))
-    assert all(c == wt.OK for c in codes)
-    assert np.all(np.isfinite(amps))
-    # Bitwise equality with one-at-a-time runs on a subsample (every 8th slice).
-    for i in range(0, 256, 8):
+    amps_b, codes_b = wt.run_slices(slices, workers=0)
+    tb = time.time() - t
+    t = time.time()
+    amps_t, codes_t = wt.run_slices(slices, workers=8)
+    tt = time.time() - t
+    say("config4: 256 slices batched %.2f s, threaded (8 workers) %.2f s" % (tb, tt))
+    assert all(c == wt.OK for c in codes_b) and all(c == wt.OK for c in codes_t)
+    assert np.all(np.isfinite(amps_b)) and np.all(np.isfinite(amps_t))
+    worst, exact = 0.0, 0
+    for i in range(0, 256, 4):
         n, chi, gates = slices[i]
-        st = wt.MpsState(n, chi)
-        st.apply_circuit(gates)
-        a = st.amplitude_at_compact_zero()
-        st.free()
-        assert a == amps[i], (i, a, amps[i])
-    # Oracle subsample: slices whose two-qubit work is small enough for the CPU oracle.
-    om.set_threads(8)
-    checked = exact_checked = 0
+        a, _, paths = one_at_a_time(wt, n, chi, gates)
+        if any(p != 0 for p in paths):
+            continue
+        exact += 1
+        e = max(rel(amps_b[i], a), rel(amps_t[i], a))
+        worst = max(worst, e)
+        assert e <= 1e-12, (i, a, amps_b[i], amps_t[i])
+    say("config4: batched/threaded vs one at a time on %d range-finder-free slices: worst %.1e" % (exact, worst))
+    assert exact >= 16
+
+
+def test_config4_slices_vs_oracle(wt):
+    slices = C.config4()
+    om.set_threads(min(16, os.cpu_count() or 1))
+    amps, codes = wt.run_slices(slices, workers=0)
+    assert all(c == wt.OK for c in codes)
+    # Range-finder-free slices cheap enough for the CPU oracle.
+    checked = 0
     for i, (n, chi, gates) in enumerate(slices):
         units = C.two_qubit_units(gates, n)
         if chi > 64 or units * min(chi, 2 ** (n // 2)) ** 3 > 4e8:
@@ -61,17 +96,45 @@ def test_config4_batch(wt):
         o = om.OracleMps(n, chi, record=True)
         o.apply_circuit(gates)
         want = o.amplitude_at_compact_zero()
-        ctr = o.counters()
-        st = wt.MpsState(n, chi)
-        st.apply_circuit(gates)
-        bonds = st.bond_dims()
-        st.free()
+        _, bonds, _ = one_at_a_time(wt, n, chi, gates)
         assert bonds == o.bond_dims(), i
-        if ctr["rsvd_kept"] == 0:
-            assert abs(amps[i] - want) <= 1e-10 * max(abs(want), 1e-300), (i, amps[i], want)
-            exact_checked += 1
+        assert o.counters()["rsvd_kept"] == 0
+        assert rel(amps[i], want) <= 1e-10, (i, amps[i], want)
         checked += 1
-        if checked >= 48:
+        if checked >= 24:
             break
-    say("config4: oracle-checked slices", checked, "of which range-finder-free", exact_checked)
-    assert exact_checked >= 16
+    say("config4: range-finder-free slices against the oracle:", checked)
+    assert checked >= 16
+    # chi = 128 and chi = 256 slices whose splits reach the range finder (theta 256^2 and
+    # 512^2), the cheapest four of each, against the oracle's exact-arithmetic rank rule.
+    om.set_qr_rank_tol(1e-10)
+    try:
+        for cap in (128, 256):
+            cands = []
+            for i, (n, chi, gates) in enumerate(slices):
+                if chi != cap or 2 ** (n // 2) < 2 * cap:
+                    continue
+                cands.append((C.two_qubit_units(gates, n) * min(cap, 2 ** (n // 2)) ** 3, i))
+            cands.sort()
+            done = 0
+            for _, i in cands:
+                n, chi, gates = slices[i]
+                a, bonds, paths = one_at_a_time(wt, n, chi, gates)
+                if paths.count(1) == 0:
+                    continue
+                t = time.time()
+                o = om.OracleMps(n, chi, record=True)
+                o.apply_circuit(gates)
+                ctr = o.counters()
+                want = o.amplitude_at_compact_zero()
+                say("config4 slice %d: %d qubits chi %d, range-finder splits gpu %d oracle %d, oracle %.1f s, "
+                    "amplitude rel diff %.1e" % (i, n, chi, paths.count(1), ctr["rsvd_kept"], time.time() - t,
+                                                 rel(a, want)))
+                assert bonds == o.bond_dims(), i
+                assert paths.count(1) == ctr["rsvd_kept"] and paths.count(2) == ctr["rsvd_fallbacks"], i
+                done += 1
+                if done >= 4:
+                    break
+            assert done >= 4, cap
+    finally:
+        om.set_qr_rank_tol(0.0)

@@ -393,17 +393,27 @@ struct Site {
     double2* ptr = nullptr;
     size_t cap = 0;  // complex elements
     int l = 1, r = 1;
+    bool owned = true;  // false: carved from the state's site arena (freed with it)
 };
 
 // One two-site unit: sites p and p+1, resolved 4x4 unitary.
+}  // namespace
+
+struct wqc_tn_state;
+
+namespace {
+// One two-site unit: sites p and p+1 of `owner` (null: the state that runs it), resolved 4x4
+// unitary. A batch of independent slices runs units of many states in one launch.
 struct Unit {
     int p;
     double2 u4[16];
+    wqc_tn_state* owner = nullptr;
 };
 
 struct PendingOne {
     int site;
     double2 u[4];
+    wqc_tn_state* owner = nullptr;
 };
 
 }  // namespace
@@ -450,7 +460,8 @@ struct wqc_tn_state {
         (void)hipSetDevice(dev ? dev->ordinal : 0);
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto& s : sites)
-            if (s.ptr) (void)hipFree(s.ptr);
+            if (s.ptr && s.owned) (void)hipFree(s.ptr);
+        site_arena.release();
         ws.release();
         ws32.release();
         ews.release();
@@ -498,11 +509,26 @@ struct wqc_tn_state {
         HIPC(hipMalloc(&np, elems * sizeof(double2)));
         if (s.ptr) {
             HIPC(hipMemcpy(np, s.ptr, std::min(s.cap, elems) * sizeof(double2), hipMemcpyDeviceToDevice));
-            HIPC(hipFree(s.ptr));
+            if (s.owned) HIPC(hipFree(s.ptr));
         }
-        site_bytes += (elems - s.cap) * sizeof(double2);
+        site_bytes += s.owned ? (elems - s.cap) * sizeof(double2) : elems * sizeof(double2);
         s.ptr = np;
         s.cap = elems;
+        s.owned = true;
+        note_peak();
+    }
+
+    // ensure_site_cap on the state that owns the site (this one, or a slice whose unit this
+    // state runs): a reallocation of another state's site first drains this state's stream,
+    // where the kernels reading that site were queued.
+    void grow_site(wqc_tn_state* owner, int i, size_t elems) {
+        if (owner == this) {
+            ensure_site_cap(i, elems);
+            return;
+        }
+        if (elems <= owner->sites[i].cap) return;
+        sync();
+        owner->ensure_site_cap(i, elems);
         note_peak();
     }
 
@@ -595,6 +621,41 @@ struct wqc_tn_state {
         note_peak();
     }
 
+    // A site container for the slice batch: the |0...0> register of qn qubits with bond cap mb,
+    // its sites carved from one device arena (light-cone capacities, one allocation), no stream,
+    // staging or workspace of its own: another state runs its operations. one_src holds the
+    // complex 1 that starts every site (set by the caller's batched copy, see run_slices).
+    DevBuf site_arena;
+    void init_container(wqc_tn_device* d, size_t qn, size_t mb, int prec, std::vector<wqc::CopyJob>& starts,
+                        const double* one_src) {
+        dev = d;
+        n = (int)qn;
+        chi = mb;
+        precision = prec;
+        sites.assign(n, Site{});
+        size_t total = 0;
+        std::vector<size_t> at(n);
+        for (int i = 0; i < n; ++i) {
+            at[i] = total;
+            const size_t cap = bond_cap(i) * 2 * bond_cap(i + 1);
+            total += (cap + 15) & ~(size_t)15;  // 256-byte aligned sites
+        }
+        site_arena.ensure(total * sizeof(double2));
+        HIPC(hipMemset(site_arena.p, 0, site_arena.bytes));
+        double2* base = reinterpret_cast<double2*>(site_arena.p);
+        for (int i = 0; i < n; ++i) {
+            Site& s = sites[i];
+            s.ptr = base + at[i];
+            s.cap = bond_cap(i) * 2 * bond_cap(i + 1);
+            s.l = s.r = 1;
+            s.owned = false;
+            starts.push_back(wqc::CopyJob{one_src, reinterpret_cast<double*>(s.ptr), 2, 0});
+        }
+        site_bytes = site_arena.bytes;
+        touched.assign(n, 0);
+        note_peak();
+    }
+
     // A fresh |0...0> register of qn qubits and bond cap mb on the same stream, staging and
     // workspaces; site buffers are kept where they are large enough. Used by the slice batch.
     void reset(size_t qn, size_t mb) {
@@ -615,7 +676,7 @@ struct wqc_tn_state {
             old[i] = Site{};
         }
         for (auto& s : old)
-            if (s.ptr) {
+            if (s.ptr && s.owned) {
                 HIPC(hipFree(s.ptr));
                 site_bytes -= s.cap * sizeof(double2);
             }
@@ -637,7 +698,9 @@ struct wqc_tn_state {
         std::vector<OneQubitJob> jobs(ops.size());
         int max_elems = 1;
         for (size_t i = 0; i < ops.size(); ++i) {
-            Site& s = sites[ops[i].site];
+            wqc_tn_state* o = ops[i].owner ? ops[i].owner : this;
+            Site& s = o->sites[ops[i].site];
+            if (o != this) o->stats.one_qubit_gates++;
             jobs[i].site = s.ptr;
             jobs[i].l = s.l;
             jobs[i].r = s.r;
@@ -651,7 +714,8 @@ struct wqc_tn_state {
             check_launch();
             klaunch[0]++;
         }
-        stats.one_qubit_gates += ops.size();
+        for (const auto& op : ops)
+            if (!op.owner || op.owner == this) stats.one_qubit_gates++;
         sync();
         resolve_timings();
     }
@@ -1083,6 +1147,7 @@ struct wqc_tn_state {
 
     // ------------------------------------------------------------ two-site batch
     struct JobInfo {
+        wqc_tn_state* st;  // the state whose sites the unit updates
         int p, dl, bond, dr, rows, cols;
         bool rsvd;
         size_t ell;
@@ -1113,9 +1178,10 @@ struct wqc_tn_state {
         const size_t C16 = sizeof(double2);
         for (int i = 0; i < nu; ++i) {
             JobInfo& j = J[i];
+            j.st = units[i].owner ? units[i].owner : this;
             j.p = units[i].p;
-            const Site& L = sites[j.p];
-            const Site& R = sites[j.p + 1];
+            const Site& L = j.st->sites[j.p];
+            const Site& R = j.st->sites[j.p + 1];
             if (L.r != R.l) fail(WQC_TN_EXEC_FAILED, "MPS bond layout mismatch between sites");
             j.dl = L.l;
             j.bond = L.r;
@@ -1123,8 +1189,8 @@ struct wqc_tn_state {
             j.rows = 2 * j.dl;
             j.cols = 2 * j.dr;
             const size_t rows = j.rows, cols = j.cols;
-            j.ell = std::min(std::min(chi + RSVD_OVERSAMPLE, rows), cols);
-            j.rsvd = !reject && !exact_only &&
+            j.ell = std::min(std::min(j.st->chi + RSVD_OVERSAMPLE, rows), cols);
+            j.rsvd = !j.st->reject && !j.st->exact_only &&
                      !(j.ell == 0 || j.ell >= std::min(rows, cols) || !rsvd_worth(rows, cols, j.ell));
             j.om = nullptr;
             j.fallback = false;
@@ -1154,8 +1220,8 @@ struct wqc_tn_state {
                 j.o_B = take(j.ell * cols * C16);
                 j.o_JB = take(j.ell * j.ell * C16);
                 j.o_sigB = take(j.ell * sizeof(double));
-                j.o_W = take(j.ell * std::min(chi, j.ell) * C16);
-                j.o_W2 = take(r * std::min(chi, j.ell) * C16 * 2);
+                j.o_W = take(j.ell * std::min(j.st->chi, j.ell) * C16);
+                j.o_W2 = take(r * std::min(j.st->chi, j.ell) * C16 * 2);
             }
         }
         const int nblk = 64;
@@ -1178,8 +1244,8 @@ struct wqc_tn_state {
             int mdl = 1, mdr = 1;
             for (int i = 0; i < nu; ++i) {
                 JobInfo& j = J[i];
-                mj[i].L = sites[j.p].ptr;
-                mj[i].R = sites[j.p + 1].ptr;
+                mj[i].L = j.st->sites[j.p].ptr;
+                mj[i].R = j.st->sites[j.p + 1].ptr;
                 mj[i].out = P(j.o_theta);
                 mj[i].dl = j.dl;
                 mj[i].bond = j.bond;
@@ -1193,7 +1259,7 @@ struct wqc_tn_state {
             check_launch();
             klaunch[1]++;
         }
-        stats.two_qubit_units += nu;
+        for (const JobInfo& j : J) j.st->stats.two_qubit_units++;
 
         SplitBatch rs, ex;
         for (int i = 0; i < nu; ++i) (J[i].rsvd ? rs.ids : ex.ids).push_back(i);
@@ -1218,14 +1284,14 @@ struct wqc_tn_state {
         }
         sync();
         resolve_timings();
-        if (split_log_on)
-            for (int i = 0; i < nu; ++i) {
+        for (int i = 0; i < nu; ++i)
+            if (J[i].st->split_log_on) {
                 const JobInfo& j = J[i];
                 const int path = !j.rsvd ? 0 : (j.fallback ? 2 : 1);
                 const int32_t e[8] = {j.rows, j.cols, j.bond, path, (int32_t)j.ell, (int32_t)j.chi_kept,
                                       j.sweeps, j.sweeps_lo};
-                split_log.insert(split_log.end(), e, e + 8);
-                split_dw.push_back(j.dw);
+                j.st->split_log.insert(j.st->split_log.end(), e, e + 8);
+                j.st->split_dw.push_back(j.dw);
             }
     }
 
@@ -1827,7 +1893,7 @@ struct wqc_tn_state {
             const bool norm_ok =
                 std::fabs(s2 - fro2) <= SVD_CHECK_TOL * std::max(fro2, 2.2250738585072014e-308) + 1e-28;
             if (!norm_ok || !probes_pass(b, k, fro2)) {
-                stats.svd_check_failures++;
+                j.st->stats.svd_check_failures++;
                 char buf[256];
                 std::snprintf(buf, sizeof buf, " (sum s^2 = %.17g, |A|_F^2 = %.17g, probe residuals %.3g %.3g, job %d of %d, n=%d m=%d)",
                               s2, fro2, std::sqrt(hread[b.hprobe + 2 * (size_t)k]),
@@ -1835,7 +1901,7 @@ struct wqc_tn_state {
                 fail(WQC_TN_EXEC_FAILED, "SVD of a " + std::to_string(j.rows) + "x" + std::to_string(j.cols) +
                                              " matrix failed its check" + buf);
             }
-            const size_t c = retained_chi(sorted, chi, j.rows, j.cols);
+            const size_t c = retained_chi(sorted, j.st->chi, j.rows, j.cols);
             if (trace_jacobi)
                 for (int i = 0; i < nn; ++i)
                     if (sorted[i] > 0.2 * SINGULAR_CUTOFF && sorted[i] < 5 * SINGULAR_CUTOFF)
@@ -1847,23 +1913,24 @@ struct wqc_tn_state {
                 if (sorted[i] > SINGULAR_CUTOFF) dropped = true;
                 drop += sorted[i] * sorted[i];
             }
-            if (reject && dropped) fail(WQC_TN_BOND_EXCEEDED, "mps bond exceeded chi");
-            if (dropped) stats.truncated_splits++;
+            wqc_tn_stats& sx = j.st->stats;
+            if (j.st->reject && dropped) fail(WQC_TN_BOND_EXCEEDED, "mps bond exceeded chi");
+            if (dropped) sx.truncated_splits++;
             const double dw = s2 > 0 ? drop / s2 : 0.0;
             j.dw = dw;
-            stats.max_discarded_weight = std::max(stats.max_discarded_weight, dw);
-            stats.sum_discarded_weight += dw;
-            stats.exact_splits++;
+            sx.max_discarded_weight = std::max(sx.max_discarded_weight, dw);
+            sx.sum_discarded_weight += dw;
+            sx.exact_splits++;
             perms[k].assign(order.begin(), order.begin() + c);
             j.chi_kept = c;
-            ensure_site_cap(j.p, (size_t)j.dl * 2 * c);
-            ensure_site_cap(j.p + 1, c * 2 * (size_t)j.dr);
+            grow_site(j.st, j.p, (size_t)j.dl * 2 * c);
+            grow_site(j.st, j.p + 1, c * 2 * (size_t)j.dr);
             const bool tall = j.rows > j.cols;
             const bool pre = !b.pre.empty() && b.pre[k];
             fj[k].X = jj[k].X;
             fj[k].J = pre ? b.orig[k] : jj[k].J;
-            fj[k].left = sites[j.p].ptr;
-            fj[k].right = sites[j.p + 1].ptr;
+            fj[k].left = j.st->sites[j.p].ptr;
+            fj[k].right = j.st->sites[j.p + 1].ptr;
             fj[k].perm = nullptr;  // set below from the batch's one upload of every perm
             fj[k].sigma = PD(j.o_sig);
             fj[k].rows = j.rows;
@@ -1884,8 +1951,8 @@ struct wqc_tn_state {
         }
         for (int k = 0; k < ne; ++k) {
             JobInfo& j = J[ids[k]];
-            sites[j.p].r = (int)j.chi_kept;
-            sites[j.p + 1].l = (int)j.chi_kept;
+            j.st->sites[j.p].r = (int)j.chi_kept;
+            j.st->sites[j.p + 1].l = (int)j.chi_kept;
         }
     }
 
@@ -1918,10 +1985,11 @@ struct wqc_tn_state {
             }
             const bool svd_ok = std::fabs(s2 - fro2) <= SVD_CHECK_TOL * std::max(fro2, 2.2250738585072014e-308) + 1e-28 &&
                                 probes_pass(b, k, fro2);
-            if (!svd_ok) stats.svd_check_failures++;
+            wqc_tn_stats& sx = j.st->stats;
+            if (!svd_ok) sx.svd_check_failures++;
             for (double& s : values)
                 if (s < SINGULAR_CUTOFF * 1e-2) s = 0.0;
-            const size_t c = retained_chi(values, chi, j.rows, j.cols);
+            const size_t c = retained_chi(values, j.st->chi, j.rows, j.cols);
             if (trace_jacobi && (!svd_ok || c == 0 || needs_f64_fallback(values, c))) {
                 std::fprintf(stderr, "rsvd fallback: site %d ell %zu kept %zu svd_ok %d", j.p, j.ell, c, (int)svd_ok);
                 for (int i = 0; i < nn; ++i)
@@ -1932,10 +2000,10 @@ struct wqc_tn_state {
             }
             if (!svd_ok || c == 0 || needs_f64_fallback(values, c)) {
                 j.fallback = true;
-                stats.rsvd_fallbacks++;
+                sx.rsvd_fallbacks++;
                 continue;
             }
-            stats.rsvd_kept++;
+            sx.rsvd_kept++;
             bool dropped = false;
             double drop = 0.0, tot = 0.0;
             for (int i = 0; i < nn; ++i) {
@@ -1945,21 +2013,21 @@ struct wqc_tn_state {
                     if (values[i] > SINGULAR_CUTOFF) dropped = true;
                 }
             }
-            if (dropped) stats.truncated_splits++;
+            if (dropped) sx.truncated_splits++;
             const double dw = tot > 0 ? drop / tot : 0.0;
             j.dw = dw;
-            stats.max_discarded_weight = std::max(stats.max_discarded_weight, dw);
-            stats.sum_discarded_weight += dw;
+            sx.max_discarded_weight = std::max(sx.max_discarded_weight, dw);
+            sx.sum_discarded_weight += dw;
             perms[k].assign(order.begin(), order.begin() + c);
             j.chi_kept = c;
-            ensure_site_cap(j.p, (size_t)j.dl * 2 * c);
-            ensure_site_cap(j.p + 1, c * 2 * (size_t)j.dr);
+            grow_site(j.st, j.p, (size_t)j.dl * 2 * c);
+            grow_site(j.st, j.p + 1, c * 2 * (size_t)j.dr);
             FactorJob f;
             const bool pre = !b.pre.empty() && b.pre[k];
             f.X = jj[k].X;
             f.J = pre ? b.orig[k] : jj[k].J;
             f.left = P(j.o_W);
-            f.right = sites[j.p + 1].ptr;
+            f.right = j.st->sites[j.p + 1].ptr;
             f.perm = nullptr;  // set below from the batch's one upload of every perm
             f.sigma = PD(j.o_sigB);
             f.rows = j.rows;
@@ -1994,7 +2062,7 @@ struct wqc_tn_state {
             double2* W3 = W2 + (size_t)r * c;
             g7.push_back(gj(P(j.o_V), r, 1, P(j.o_W), c, 0, W2, c, r, c, (int)j.ell, 1, 0));
             g8.push_back(gj(P(j.o_T), r, 0, W2, c, 0, W3, c, r, c, r, 1, 0));
-            double2* left = sites[j.p].ptr;
+            double2* left = j.st->sites[j.p].ptr;
             HIPC(hipMemcpyAsync(left, P(j.o_W), j.ell * (size_t)c * sizeof(double2), hipMemcpyDeviceToDevice, stream));
             if ((size_t)j.rows > j.ell)
                 HIPC(hipMemsetAsync(left + j.ell * (size_t)c, 0, ((size_t)j.rows - j.ell) * c * sizeof(double2), stream));
@@ -2005,13 +2073,31 @@ struct wqc_tn_state {
         gemm(g9);
         for (int i : kept) {
             JobInfo& j = J[i];
-            sites[j.p].r = (int)j.chi_kept;
-            sites[j.p + 1].l = (int)j.chi_kept;
+            j.st->sites[j.p].r = (int)j.chi_kept;
+            j.st->sites[j.p + 1].l = (int)j.chi_kept;
         }
     }
 
     // ------------------------------------------------------------ scheduling
+    // With `defer` set, flush() records the pending site-disjoint operations as one layer
+    // instead of running them: the slice batch runs the layers of many states together.
+    bool defer = false;
+    struct Layer {
+        std::vector<PendingOne> ones;
+        std::vector<Unit> units;
+    };
+    std::vector<Layer> layers;
+
     void flush() {
+        if (defer) {
+            if (!pend1.empty() || !pend2.empty()) {
+                layers.push_back(Layer{std::move(pend1), std::move(pend2)});
+                pend1.clear();
+                pend2.clear();
+            }
+            std::fill(touched.begin(), touched.end(), 0);
+            return;
+        }
         if (!pend1.empty()) {
             std::vector<PendingOne> ops;
             ops.swap(pend1);
@@ -2287,6 +2373,23 @@ struct wqc_tn_state {
         HIPC(hipMemcpyAsync(&out, v, sizeof(double2), hipMemcpyDeviceToHost, stream));
         sync();
         return {out.x, out.y};
+    }
+
+    // <0...0|psi> of another state's sites (a slice container of the batch), computed with this
+    // state's stream and buffers.
+    std::pair<double, double> amplitude_of(wqc_tn_state& o) {
+        std::swap(sites, o.sites);
+        std::swap(n, o.n);
+        try {
+            auto a = amplitude_zero();
+            std::swap(sites, o.sites);
+            std::swap(n, o.n);
+            return a;
+        } catch (...) {
+            std::swap(sites, o.sites);
+            std::swap(n, o.n);
+            throw;
+        }
     }
 
     std::pair<double, double> amplitude_zero_timed() {
@@ -3235,6 +3338,101 @@ int wqc_tn_outcome_probabilities(wqc_tn_state* st, const int32_t* measure_qubits
     });
 }
 
+namespace {
+
+// Slices as one batch: every slice's gate list is scheduled into layers of site-disjoint
+// operations up front (without running them), then step t gathers layer t of every slice into
+// one one-qubit launch and shared two-site batches (merge, Jacobi rounds, factors launched once
+// for units of many slices of different widths and bond caps). One executor state owns the
+// stream, staging and workspace; the slices are site containers. Units of a step are cut into
+// batches of at most kSliceBatchUnits units and about kSliceBatchBytes of split workspace.
+// Returns false (nothing reported) if a batch fails, so that the caller reruns the slices one by
+// one for per-slice statuses.
+constexpr size_t kSliceBatchUnits = 512;
+constexpr size_t kSliceBatchBytes = size_t(16) << 30;
+
+bool run_slices_batched(wqc_tn_device* dev, const wqc_tn_slice* slices, size_t n_slices, const wqc_tn_gate* gates,
+                        int precision, double* out_amps, int* out_status, std::string& first_error) {
+    auto ex = std::make_unique<wqc_tn_state>();
+    ex->init(dev, 1, 1, precision);
+    DevBuf one;
+    one.ensure(sizeof(double2));
+    HIPC(hipMemcpy(one.p, &wqc_tn_state::kOne, sizeof(double2), hipMemcpyHostToDevice));
+    std::vector<std::unique_ptr<wqc_tn_state>> st(n_slices);
+    std::vector<wqc::CopyJob> starts;
+    size_t max_layers = 0;
+    for (size_t i = 0; i < n_slices; ++i) {
+        const wqc_tn_slice& sl = slices[i];
+        out_amps[2 * i] = out_amps[2 * i + 1] = std::nan("");
+        out_status[i] = guarded([&] {
+            auto s = std::make_unique<wqc_tn_state>();
+            s->init_container(dev, sl.qubit_count, sl.max_bond_dim, precision, starts,
+                              reinterpret_cast<const double*>(one.p));
+            s->defer = true;
+            s->queue_circuit(gates + sl.gate_offset, sl.gate_count);
+            s->flush();
+            max_layers = std::max(max_layers, s->layers.size());
+            st[i] = std::move(s);
+        });
+        if (out_status[i] != WQC_TN_OK && first_error.empty())
+            first_error = "slice " + std::to_string(i) + ": " + g_last_error;
+    }
+    // Every site starts as (1, 0): one batched copy of the complex one.
+    for (size_t k = 0; k < starts.size(); k += 65536) {
+        ex->tables_reset();
+        std::vector<wqc::CopyJob> part(starts.begin() + k, starts.begin() + std::min(starts.size(), k + 65536));
+        wqc::launch_copy_batch(ex->upload(part), (int)part.size(), 2, ex->stream);
+        ex->check_launch();
+        ex->sync();
+    }
+    try {
+        for (size_t t = 0; t < max_layers; ++t) {
+            std::vector<PendingOne> ones;
+            std::vector<Unit> units;
+            for (size_t i = 0; i < n_slices; ++i) {
+                if (!st[i] || t >= st[i]->layers.size()) continue;
+                wqc_tn_state::Layer& L = st[i]->layers[t];
+                for (auto op : L.ones) {
+                    op.owner = st[i].get();
+                    ones.push_back(op);
+                }
+                for (auto u : L.units) {
+                    u.owner = st[i].get();
+                    units.push_back(u);
+                }
+                L = wqc_tn_state::Layer{};
+            }
+            ex->run_one_qubit(ones);
+            for (size_t k = 0; k < units.size();) {
+                size_t e = k, bytes = 0;
+                while (e < units.size() && e - k < kSliceBatchUnits) {
+                    const wqc_tn_state* o = units[e].owner;
+                    const size_t rows = 2 * (size_t)o->sites[units[e].p].l, cols = 2 * (size_t)o->sites[units[e].p + 1].r;
+                    const size_t est = 8 * rows * cols * sizeof(double2);
+                    if (e > k && bytes + est > kSliceBatchBytes) break;
+                    bytes += est;
+                    ++e;
+                }
+                std::vector<Unit> chunk(units.begin() + k, units.begin() + e);
+                ex->run_units(chunk);
+                k = e;
+            }
+        }
+        for (size_t i = 0; i < n_slices; ++i) {
+            if (!st[i]) continue;
+            auto a = ex->amplitude_of(*st[i]);
+            out_amps[2 * i] = a.first;
+            out_amps[2 * i + 1] = a.second;
+        }
+    } catch (const Failure& f) {
+        if (f.code == WQC_TN_CANCELLED) throw;
+        return false;
+    }
+    return true;
+}
+
+}  // namespace
+
 int wqc_tn_run_slices(wqc_tn_device* dev, const wqc_tn_slice* slices, size_t n_slices,
                       const wqc_tn_gate* gates, int precision, int workers, double* out_amps,
                       int* out_status) {
@@ -3248,6 +3446,14 @@ int wqc_tn_run_slices(wqc_tn_device* dev, const wqc_tn_slice* slices, size_t n_s
             if (slices[i].gate_count && !gates) fail(WQC_TN_INVALID_ARG, "null argument");
         }
         if (!n_slices) return;
+        HIPC(hipSetDevice(dev->ordinal));
+        if (workers == 0) {
+            std::string err;
+            if (run_slices_batched(dev, slices, n_slices, gates, precision, out_amps, out_status, err)) {
+                if (!err.empty()) fail(WQC_TN_EXEC_FAILED, err);
+                return;
+            }
+        }
         const int nw = std::max(1, std::min(workers > 0 ? workers : 8, (int)std::min<size_t>(n_slices, 64)));
         // Largest estimated cost first (two-qubit gates x chi^3 at the slice's reachable bond).
         std::vector<size_t> order(n_slices);

@@ -180,10 +180,11 @@ def default_device() -> Device:
     return _default_device
 
 
-def run_slices(slices, precision: int = PRECISION_FP64, workers: int = 8, device=None):
-    """<0...0|psi> of independent slices [(qubit_count, max_bond_dim, gates), ...] run
-    concurrently (contract_slice per slice, src/tn/contract.rs:10-30); returns
-    (amplitudes, status codes)."""
+def run_slices(slices, precision: int = PRECISION_FP64, workers: int = 0, device=None):
+    """<0...0|psi> of independent slices [(qubit_count, max_bond_dim, gates), ...]
+    (contract_slice per slice, src/tn/contract.rs:10-30); returns (amplitudes, status codes).
+    workers = 0: one batch, every layer step of all slices in shared launches; workers > 0:
+    that many host threads, one state per slice at a time."""
     L = load()
     dev = device or default_device()
     descs = (Slice * max(1, len(slices)))()
