@@ -83,10 +83,14 @@ typedef struct wqc_tn_slice {
  *  MIXED_FP32 : the Jacobi SVD runs its early sweeps on complex64 rows (FP32 MFMA Gram and
  *               rotation, FP32 eigensolve) while the accumulated rotation stays unitary in
  *               FP64; the rows are then recomputed in FP64 and the last sweeps, the
- *               singular values, the truncation decision and the factors are FP64. */
+ *               singular values, the truncation decision and the factors are FP64.
+ *  MIXED_BF16 : as MIXED_FP32, with the early sweeps' Gram matrices on BF16 MFMA in split-BF16
+ *               (each FP32 operand as two bf16, three products, FP32 accumulation); their
+ *               rotations stay FP32, the finish FP64. */
 typedef enum wqc_tn_precision {
     WQC_TN_PRECISION_FP64 = 0,
-    WQC_TN_PRECISION_MIXED_FP32 = 1
+    WQC_TN_PRECISION_MIXED_FP32 = 1,
+    WQC_TN_PRECISION_MIXED_BF16 = 2
 } wqc_tn_precision;
 
 typedef struct wqc_tn_device wqc_tn_device;

@@ -134,12 +134,13 @@ def noise_bond_diffs(got, want):
     return d, ok
 
 
-def test_mixed_form_config3(wt):
+@pytest.mark.parametrize("prec", [1, 2])
+def test_mixed_form_config3(wt, prec):
     with open(os.path.join(GOLDEN, "config3.json")) as f:
         want = json.load(f)
     n, chi, gates = C.config3()
     L = layers_of(gates, n)
-    mx = wt.MpsState(n, chi, wt.PRECISION_MIXED_FP32)
+    mx = wt.MpsState(n, chi, prec)
     f64 = wt.MpsState(n, chi, wt.PRECISION_FP64)
     for k, lay in enumerate(L[:24]):
         mx.apply_circuit(lay)

@@ -456,20 +456,23 @@ def test_golden_config3_chi512(wt):
 
 
 # ---------------------------------------------------------------- mixed form
+@pytest.mark.parametrize("prec", [1, 2])
 @pytest.mark.parametrize("name", ["config1", "config2"])
-def test_mixed_form_matches_golden(wt, name):
-    # FP32 early sweeps, FP64 finish: same bonds and counters, amplitude to 1e-10.
+def test_mixed_form_matches_golden(wt, name, prec):
+    # Low-precision early sweeps (FP32 Gram, or split-BF16 Gram on BF16 MFMA), FP64 finish:
+    # same bonds and counters, amplitude to 1e-10.
     n, chi, gates = getattr(C, name)()
-    g = wt.MpsState(n, chi, wt.PRECISION_MIXED_FP32)
+    g = wt.MpsState(n, chi, prec)
     g.apply_circuit(gates)
-    check_against_golden(g, load_golden(name), "mixed " + name)
+    check_against_golden(g, load_golden(name), "mixed(%d) %s" % (prec, name))
     assert g.stats()["jacobi_sweeps_lo"] > 0
 
 
-def test_mixed_form_truncating_and_range_finder(wt):
+@pytest.mark.parametrize("prec", [1, 2])
+def test_mixed_form_truncating_and_range_finder(wt, prec):
     want = load_golden("rsvd_brickwork40_chi128")
     gates = C.rotation_cz_brickwork(40, 26, 0x5EED)
-    g = wt.MpsState(40, 128, wt.PRECISION_MIXED_FP32)
+    g = wt.MpsState(40, 128, prec)
     for L, lay in enumerate(layers_of(gates, 40)):
         g.apply_circuit(lay)
         cp = want["checkpoints"].get(str(L + 1))
@@ -523,7 +526,7 @@ def test_lq_preconditioned_split_vs_oracle(wt, dl, dr, bond, chi, lq_env):
     g.free()
 
 
-@pytest.mark.parametrize("prec,bar", [(0, 1e-12), (1, 1e-11)])
+@pytest.mark.parametrize("prec,bar", [(0, 1e-12), (1, 1e-11), (2, 1e-11)])
 def test_wide_and_bulk_rounds_agree(wt, prec, bar):
     """The Jacobi rounds (FP64, and the mixed form's low-precision ones) have a 512-thread form
     for sweeps with at most one block pair per CU and a 256-thread form for the rest; they

@@ -146,6 +146,10 @@ bool needs_f64_fallback(const std::vector<double>& v, size_t chi) {
     return false;
 }
 
+bool valid_precision(int p) {
+    return p == WQC_TN_PRECISION_FP64 || p == WQC_TN_PRECISION_MIXED_FP32 || p == WQC_TN_PRECISION_MIXED_BF16;
+}
+
 bool rsvd_worth(size_t rows, size_t cols, size_t ell) {
     unsigned __int128 f = (unsigned __int128)rows * cols * ell;
     unsigned __int128 side = RSVD_MIN_SIDE;
@@ -579,7 +583,7 @@ struct wqc_tn_state {
         // the rows of theta' keeps small singular values relatively accurate; R = L^H carries
         // the LQ's rounding at the scale of the largest rows, which moves near-cutoff
         // decisions); WQC_TN_LQ=0/1 overrides either.
-        use_lq = prec == WQC_TN_PRECISION_MIXED_FP32;
+        use_lq = prec != WQC_TN_PRECISION_FP64;
         if (const char* e = std::getenv("WQC_TN_LQ")) use_lq = std::atoi(e) != 0;
         if (const char* e = std::getenv("WQC_TN_SPLITK")) use_splitk = std::atoi(e) != 0;
         if (const char* e = std::getenv("WQC_TN_JACOBI_WIDE")) use_wide = std::atoi(e) != 0;
@@ -982,7 +986,7 @@ struct wqc_tn_state {
                 Timer tm(this, 3);
                 for (int r = 0; r < rounds; ++r)
                     wqc::launch_jacobi_round_lo(dj, nj, pairs, r, tol, nu2, (float)floor_lo, inner_cap_lo, d_rot, d_done,
-                                                d_jstats, d_off, wide, stream);
+                                                d_jstats, d_off, wide, precision == WQC_TN_PRECISION_MIXED_BF16, stream);
                 check_launch();
                 klaunch[3] += rounds;
             }
@@ -1078,7 +1082,7 @@ struct wqc_tn_state {
     void jacobi_any(std::vector<JacobiJob>& jobs, double* d_sigma, const std::vector<long>& sig_off,
                     int* d_flags) {
         last_sweeps_lo.assign(jobs.size(), 0);
-        if (precision == WQC_TN_PRECISION_MIXED_FP32)
+        if (precision != WQC_TN_PRECISION_FP64)
             jacobi_mixed(jobs, d_sigma, sig_off, d_flags);
         else
             jacobi(jobs, d_sigma, sig_off, d_flags);
@@ -3023,8 +3027,7 @@ int wqc_tn_state_new(wqc_tn_device* dev, size_t qubit_count, size_t max_bond_dim
         if (qubit_count == 0) fail(WQC_TN_INVALID_ARG, "invalid qubit count: 0");
         if (max_bond_dim == 0) fail(WQC_TN_EXEC_FAILED, "MPS max bond dimension must be > 0");
         if (qubit_count > (1u << 20)) fail(WQC_TN_INVALID_ARG, "qubit count too large");
-        if (precision != WQC_TN_PRECISION_FP64 && precision != WQC_TN_PRECISION_MIXED_FP32)
-            fail(WQC_TN_INVALID_ARG, "unknown precision mode");
+        if (!valid_precision(precision)) fail(WQC_TN_INVALID_ARG, "unknown precision mode");
         // ensure_mps_workspace_budget analogue (src/tn/mps.rs:26-48): N chi^2 32 bytes
         // against free device memory.
         HIPC(hipSetDevice(dev->ordinal));
@@ -3438,8 +3441,7 @@ int wqc_tn_run_slices(wqc_tn_device* dev, const wqc_tn_slice* slices, size_t n_s
                       int* out_status) {
     return guarded([&] {
         if (!dev || (n_slices && (!slices || !out_amps || !out_status))) fail(WQC_TN_INVALID_ARG, "null argument");
-        if (precision != WQC_TN_PRECISION_FP64 && precision != WQC_TN_PRECISION_MIXED_FP32)
-            fail(WQC_TN_INVALID_ARG, "unknown precision mode");
+        if (!valid_precision(precision)) fail(WQC_TN_INVALID_ARG, "unknown precision mode");
         for (size_t i = 0; i < n_slices; ++i) {
             if (slices[i].qubit_count == 0) fail(WQC_TN_INVALID_ARG, "invalid qubit count: 0");
             if (slices[i].max_bond_dim == 0) fail(WQC_TN_EXEC_FAILED, "MPS max bond dimension must be > 0");
@@ -3600,8 +3602,7 @@ int wqc_tn_backward(wqc_tn_device* dev, size_t qubit_count, size_t max_bond_dim,
         if (qubit_count == 0) fail(WQC_TN_INVALID_ARG, "invalid qubit count: 0");
         if (max_bond_dim == 0) fail(WQC_TN_EXEC_FAILED, "MPS max bond dimension must be > 0");
         if (!n_terms) fail(WQC_TN_INVALID_ARG, "observable must have at least one term");
-        if (precision != WQC_TN_PRECISION_FP64 && precision != WQC_TN_PRECISION_MIXED_FP32)
-            fail(WQC_TN_INVALID_ARG, "unknown precision mode");
+        if (!valid_precision(precision)) fail(WQC_TN_INVALID_ARG, "unknown precision mode");
         const int n = (int)qubit_count;
         for (size_t j = 0; j < n_terms * qubit_count; ++j) {
             const char ch = labels[j];

@@ -176,7 +176,8 @@ void launch_jacobi_round(const JacobiJob* jobs, int njobs, int max_pairs, int ro
                          unsigned long long* stats, unsigned long long* offmax, bool wide, hipStream_t s);
 void launch_jacobi_round_lo(const JacobiLoJob* jobs, int njobs, int max_pairs, int round,
                             float tol, double nu2, float rel_floor, int inner_cap, int* rotated, const int* done,
-                            unsigned long long* stats, unsigned long long* offmax, bool wide, hipStream_t s);
+                            unsigned long long* stats, unsigned long long* offmax, bool wide, bool bf16_gram,
+                            hipStream_t s);
 void launch_to_f32(const double2* const* src, float2* const* dst, const long* counts, int njobs,
                    long max_count, hipStream_t s);
 void launch_to_f64(const float2* const* src, double2* const* dst, const long* counts, int njobs,

@@ -710,8 +710,16 @@ __device__ __forceinline__ f4v mfma_f32(float a, float b, f4v c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// NT = 512 for rounds with at most one block pair per CU, as in k_jacobi_round.
-template <int NT>
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ f4v mfma_bf16(bf8v a, bf8v b, f4v c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// NT = 512 for rounds with at most one block pair per CU, as in k_jacobi_round. GB: the Gram
+// matrix on BF16 MFMA (split-BF16, FP32 accumulation) instead of FP32 MFMA; the eigensolve and
+// the rotations stay FP32.
+template <int NT, bool GB>
 __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round_lo(const JacobiLoJob* __restrict__ jobs,
                                                          int round, float tol_base, double nu2_base,
                                                          float rel_floor, int inner_cap,
@@ -772,6 +780,48 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round_lo(const
         // 3M products and register double-buffered column chunks, as in the FP64 round.
         f4v p00 = {0, 0, 0, 0}, q00 = p00, t00 = p00, p01 = p00, q01 = p00, t01 = p00, p11 = p00, q11 = p00, t11 = p00;
         const float2 zf = make_float2(0.f, 0.f);
+        if constexpr (GB) {
+            // Split-BF16 Gram: every FP32 operand a = a_hi + a_lo (two bf16, round to nearest)
+            // and every real product a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi on
+            // v_mfma_f32_16x16x32_bf16 with FP32 accumulation (about 2^-17 relative per
+            // product). Lane (i, g) supplies columns c0 + 8g .. c0 + 8g + 7 of local rows i and
+            // 16 + i as both operands, so the k order inside the instruction is the same for A
+            // and B; 32 columns per wave and chunk.
+            for (int c0 = wave * 32; c0 < jb.m; c0 += 32 * NW) {
+                bf8v uh[4], ul[4], vh[4], vl[4];  // re, im, re + im, re - im
+#pragma unroll
+                for (int s = 0; s < 8; ++s) {
+                    const int c = c0 + 8 * g + s;
+                    const float2 u = (x0 && c < jb.m) ? x0[c] : zf;
+                    const float2 v = (x1 && c < jb.m) ? x1[c] : zf;
+                    const float uv[4] = {u.x, u.y, u.x + u.y, u.x - u.y};
+                    const float vv[4] = {v.x, v.y, v.x + v.y, v.x - v.y};
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const __bf16 a = (__bf16)uv[t];
+                        uh[t][s] = a;
+                        ul[t][s] = (__bf16)(uv[t] - (float)a);
+                        const __bf16 b = (__bf16)vv[t];
+                        vh[t][s] = b;
+                        vl[t][s] = (__bf16)(vv[t] - (float)b);
+                    }
+                }
+#define WQC_MFMA_X3(ACC, AH, AL, BH, BL) \
+    ACC = mfma_bf16(AL, BH, ACC);        \
+    ACC = mfma_bf16(AH, BL, ACC);        \
+    ACC = mfma_bf16(AH, BH, ACC);
+                WQC_MFMA_X3(p00, uh[0], ul[0], uh[0], ul[0])
+                WQC_MFMA_X3(q00, uh[1], ul[1], uh[1], ul[1])
+                WQC_MFMA_X3(t00, uh[2], ul[2], uh[3], ul[3])
+                WQC_MFMA_X3(p01, uh[0], ul[0], vh[0], vl[0])
+                WQC_MFMA_X3(q01, uh[1], ul[1], vh[1], vl[1])
+                WQC_MFMA_X3(t01, uh[2], ul[2], vh[3], vl[3])
+                WQC_MFMA_X3(p11, vh[0], vl[0], vh[0], vl[0])
+                WQC_MFMA_X3(q11, vh[1], vl[1], vh[1], vl[1])
+                WQC_MFMA_X3(t11, vh[2], vl[2], vh[3], vl[3])
+#undef WQC_MFMA_X3
+            }
+        } else {
         float2 ua[4], va[4], ub[4], vb[4];
 #define WQC_GRAM_LOAD(C0, U, V)                                  \
     _Pragma("unroll") for (int s = 0; s < 4; ++s) {              \
@@ -807,6 +857,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round_lo(const
         }
 #undef WQC_GRAM_LOAD
 #undef WQC_GRAM_ACC
+        }
         f4v a00r = p00 + q00, a00i = t00 - p00 + q00;
         f4v a01r = p01 + q01, a01i = t01 - p01 + q01;
         f4v a11r = p11 + q11, a11i = t11 - p11 + q11;
@@ -2036,13 +2087,22 @@ void launch_jacobi_round(const JacobiJob* jobs, int njobs, int max_pairs, int ro
 
 void launch_jacobi_round_lo(const JacobiLoJob* jobs, int njobs, int max_pairs, int round, float tol,
                             double nu2, float rel_floor, int inner_cap, int* rotated, const int* done, unsigned long long* stats,
-                            unsigned long long* offmax, bool wide, hipStream_t s) {
-    if (wide)
-        hipLaunchKernelGGL(k_jacobi_round_lo<512>, dim3(max_pairs, njobs), dim3(512), 0, s, jobs, round, tol, nu2,
-                           rel_floor, inner_cap, rotated, done, stats, offmax);
-    else
-        hipLaunchKernelGGL(k_jacobi_round_lo<256>, dim3(max_pairs, njobs), dim3(256), 0, s, jobs, round, tol, nu2,
-                           rel_floor, inner_cap, rotated, done, stats, offmax);
+                            unsigned long long* offmax, bool wide, bool bf16_gram, hipStream_t s) {
+#define WQC_LO(NT, GB)                                                                                        \
+    hipLaunchKernelGGL((k_jacobi_round_lo<NT, GB>), dim3(max_pairs, njobs), dim3(NT), 0, s, jobs, round, tol, \
+                       nu2, rel_floor, inner_cap, rotated, done, stats, offmax)
+    if (wide) {
+        if (bf16_gram)
+            WQC_LO(512, true);
+        else
+            WQC_LO(512, false);
+    } else {
+        if (bf16_gram)
+            WQC_LO(256, true);
+        else
+            WQC_LO(256, false);
+    }
+#undef WQC_LO
 }
 
 void launch_to_f32(const double2* const* src, float2* const* dst, const long* counts, int njobs,

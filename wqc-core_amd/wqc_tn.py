@@ -17,7 +17,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libwqc_tn.so")
 
 OK, INVALID_ARG, OUT_OF_MEMORY, EXEC_FAILED, CANCELLED, BOND_EXCEEDED, NO_DEVICE = range(7)
-PRECISION_FP64, PRECISION_MIXED_FP32 = 0, 1
+PRECISION_FP64, PRECISION_MIXED_FP32, PRECISION_MIXED_BF16 = 0, 1, 2
 SAMPLE_EXACT, SAMPLE_MPS, SAMPLE_AUTO = 0, 1, 2
 
 EXPORTED = (
