@@ -319,6 +319,28 @@ struct wqc_tn_device {
     double2* scratch = nullptr;  // this device's address of the scratch array
     std::mutex mu;
     std::map<std::tuple<size_t, size_t, size_t>, OmegaEntry> omega;
+    // GEMM autotuner: the tile (64 or 32) measured fastest per shape class (log2 buckets of the
+    // largest M, N, K and of the job count, split-K factor, operand ops), kept for the process
+    // and, with WQC_TN_TUNE_CACHE=<file>, in that file across processes.
+    std::map<std::array<int, 6>, int> gemm_tile;
+    std::string tune_path;
+    void load_tune_cache() {
+        if (tune_path.empty()) return;
+        FILE* f = std::fopen(tune_path.c_str(), "r");
+        if (!f) return;
+        std::array<int, 6> k{};
+        int tile = 0;
+        while (std::fscanf(f, "gemm %d %d %d %d %d %d %d\n", &k[0], &k[1], &k[2], &k[3], &k[4], &k[5], &tile) == 7)
+            if (tile == 32 || tile == 64) gemm_tile[k] = tile;
+        std::fclose(f);
+    }
+    void save_tune_entry(const std::array<int, 6>& k, int tile) {  // caller holds mu
+        if (tune_path.empty()) return;
+        if (FILE* f = std::fopen(tune_path.c_str(), "a")) {
+            std::fprintf(f, "gemm %d %d %d %d %d %d %d\n", k[0], k[1], k[2], k[3], k[4], k[5], tile);
+            std::fclose(f);
+        }
+    }
     ~wqc_tn_device() {
         (void)hipSetDevice(ordinal);
         for (auto& kv : omega) {
@@ -587,6 +609,12 @@ struct wqc_tn_state {
         if (const char* e = std::getenv("WQC_TN_LQ")) use_lq = std::atoi(e) != 0;
         if (const char* e = std::getenv("WQC_TN_SPLITK")) use_splitk = std::atoi(e) != 0;
         if (const char* e = std::getenv("WQC_TN_JACOBI_WIDE")) use_wide = std::atoi(e) != 0;
+        if (const char* e = std::getenv("WQC_TN_GRAPH")) use_graph = std::atoi(e) != 0;
+        if (const char* e = std::getenv("WQC_TN_AUTOTUNE")) autotune = std::atoi(e) != 0;
+        if (const char* e = std::getenv("WQC_TN_GEMM_TILE")) {
+            const int t = std::atoi(e);
+            if (t == 32 || t == 64) forced_tile = t;
+        }
         if (const char* e = std::getenv("WQC_TN_JACOBI_INNER_LO")) {
             const int v = std::atoi(e);
             if (v >= 1 && v <= 32) inner_cap_lo = v;
@@ -764,6 +792,44 @@ struct wqc_tn_state {
         spans.clear();
     }
 
+    // ------------------------------------------------------------ sweep graphs
+    // The launches of one Jacobi sweep (flag clearing and every round of the batch) captured
+    // once as a HIP graph and replayed for each sweep of the same form; the kernel arguments stay
+    // fixed for the whole Jacobi call (the done flags are rewritten in place between sweeps).
+    // WQC_TN_GRAPH=0 launches them one by one instead. The graph holds kernel nodes only.
+    bool use_graph = true;
+    struct SweepGraph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        SweepGraph() = default;
+        SweepGraph(const SweepGraph&) = delete;
+        SweepGraph& operator=(const SweepGraph&) = delete;
+        ~SweepGraph() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+        }
+    };
+    template <class F>
+    void run_sweep(SweepGraph& g, F&& launches) {
+        if (!use_graph) {
+            launches();
+            check_launch();
+            return;
+        }
+        if (!g.exec) {
+            HIPC(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            launches();
+            const hipError_t le = hipGetLastError();
+            hipGraph_t graph = nullptr;
+            const hipError_t ce = hipStreamEndCapture(stream, &graph);
+            g.graph = graph;
+            HIPC(le);
+            HIPC(ce);
+            HIPC(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+        }
+        HIPC(hipGraphLaunch(g.exec, stream));
+    }
+
     // ------------------------------------------------------------ Jacobi driver
     // Rotates the rows of every job's X until no block pair needs a rotation, then
     // writes the row norms into sigma (device, at sig_off[j]). Returns sweeps used.
@@ -813,19 +879,21 @@ struct wqc_tn_state {
             jpairs[j] = (b + (b & 1)) / 2;
         }
         const int cus = dev->prop.multiProcessorCount;
+        SweepGraph graphs[2];  // 256-thread and 512-thread rounds
         for (int sweep = 0; sweep < MAX_SWEEPS && active > 0; ++sweep) {
             check_cancel();
-            wqc::launch_fill_int(reinterpret_cast<int*>(d_off), 0, 3 * nj, stream);
             int wgs = 0;
             for (int j = 0; j < nj; ++j)
                 if (!done[j]) wgs += jpairs[j];
             const bool wide = use_wide && wgs <= cus;
             {
                 Timer tm(this, 3);
-                for (int r = 0; r < rounds; ++r)
-                    wqc::launch_jacobi_round(dj, nj, pairs, r, tol, nu2, inner_cap, d_rot, d_done, d_jstats,
-                                             d_off, wide, stream);
-                check_launch();
+                run_sweep(graphs[wide ? 1 : 0], [&] {
+                    wqc::launch_fill_int(reinterpret_cast<int*>(d_off), 0, 3 * nj, stream);
+                    for (int r = 0; r < rounds; ++r)
+                        wqc::launch_jacobi_round(dj, nj, pairs, r, tol, nu2, inner_cap, d_rot, d_done, d_jstats,
+                                                 d_off, wide, stream);
+                });
                 klaunch[3] += rounds;
             }
             stats.jacobi_sweeps += active;
@@ -863,8 +931,9 @@ struct wqc_tn_state {
                     if (stalled) stats.jacobi_stalled++;
                 }
             }
-            // The rounds read the done flags from the uploaded table itself.
-            if (changed && active > 0) d_done = upload_ints(done);
+            // The rounds read the done flags at one address: the new flags are copied there.
+            if (changed && active > 0)
+                HIPC(hipMemcpyAsync(d_done, upload_ints(done), nj * sizeof(int), hipMemcpyDeviceToDevice, stream));
         }
         // A job still rotating after MAX_SWEEPS is counted; as the reference's Jacobi at its sweep
         // cap (src/tn/svd_guard.rs:165-229, then svd_reproduces), its factorization is kept when
@@ -972,22 +1041,25 @@ struct wqc_tn_state {
             jpairs[j] = (b + (b & 1)) / 2;
         }
         const int cus = dev->prop.multiProcessorCount;
+        SweepGraph graphs[2];  // 256-thread and 512-thread rounds
+        const bool bf16 = precision == WQC_TN_PRECISION_MIXED_BF16;
         for (int sweep = 0; sweep < max_sweeps_lo && active > 0; ++sweep) {
             check_cancel();
-            if (d_off)
-                wqc::launch_fill_int(reinterpret_cast<int*>(d_off), 0, 3 * nj, stream);
-            else
-                wqc::launch_fill_int(d_rot, 0, nj, stream);
             int wgs = 0;
             for (int j = 0; j < nj; ++j)
                 if (!done[j]) wgs += jpairs[j];
             const bool wide = use_wide && wgs <= cus;
             {
                 Timer tm(this, 3);
-                for (int r = 0; r < rounds; ++r)
-                    wqc::launch_jacobi_round_lo(dj, nj, pairs, r, tol, nu2, (float)floor_lo, inner_cap_lo, d_rot, d_done,
-                                                d_jstats, d_off, wide, precision == WQC_TN_PRECISION_MIXED_BF16, stream);
-                check_launch();
+                run_sweep(graphs[wide ? 1 : 0], [&] {
+                    if (d_off)
+                        wqc::launch_fill_int(reinterpret_cast<int*>(d_off), 0, 3 * nj, stream);
+                    else
+                        wqc::launch_fill_int(d_rot, 0, nj, stream);
+                    for (int r = 0; r < rounds; ++r)
+                        wqc::launch_jacobi_round_lo(dj, nj, pairs, r, tol, nu2, (float)floor_lo, inner_cap_lo, d_rot,
+                                                    d_done, d_jstats, d_off, wide, bf16, stream);
+                });
                 klaunch[3] += rounds;
             }
             stats.jacobi_sweeps_lo += active;
@@ -1018,7 +1090,8 @@ struct wqc_tn_state {
                     active--;
                     changed = true;
                 }
-            if (changed && active > 0) d_done = upload_ints(done);
+            if (changed && active > 0)
+                HIPC(hipMemcpyAsync(d_done, upload_ints(done), nj * sizeof(int), hipMemcpyDeviceToDevice, stream));
         }
         // J = (complex128) J32, made unitary to FP64 precision by two Newton-Schulz steps
         // J <- J (3I - J^H J) / 2 (the FP32 rounds leave it unitary to about 1e-6).
@@ -1308,7 +1381,15 @@ struct wqc_tn_state {
     struct GemmCall {
         size_t off = 0;
         int n = 0, mm = 1, mn = 1, splits = 1;
+        int tile = 64;               // the autotuner's choice for this shape class (or 64)
+        bool tuned = false;          // the class has a measured tile
+        std::array<int, 6> key{};    // shape class
     };
+    static int log2_bucket(int v) {
+        int b = 0;
+        while ((1 << b) < v && b < 30) ++b;
+        return b;
+    }
     GemmCall size_gemm(const std::vector<GemmJob>& g, size_t& need) const {
         GemmCall c;
         int mk = 1;
@@ -1325,7 +1406,48 @@ struct wqc_tn_state {
             for (auto& x : g) nd += (size_t)c.splits * x.M * x.N;
             need = std::max(need, nd);
         }
+        c.key = {log2_bucket(c.mm), log2_bucket(c.mn), log2_bucket(mk), log2_bucket(c.n), c.splits,
+                 g.empty() ? 0 : g[0].opA * 2 + g[0].opB};
+        std::lock_guard<std::mutex> lk(dev->mu);
+        auto it = dev->gemm_tile.find(c.key);
+        if (it != dev->gemm_tile.end()) {
+            c.tile = it->second;
+            c.tuned = true;
+        }
         return c;
+    }
+    // A product may be timed (run more than once) only when it overwrites its output: every job
+    // with beta = 0 and an output that is none of its inputs.
+    static bool rerunnable(const std::vector<GemmJob>& g) {
+        for (const auto& x : g)
+            if (x.beta.x != 0.0 || x.beta.y != 0.0 || x.C == x.A || x.C == x.B) return false;
+        return true;
+    }
+    bool autotune = true;  // WQC_TN_AUTOTUNE=0: no measurements (cached or default tiles)
+    int forced_tile = 0;   // WQC_TN_GEMM_TILE=32|64: that tile for every product, no tuning
+    // The autotuner: both tiles run twice on the call's own (rerunnable) inputs, alternating, and
+    // the faster minimum is kept for the shape class. Both tiles give the same bits, so the
+    // outputs are valid whichever ran last and the choice never changes a result.
+    void tune_call(const GemmJob* table, GemmCall& c, int family) {
+        hipEvent_t ev[5];
+        for (auto& e : ev) e = take_event();
+        HIPC(hipEventRecord(ev[0], stream));
+        for (int rep = 0; rep < 2; ++rep)
+            for (int t = 0; t < 2; ++t) {
+                GemmCall x = c;
+                x.tile = t == 0 ? 64 : 32;
+                launch_one(table, x, family);
+                HIPC(hipEventRecord(ev[1 + 2 * rep + t], stream));
+            }
+        HIPC(hipEventSynchronize(ev[4]));
+        float ms[4] = {0, 0, 0, 0};
+        for (int k = 0; k < 4; ++k) HIPC(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
+        const float t64 = std::min(ms[0], ms[2]), t32 = std::min(ms[1], ms[3]);
+        for (auto& e : ev) event_pool.push_back(e);
+        c.tile = t32 < t64 ? 32 : 64;
+        c.tuned = true;
+        std::lock_guard<std::mutex> lk(dev->mu);
+        if (dev->gemm_tile.emplace(c.key, c.tile).second) dev->save_tune_entry(c.key, c.tile);
     }
     void point_parts(std::vector<GemmJob>& g, const GemmCall& c) {
         if (c.splits <= 1) return;
@@ -1335,14 +1457,28 @@ struct wqc_tn_state {
             p += (size_t)c.splits * x.M * x.N;
         }
     }
-    void launch_call(const GemmJob* table, const GemmCall& c, int family) {
-        if (c.n == 0) return;
+    void launch_one(const GemmJob* table, const GemmCall& c, int family) {
         if (c.splits > 1)
-            wqc::launch_gemm_splitk(table + c.off, c.n, c.mm, c.mn, c.splits, stream);
+            wqc::launch_gemm_splitk(table + c.off, c.n, c.mm, c.mn, c.splits, stream, c.tile);
         else
-            wqc::launch_gemm(table + c.off, c.n, c.mm, c.mn, stream);
+            wqc::launch_gemm(table + c.off, c.n, c.mm, c.mn, stream, c.tile);
         check_launch();
         klaunch[family]++;
+    }
+    // Launches a call with its class's tile; an untuned class whose jobs may be rerun is measured
+    // first (its timing runs leave the product in place).
+    void launch_call(const GemmJob* table, GemmCall c, int family, const std::vector<GemmJob>* host = nullptr) {
+        if (c.n == 0) return;
+        if (forced_tile) {
+            c.tile = forced_tile;
+            launch_one(table, c, family);
+            return;
+        }
+        if (!c.tuned && autotune && host && rerunnable(*host)) {
+            tune_call(table, c, family);
+            return;
+        }
+        launch_one(table, c, family);
     }
     void gemm(std::vector<GemmJob>& g, int family = 2) {
         if (g.empty()) return;
@@ -1353,7 +1489,7 @@ struct wqc_tn_state {
             note_peak();
         }
         point_parts(g, c);
-        launch_call(upload(g), c, family);
+        launch_call(upload(g), c, family, &g);
     }
     // A fixed sequence of products (and other launches between them, run by `between`):
     // every call's jobs go up in one table upload instead of one per call.
@@ -1375,7 +1511,7 @@ struct wqc_tn_state {
         const GemmJob* table = all.empty() ? nullptr : upload(all);
         for (size_t i = 0; i < calls.size(); ++i) {
             between(i);
-            launch_call(table, cs[i], family);
+            launch_call(table, cs[i], family, &calls[i]);
         }
         between(calls.size());
     }
@@ -3014,6 +3150,8 @@ int wqc_tn_device_open(int ordinal, wqc_tn_device** out) {
         HIPC(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
         if (!sp) fail(WQC_TN_EXEC_FAILED, "scratch array has no device address");
         d->scratch = static_cast<double2*>(sp);
+        if (const char* e = std::getenv("WQC_TN_TUNE_CACHE")) d->tune_path = e;
+        d->load_tune_cache();
         *out = d.release();
     });
 }

@@ -195,8 +195,10 @@ struct ProbeJob {
 void launch_probe_residual(const ProbeJob* jobs, int njobs, hipStream_t s);
 void launch_row_norms(const JacobiJob* jobs, int njobs, int max_n, double* sigma,
                       const long* sigma_off, hipStream_t s);
-void launch_gemm(const GemmJob* jobs, int njobs, int max_m, int max_n, hipStream_t s);
-void launch_gemm_splitk(const GemmJob* jobs, int njobs, int max_m, int max_n, int splits, hipStream_t s);
+// tile: 64 (default) or 32 (four times the workgroups, the same bits).
+void launch_gemm(const GemmJob* jobs, int njobs, int max_m, int max_n, hipStream_t s, int tile = 64);
+void launch_gemm_splitk(const GemmJob* jobs, int njobs, int max_m, int max_n, int splits, hipStream_t s,
+                        int tile = 64);
 void launch_factor(const FactorJob* jobs, int njobs, long max_left, long max_right,
                    hipStream_t s);
 void launch_householder(const HouseJob* jobs, int njobs, hipStream_t s);

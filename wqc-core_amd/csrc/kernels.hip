@@ -1214,15 +1214,17 @@ __global__ __launch_bounds__(256) void k_row_norms(const JacobiJob* __restrict__
 // ------------------------------------------------------------------ MFMA GEMM
 // Batched complex FP64 GEMM C = alpha op(A) op(B) + beta C on v_mfma_f64_16x16x4f64 with 3M
 // products (E1 = Ar Br, E2 = Ai Bi, E3 = (Ar + Ai)(Br + Bi); Re = E1 - E2, Im = E3 - E1 - E2).
-// A workgroup computes a 64 x 64 tile, each wave a 32 x 32 quarter (2 x 2 MFMA tiles); K advances
-// in steps of 16 staged through LDS as real / imaginary planes (op C transposes and conjugates
-// while staging). Edges are zero-padded.
+// A workgroup computes a TL x TL tile (TL = 64: each wave a 32 x 32 quarter of 2 x 2 MFMA tiles;
+// TL = 32: each wave one 16 x 16 MFMA tile, four times the workgroups for skinny products); K
+// advances in steps of 16 staged through LDS as real / imaginary planes (op C transposes and
+// conjugates while staging). Edges are zero-padded. Every output element sees the same sequence
+// of MFMA accumulations over k at either tile size, so the two give bit-identical results.
 constexpr int MG = 64, MGK = 16;
 
-template <bool PARTIAL>
+template <bool PARTIAL, int TL>
 __global__ __launch_bounds__(256, 2) void k_gemm_mfma(const GemmJob* __restrict__ jobs, int splits) {
     const GemmJob jb = jobs[PARTIAL ? blockIdx.z / splits : blockIdx.z];
-    const int m0 = blockIdx.y * MG, n0 = blockIdx.x * MG;
+    const int m0 = blockIdx.y * TL, n0 = blockIdx.x * TL;
     if (m0 >= jb.M || n0 >= jb.N) return;
     int kbeg = 0, kend = jb.K;
     if (PARTIAL) {
@@ -1231,22 +1233,23 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(const GemmJob* __restrict_
         kbeg = slice * ks;
         kend = min(jb.K, kbeg + ks);
     }
-    __shared__ double Ar[MG][MGK + 1], Ai[MG][MGK + 1], Br[MGK][MG + 1], Bi[MGK][MG + 1];
+    __shared__ double Ar[TL][MGK + 1], Ai[TL][MGK + 1], Br[MGK][TL + 1], Bi[MGK][TL + 1];
+    constexpr int WT = TL / 2, NA = TL / 32;  // each wave: a WT x WT quarter of NA x NA MFMA tiles
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int wm = (wave >> 1) * WT, wn = (wave & 1) * WT;
     const int li = lane & 15, lk = lane >> 4;
-    d4v e1[2][2], e2[2][2], e3[2][2];
+    d4v e1[NA][NA], e2[NA][NA], e3[NA][NA];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < NA; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b < NA; ++b) {
             e1[a][b] = d4v{0, 0, 0, 0};
             e2[a][b] = e1[a][b];
             e3[a][b] = e1[a][b];
         }
     // The next K tile is fetched into registers while the MFMAs consume the current one from
     // LDS (the accumulation order over k is unchanged).
-    constexpr int PER = (MG * MGK) / 256;
+    constexpr int PER = (TL * MGK) / 256;
     double2 ra[PER], rb[PER];
     auto fetch = [&](int k0) {
 #pragma unroll
@@ -1256,11 +1259,11 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(const GemmJob* __restrict_
                 const int m = m0 + e / MGK, k = k0 + e % MGK;
                 ra[u] = (m < jb.M && k < kend) ? jb.A[(long)m * jb.lda + k] : cz();
             } else {
-                const int m = m0 + e % MG, k = k0 + e / MG;
+                const int m = m0 + e % TL, k = k0 + e / TL;
                 ra[u] = (m < jb.M && k < kend) ? cconj(jb.A[(long)k * jb.lda + m]) : cz();
             }
             if (jb.opB == 0) {
-                const int n = n0 + e % MG, k = k0 + e / MG;
+                const int n = n0 + e % TL, k = k0 + e / TL;
                 rb[u] = (n < jb.N && k < kend) ? jb.B[(long)k * jb.ldb + n] : cz();
             } else {
                 const int n = n0 + e / MGK, k = k0 + e % MGK;
@@ -1273,10 +1276,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(const GemmJob* __restrict_
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
             const int e = tid + 256 * u;
-            const int am = jb.opA == 0 ? e / MGK : e % MG, ak = jb.opA == 0 ? e % MGK : e / MG;
+            const int am = jb.opA == 0 ? e / MGK : e % TL, ak = jb.opA == 0 ? e % MGK : e / TL;
             Ar[am][ak] = ra[u].x;
             Ai[am][ak] = ra[u].y;
-            const int bn = jb.opB == 0 ? e % MG : e / MGK, bk = jb.opB == 0 ? e / MG : e % MGK;
+            const int bn = jb.opB == 0 ? e % TL : e / MGK, bk = jb.opB == 0 ? e / TL : e % MGK;
             Br[bk][bn] = rb[u].x;
             Bi[bk][bn] = rb[u].y;
         }
@@ -1285,9 +1288,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(const GemmJob* __restrict_
 #pragma unroll
         for (int ks = 0; ks < MGK / 4; ++ks) {
             const int kk = 4 * ks + lk;
-            double ar[2], ai[2], as[2], br[2], bi[2], bs[2];
+            double ar[NA], ai[NA], as[NA], br[NA], bi[NA], bs[NA];
 #pragma unroll
-            for (int a = 0; a < 2; ++a) {
+            for (int a = 0; a < NA; ++a) {
                 ar[a] = Ar[wm + 16 * a + li][kk];
                 ai[a] = Ai[wm + 16 * a + li][kk];
                 as[a] = ar[a] + ai[a];
@@ -1296,9 +1299,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(const GemmJob* __restrict_
                 bs[a] = br[a] + bi[a];
             }
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < NA; ++a)
 #pragma unroll
-                for (int b = 0; b < 2; ++b) {
+                for (int b = 0; b < NA; ++b) {
                     e1[a][b] = mfma_f64(ar[a], br[b], e1[a][b]);
                     e2[a][b] = mfma_f64(ai[a], bi[b], e2[a][b]);
                     e3[a][b] = mfma_f64(as[a], bs[b], e3[a][b]);
@@ -1308,9 +1311,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(const GemmJob* __restrict_
     }
     // MFMA f64 output: element j of lane l is row 4j + l/16, column l%16 of the 16 x 16 tile.
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < NA; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < NA; ++b)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int m = m0 + wm + 16 * a + 4 * j + lk, n = n0 + wn + 16 * b + li;
@@ -2169,14 +2172,24 @@ void launch_env_close(const CloseJob* jobs, int njobs, hipStream_t s) {
     hipLaunchKernelGGL(k_env_close, dim3(njobs), dim3(256), 0, s, jobs);
 }
 
-void launch_gemm(const GemmJob* jobs, int njobs, int max_m, int max_n, hipStream_t s) {
+void launch_gemm(const GemmJob* jobs, int njobs, int max_m, int max_n, hipStream_t s, int tile) {
+    if (tile == 32) {
+        dim3 grid((max_n + 31) / 32, (max_m + 31) / 32, njobs);
+        hipLaunchKernelGGL((k_gemm_mfma<false, 32>), grid, dim3(256), 0, s, jobs, 1);
+        return;
+    }
     dim3 grid((max_n + MG - 1) / MG, (max_m + MG - 1) / MG, njobs);
-    hipLaunchKernelGGL(k_gemm_mfma<false>, grid, dim3(256), 0, s, jobs, 1);
+    hipLaunchKernelGGL((k_gemm_mfma<false, MG>), grid, dim3(256), 0, s, jobs, 1);
 }
 
-void launch_gemm_splitk(const GemmJob* jobs, int njobs, int max_m, int max_n, int splits, hipStream_t s) {
-    dim3 grid((max_n + MG - 1) / MG, (max_m + MG - 1) / MG, njobs * splits);
-    hipLaunchKernelGGL(k_gemm_mfma<true>, grid, dim3(256), 0, s, jobs, splits);
+void launch_gemm_splitk(const GemmJob* jobs, int njobs, int max_m, int max_n, int splits, hipStream_t s, int tile) {
+    if (tile == 32) {
+        dim3 grid((max_n + 31) / 32, (max_m + 31) / 32, njobs * splits);
+        hipLaunchKernelGGL((k_gemm_mfma<true, 32>), grid, dim3(256), 0, s, jobs, splits);
+    } else {
+        dim3 grid((max_n + MG - 1) / MG, (max_m + MG - 1) / MG, njobs * splits);
+        hipLaunchKernelGGL((k_gemm_mfma<true, MG>), grid, dim3(256), 0, s, jobs, splits);
+    }
     hipLaunchKernelGGL(k_gemm_splitk_reduce, dim3(blocks_for((long)max_m * max_n, 256, 256), njobs), dim3(256), 0,
                        s, jobs, splits);
 }
