@@ -66,7 +66,8 @@ def test_config4_batched_and_threaded(wt):
     amps_t, codes_t = wt.run_slices(slices, workers=8)
     tt = time.time() - t
     say("config4: 256 slices batched %.2f s, threaded (8 workers) %.2f s" % (tb, tt))
-    assert all(c == wt.OK for c in codes_b) and all(c == wt.OK for c in codes_t)
+    bad = [(i, codes_b[i], codes_t[i]) for i in range(256) if codes_b[i] != wt.OK or codes_t[i] != wt.OK]
+    assert not bad, bad
     assert np.all(np.isfinite(amps_b)) and np.all(np.isfinite(amps_t))
     worst, exact = 0.0, 0
     for i in range(0, 256, 4):
@@ -97,7 +98,7 @@ def test_config4_slices_vs_oracle(wt):
         o.apply_circuit(gates)
         want = o.amplitude_at_compact_zero()
         _, bonds, _ = one_at_a_time(wt, n, chi, gates)
-        assert bonds == o.bond_dims(), i
+        assert bonds == o.bond_dims(), (i, [(q, x, y) for q, (x, y) in enumerate(zip(bonds, o.bond_dims())) if x != y])
         assert o.counters()["rsvd_kept"] == 0
         assert rel(amps[i], want) <= 1e-10, (i, amps[i], want)
         checked += 1

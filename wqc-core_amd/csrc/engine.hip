@@ -3621,6 +3621,9 @@ int wqc_tn_run_slices(wqc_tn_device* dev, const wqc_tn_slice* slices, size_t n_s
                     if (!st) {
                         st = std::make_unique<wqc_tn_state>();
                         st->init(dev, sl.qubit_count, sl.max_bond_dim, precision);
+                        // Sweep graphs are captured on one thread's stream while the other
+                        // workers allocate, free and copy: capture stays off here.
+                        st->use_graph = false;
                     } else {
                         st->reset(sl.qubit_count, sl.max_bond_dim);
                     }

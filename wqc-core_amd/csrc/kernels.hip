@@ -40,6 +40,12 @@ __device__ __forceinline__ void ccmac(double2& acc, double2 a, double2 b) {
     acc.y = fma(-a.y, b.x, acc.y);
 }
 
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4v mfma_f64(double a, double b, d4v c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -68,60 +74,118 @@ __global__ __launch_bounds__(256) void k_one_qubit(const OneQubitJob* __restrict
 }
 
 // ------------------------------------------------------------------ merge + U
-// One workgroup owns 16 values of a and 16 values of b, i.e. theta rows {2a, 2a+1} and
-// columns {b, dr + b}: exactly the four entries (s, t) that the two-qubit unitary mixes.
-constexpr int MT = 16;
-constexpr int MK = 16;
+// theta' = U4 (L R) on FP64 MFMA (v_mfma_f64_16x16x4f64, four real products per complex one:
+// Re = Lr Rr - Li Ri, Im = Lr Ri + Li Rr, so every entry carries the rounding of a plain complex
+// product; exact zeros of the spectrum stay at that noise floor). A workgroup owns 32
+// values of a and 32 of b: theta rows {2a, 2a+1} and columns {b, dr + b}, the four entries (s, t)
+// that the two-qubit unitary mixes. K advances in steps of 16 staged through LDS as real /
+// imaginary planes (L rows ordered a*2 + s, R columns t*32 + b). Wave w holds the 16 x 16 (a, b)
+// quarter (16 (w >> 1), 16 (w & 1)) for all four (s, t), so the unitary is applied in registers
+// and theta' (or its conjugate transpose) is written directly.
+constexpr int MA = 32, MK = 16;
 
-__global__ __launch_bounds__(256) void k_merge(const MergeJob* __restrict__ jobs) {
+__global__ __launch_bounds__(256, 2) void k_merge(const MergeJob* __restrict__ jobs) {
     const MergeJob jb = jobs[blockIdx.z];
-    const int a0 = blockIdx.y * MT, b0 = blockIdx.x * MT;
+    const int a0 = blockIdx.y * MA, b0 = blockIdx.x * MA;
     if (a0 >= jb.dl || b0 >= jb.dr) return;
-    __shared__ double2 Ls[2 * MT][MK + 1];
-    __shared__ double2 Rs[MK][2 * MT + 1];
-    const int tid = threadIdx.x;
-    const int ta = tid >> 4, tb = tid & 15;
-    double2 acc00 = cz(), acc01 = cz(), acc10 = cz(), acc11 = cz();
+    __shared__ double Lr[2 * MA][MK + 1], Li[2 * MA][MK + 1], Rr[MK][2 * MA + 1], Ri[MK][2 * MA + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int aw = (wave >> 1) * 16, bw = (wave & 1) * 16;
+    const int li = lane & 15, lk = lane >> 4;
     const long ldl = jb.bond, ldr = 2L * jb.dr;
-    for (int k0 = 0; k0 < jb.bond; k0 += MK) {
-        for (int e = tid; e < 2 * MT * MK; e += 256) {
-            const int lr = e / MK, kk = e % MK;
-            const int a = a0 + (lr >> 1), s = lr & 1, k = k0 + kk;
-            Ls[lr][kk] = (a < jb.dl && k < jb.bond) ? jb.L[(long)(2 * a + s) * ldl + k] : cz();
+    d4v e1[2][2], e2[2][2], e3[2][2], e4[2][2];  // [s][t]: Lr Rr, Li Ri, Lr Ri, Li Rr
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            e1[s][t] = d4v{0, 0, 0, 0};
+            e2[s][t] = e1[s][t];
+            e3[s][t] = e1[s][t];
+            e4[s][t] = e1[s][t];
         }
-        for (int e = tid; e < MK * 2 * MT; e += 256) {
-            const int kk = e / (2 * MT), lc = e % (2 * MT);
-            const int t = lc / MT, b = b0 + (lc % MT), k = k0 + kk;
-            Rs[kk][lc] = (b < jb.dr && k < jb.bond) ? jb.R[(long)k * ldr + (long)t * jb.dr + b] : cz();
+    // Each thread stages 4 entries of L and 4 of R per K tile; the next tile is fetched into
+    // registers while the MFMAs consume the current one.
+    constexpr int PER = (2 * MA * MK) / 256;
+    double2 rl[PER], rr[PER];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int e = tid + 256 * u;
+            {
+                const int lr = e / MK, kk = e % MK;
+                const int a = a0 + (lr >> 1), s = lr & 1, k = k0 + kk;
+                rl[u] = (a < jb.dl && k < jb.bond) ? jb.L[(long)(2 * a + s) * ldl + k] : cz();
+            }
+            {
+                const int kk = e / (2 * MA), lc = e % (2 * MA);
+                const int t = lc / MA, b = b0 + (lc % MA), k = k0 + kk;
+                rr[u] = (b < jb.dr && k < jb.bond) ? jb.R[(long)k * ldr + (long)t * jb.dr + b] : cz();
+            }
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < jb.bond; k0 += MK) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int e = tid + 256 * u;
+            Lr[e / MK][e % MK] = rl[u].x;
+            Li[e / MK][e % MK] = rl[u].y;
+            Rr[e / (2 * MA)][e % (2 * MA)] = rr[u].x;
+            Ri[e / (2 * MA)][e % (2 * MA)] = rr[u].y;
         }
         __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < MK; ++kk) {
-            const double2 l0 = Ls[2 * ta][kk], l1 = Ls[2 * ta + 1][kk];
-            const double2 r0 = Rs[kk][tb], r1 = Rs[kk][MT + tb];
-            cmac(acc00, l0, r0);
-            cmac(acc01, l0, r1);
-            cmac(acc10, l1, r0);
-            cmac(acc11, l1, r1);
+        if (k0 + MK < jb.bond) fetch(k0 + MK);
+#pragma unroll
+        for (int ks = 0; ks < MK / 4; ++ks) {
+            const int kk = 4 * ks + lk;
+            double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                ar[s] = Lr[(aw + li) * 2 + s][kk];
+                ai[s] = Li[(aw + li) * 2 + s][kk];
+            }
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                br[t] = Rr[kk][t * MA + bw + li];
+                bi[t] = Ri[kk][t * MA + bw + li];
+            }
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    e1[s][t] = mfma_f64(ar[s], br[t], e1[s][t]);
+                    e2[s][t] = mfma_f64(ai[s], bi[t], e2[s][t]);
+                    e3[s][t] = mfma_f64(ar[s], bi[t], e3[s][t]);
+                    e4[s][t] = mfma_f64(ai[s], br[t], e4[s][t]);
+                }
         }
         __syncthreads();
     }
-    const int a = a0 + ta, b = b0 + tb;
-    if (a >= jb.dl || b >= jb.dr) return;
-    const double2 th[4] = {acc00, acc01, acc10, acc11};  // index s*2 + t
+    // MFMA f64 output: element j of lane l is row 4j + l/16 (a), column l%16 (b) of the quarter.
 #pragma unroll
-    for (int so = 0; so < 2; ++so)
+    for (int j = 0; j < 4; ++j) {
+        const int a = a0 + aw + 4 * j + lk, b = b0 + bw + li;
+        if (a >= jb.dl || b >= jb.dr) continue;
+        double2 th[4];  // index s*2 + t
 #pragma unroll
-        for (int to = 0; to < 2; ++to) {
-            double2 v = cz();
+        for (int s = 0; s < 2; ++s)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) cmac(v, jb.u4[(so * 2 + to) * 4 + i], th[i]);
-            const long row = 2L * a + so, col = (long)to * jb.dr + b;
-            if (!jb.transpose)
-                jb.out[row * ldr + col] = v;
-            else
-                jb.out[col * (2L * jb.dl) + row] = cconj(v);
-        }
+            for (int t = 0; t < 2; ++t)
+                th[s * 2 + t] = make_double2(e1[s][t][j] - e2[s][t][j], e3[s][t][j] + e4[s][t][j]);
+#pragma unroll
+        for (int so = 0; so < 2; ++so)
+#pragma unroll
+            for (int to = 0; to < 2; ++to) {
+                double2 v = cz();
+#pragma unroll
+                for (int i = 0; i < 4; ++i) cmac(v, jb.u4[(so * 2 + to) * 4 + i], th[i]);
+                const long row = 2L * a + so, col = (long)to * jb.dr + b;
+                if (!jb.transpose)
+                    jb.out[row * ldr + col] = v;
+                else
+                    jb.out[col * (2L * jb.dl) + row] = cconj(v);
+            }
+    }
 }
 
 // ------------------------------------------------------------------ sums of squares
@@ -233,7 +297,6 @@ __device__ __forceinline__ void pair_order(int tid, const int* s_row, const Key*
     dst[tid] = out;
 }
 
-typedef double d4v __attribute__((ext_vector_type(4)));
 
 // Reciprocal and reciprocal square root from the hardware estimates plus two Newton steps
 // (full FP64 precision), a third of the dependent instructions of IEEE division and sqrt.
@@ -321,9 +384,6 @@ __device__ __forceinline__ Rot64 rot_from_lane(const Rot64& own, int src) {
     return r;
 }
 
-__device__ __forceinline__ d4v mfma_f64(double a, double b, d4v c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
 
 // NT = 256: four waves, three workgroups per CU (the bulk of a batch's launches). NT = 512:
 // eight waves share the Gram and the rotation of one block pair, for launches with no more
@@ -2051,7 +2111,7 @@ void launch_one_qubit(const OneQubitJob* jobs, int njobs, int max_elems, hipStre
 }
 
 void launch_merge(const MergeJob* jobs, int njobs, int max_dl, int max_dr, hipStream_t s) {
-    dim3 grid((max_dr + MT - 1) / MT, (max_dl + MT - 1) / MT, njobs);
+    dim3 grid((max_dr + MA - 1) / MA, (max_dl + MA - 1) / MA, njobs);
     hipLaunchKernelGGL(k_merge, grid, dim3(256), 0, s, jobs);
 }
 
