@@ -286,6 +286,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-units", type=int, default=2, help="saturated units of the one-thread CPU leg")
     ap.add_argument("--sat-units", type=int, default=16, help="saturated units of the GPU batch")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the timed steps and the event-timed pass (no per-layer errors, saturated "
+                         "batch or CPU leg): the command the rocprofv3 summaries profile")
     ap.add_argument("--cpu-all-cores", action="store_true",
                     help="add the all-core CPU leg (one saturated unit per core, about two minutes)")
     ap.add_argument("--form", default="all", choices=["all", "both", "fp64", "mixed", "mixed_bf16"],
@@ -297,14 +300,21 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE readings scale the counter runs")
     args = ap.parse_args()
     if args.copy_probe:
-        import torch
+        # Three device-to-device copies of 256 MiB through the HIP runtime (its copy kernel).
+        import ctypes as ct
 
-        x = torch.ones((256 << 20) // 4, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
+        hip = ct.CDLL("libamdhip64.so")
+        nbytes = 256 << 20
+        a, b = ct.c_void_p(), ct.c_void_p()
+        assert hip.hipMalloc(ct.byref(a), ct.c_size_t(nbytes)) == 0
+        assert hip.hipMalloc(ct.byref(b), ct.c_size_t(nbytes)) == 0
+        assert hip.hipMemset(a, 1, ct.c_size_t(nbytes)) == 0
         for _ in range(3):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        print("# copied 268435456 bytes three times", flush=True)
+            assert hip.hipMemcpy(b, a, ct.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+        assert hip.hipDeviceSynchronize() == 0
+        hip.hipFree(a)
+        hip.hipFree(b)
+        print(f"# copied {nbytes} bytes three times", flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -374,9 +384,11 @@ def main():
     prec_of = dict(all_forms)
     low_forms = [(k, prec_of[k]) for k in forms if k != "fp64"]
     print("# per-layer errors on identical inputs", file=sys.stderr, flush=True)
-    per_layer = per_layer_errors(wt, n, chi, gates, low_forms) if low_forms and args.depth > 24 else {}
+    extras = not args.no_extras
+    per_layer = per_layer_errors(wt, n, chi, gates, low_forms) if extras and low_forms and args.depth > 24 else {}
     print("# saturated units", file=sys.stderr, flush=True)
-    pairs, sat_positions = saturated_units(wt, n, chi, gates, max(args.sat_units, 16)) if n >= 40 else ([], [])
+    pairs, sat_positions = (saturated_units(wt, n, chi, gates, max(args.sat_units, 16))
+                            if extras and n >= 40 else ([], []))
     form_report = {}
     for name, r in forms.items():
         kq = r["kernel_ms"].get("one_qubit", 0.0)
