@@ -7,15 +7,17 @@ one pass of the whole circuit from |0...0> through the C ABI (wqc_tn_apply_circu
 the final <0...0|psi>; inputs (the gate list) are on the host, every site tensor lives in
 HBM for the whole step.
 
-Printed JSON line: value = merges/s of the mixed form (Householder-LQ-preconditioned Jacobi:
-up to 5 FP32 sweeps, FP64 finish). Riding along: the plain FP64 form's rate and their ratio;
+Printed JSON line: value = merges/s of the faster mixed form (Householder-LQ-preconditioned
+Jacobi: up to 5 low sweeps with an FP32 or a split-BF16 Gram, FP64 finish). Riding along: every
+form's rate (plain FP64, mixed FP32, mixed split-BF16) and the headline's ratio to FP64;
 per form the amplitude error against the complex128 oracle after the range-finder-free first
 24 layers (the golden checkpoint), and for layers 25-30 the error of each layer on identical
 inputs (the FP64 form's state copied into the form's state before the layer: amplitude and
 per-split discarded weight against the FP64 form, whose splits match the oracle's to 1e-13);
 the roofline record of the dominant kernel family (Jacobi) and of the one-qubit kernel (GB/s);
 the GPU rate on a batch of saturated chi = 512 units taken from the workload's own state; and
-the CPU baseline: the FP64 oracle on the same saturated units, one thread and all host cores.
+the CPU baseline: the FP64 oracle on the same saturated units, one thread (all host cores
+with --cpu-all-cores).
 `--gpus N` with N > 1 exits: this build runs on one GPU.
 """
 

@@ -9,7 +9,15 @@
   device split: the 2 x 2 matrix on which nalgebra returned a top singular value of 1.00795
   is split with top singular value 1 to 1e-12 and a rebuild error below 1e-12, the split's
   checks (norm and probe vectors of svd_reproduces, src/tn/svd_guard.rs:242-283) passing.
+* the probe half of svd_reproduces (src/tn/svd_guard.rs:261-280) rejects a factorization that
+  the norm half accepts: with the FP64 sweep cap set to 1 from the host (WQC_TN_JACOBI_MAX),
+  the rotated rows of a 64-row split are not yet orthogonal, their norms still sum to |A|_F^2
+  (the rotation is unitary), and the split must fail with EXEC_FAILED through the probe
+  residuals; the same circuit at the default cap runs with no check failure.
 """
+
+import os
+import re
 
 import numpy as np
 import pytest
@@ -82,3 +90,36 @@ def test_counters_of_a_truncating_run(wt):
     assert s["svd_check_failures"] == 0
     assert s["jacobi_unconverged"] == 0
     assert s["peak_vram_bytes"] >= sum(st.site(i).nbytes for i in range(n))
+
+
+def test_probe_check_rejects_an_unconverged_split(wt):
+    n, chi = 10, 64
+    gates = C.rotation_cz_brickwork(n, 12, seed=314)
+    os.environ["WQC_TN_JACOBI_MAX"] = "1"
+    try:
+        st = wt.MpsState(n, chi)
+        with pytest.raises(wt.WqcTnError) as e:
+            st.apply_circuit(gates)
+        s = st.stats()
+        st.free()
+    finally:
+        del os.environ["WQC_TN_JACOBI_MAX"]
+    assert e.value.code == wt.EXEC_FAILED
+    msg = str(e.value)
+    print(msg, flush=True)
+    assert "failed its check" in msg
+    g = re.search(r"sum s\^2 = ([0-9.eE+-]+), \|A\|_F\^2 = ([0-9.eE+-]+), probe residuals ([0-9.eE+-]+) "
+                  r"([0-9.eE+-]+), job \d+ of \d+, n=(\d+) m=(\d+)", msg)
+    assert g, msg
+    s2, fro2, r0, r1 = (float(g.group(i)) for i in range(1, 5))
+    m = int(g.group(6))
+    # the norm half passes: the rows were only rotated
+    assert abs(s2 - fro2) <= 1e-9 * fro2
+    # the probe half fails: a residual above 1e-9 |A|_F |x| + 1e-14, |x| = sqrt(m)
+    assert max(r0, r1) > 1e-9 * np.sqrt(fro2) * np.sqrt(m) + 1e-14
+    assert s["svd_check_failures"] >= 1 and s["jacobi_unconverged"] >= 1
+    st = wt.MpsState(n, chi)
+    st.apply_circuit(gates)
+    s = st.stats()
+    st.free()
+    assert s["svd_check_failures"] == 0 and s["jacobi_unconverged"] == 0

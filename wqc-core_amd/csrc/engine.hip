@@ -586,6 +586,7 @@ struct wqc_tn_state {
     double tol_lo = JACOBI_TOL_LO;  // their relative tolerance (WQC_TN_JACOBI_TOL_LO)
     double floor_lo = 1e-12;        // squared-norm floor relative to a pair's largest row (WQC_TN_JACOBI_FLOOR_LO)
     int max_sweeps_lo = LO_SWEEP_CAP;  // cap on the low sweeps per job (WQC_TN_JACOBI_MAX_LO)
+    int max_sweeps = MAX_SWEEPS;       // cap on the FP64 sweeps per job (WQC_TN_JACOBI_MAX, 1..60)
     bool trace_jacobi = false;
     double stall_off = STALL_OFF;
     bool use_lq = true;
@@ -626,6 +627,10 @@ struct wqc_tn_state {
         if (const char* e = std::getenv("WQC_TN_JACOBI_MAX_LO")) {
             const int v = std::atoi(e);
             if (v >= 1 && v <= MAX_SWEEPS_LO) max_sweeps_lo = v;
+        }
+        if (const char* e = std::getenv("WQC_TN_JACOBI_MAX")) {
+            const int v = std::atoi(e);
+            if (v >= 1 && v <= MAX_SWEEPS) max_sweeps = v;
         }
         if (const char* e = std::getenv("WQC_TN_JACOBI_TOL_LO")) {
             const double v = std::atof(e);
@@ -880,7 +885,7 @@ struct wqc_tn_state {
         }
         const int cus = dev->prop.multiProcessorCount;
         SweepGraph graphs[2];  // 256-thread and 512-thread rounds
-        for (int sweep = 0; sweep < MAX_SWEEPS && active > 0; ++sweep) {
+        for (int sweep = 0; sweep < max_sweeps && active > 0; ++sweep) {
             check_cancel();
             int wgs = 0;
             for (int j = 0; j < nj; ++j)
@@ -935,7 +940,7 @@ struct wqc_tn_state {
             if (changed && active > 0)
                 HIPC(hipMemcpyAsync(d_done, upload_ints(done), nj * sizeof(int), hipMemcpyDeviceToDevice, stream));
         }
-        // A job still rotating after MAX_SWEEPS is counted; as the reference's Jacobi at its sweep
+        // A job still rotating after max_sweeps is counted; as the reference's Jacobi at its sweep
         // cap (src/tn/svd_guard.rs:165-229, then svd_reproduces), its factorization is kept when
         // it passes the norm and probe checks, which the caller applies to every split.
         for (int j = 0; j < nj; ++j)
