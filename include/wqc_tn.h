@@ -351,6 +351,12 @@ size_t wqc_tn_split_log(const wqc_tn_state *st, int32_t *out, size_t cap_entries
  * truncation, no counterpart in the reference. Copies up to cap entries, returns the number
  * recorded. */
 size_t wqc_tn_split_discarded(const wqc_tn_state *st, double *out, size_t cap_entries);
+/* The probe residual of each logged unit's kept factorization, in split-log order: the larger of
+ * |A x_p - U S V^H x_p| / (|A|_F |x_p|) over the two probe vectors of svd_reproduces
+ * (src/tn/svd_guard.rs:261-280; A is the matrix the checked SVD took). The reference computes it
+ * inside svd_reproduces and keeps only the verdict; this exposes the value. Copies up to cap
+ * entries, returns the number recorded. */
+size_t wqc_tn_split_probe_residuals(const wqc_tn_state *st, double *out, size_t cap_entries);
 
 #ifdef __cplusplus
 }

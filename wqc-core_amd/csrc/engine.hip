@@ -478,6 +478,7 @@ struct wqc_tn_state {
     bool split_log_on = false;
     std::vector<int32_t> split_log;
     std::vector<double> split_dw;  // discarded weight of each logged unit
+    std::vector<double> split_probe;  // relative probe residual of each logged unit
     std::vector<PendingOne> pend1;
     std::vector<Unit> pend2;
     std::vector<char> touched;
@@ -586,7 +587,6 @@ struct wqc_tn_state {
     double tol_lo = JACOBI_TOL_LO;  // their relative tolerance (WQC_TN_JACOBI_TOL_LO)
     double floor_lo = 1e-12;        // squared-norm floor relative to a pair's largest row (WQC_TN_JACOBI_FLOOR_LO)
     int max_sweeps_lo = LO_SWEEP_CAP;  // cap on the low sweeps per job (WQC_TN_JACOBI_MAX_LO)
-    int max_sweeps = MAX_SWEEPS;       // cap on the FP64 sweeps per job (WQC_TN_JACOBI_MAX, 1..60)
     bool trace_jacobi = false;
     double stall_off = STALL_OFF;
     bool use_lq = true;
@@ -627,10 +627,6 @@ struct wqc_tn_state {
         if (const char* e = std::getenv("WQC_TN_JACOBI_MAX_LO")) {
             const int v = std::atoi(e);
             if (v >= 1 && v <= MAX_SWEEPS_LO) max_sweeps_lo = v;
-        }
-        if (const char* e = std::getenv("WQC_TN_JACOBI_MAX")) {
-            const int v = std::atoi(e);
-            if (v >= 1 && v <= MAX_SWEEPS) max_sweeps = v;
         }
         if (const char* e = std::getenv("WQC_TN_JACOBI_TOL_LO")) {
             const double v = std::atof(e);
@@ -701,6 +697,7 @@ struct wqc_tn_state {
         pend2.clear();
         split_log.clear();
         split_dw.clear();
+        split_probe.clear();
         stats = wqc_tn_stats{};
         cancel.store(0);
         reject = false;
@@ -885,7 +882,7 @@ struct wqc_tn_state {
         }
         const int cus = dev->prop.multiProcessorCount;
         SweepGraph graphs[2];  // 256-thread and 512-thread rounds
-        for (int sweep = 0; sweep < max_sweeps && active > 0; ++sweep) {
+        for (int sweep = 0; sweep < MAX_SWEEPS && active > 0; ++sweep) {
             check_cancel();
             int wgs = 0;
             for (int j = 0; j < nj; ++j)
@@ -940,7 +937,7 @@ struct wqc_tn_state {
             if (changed && active > 0)
                 HIPC(hipMemcpyAsync(d_done, upload_ints(done), nj * sizeof(int), hipMemcpyDeviceToDevice, stream));
         }
-        // A job still rotating after max_sweeps is counted; as the reference's Jacobi at its sweep
+        // A job still rotating after MAX_SWEEPS is counted; as the reference's Jacobi at its sweep
         // cap (src/tn/svd_guard.rs:165-229, then svd_reproduces), its factorization is kept when
         // it passes the norm and probe checks, which the caller applies to every split.
         for (int j = 0; j < nj; ++j)
@@ -1242,6 +1239,7 @@ struct wqc_tn_state {
         bool fallback;
         int sweeps = 0, sweeps_lo = 0;
         double dw = 0.0;  // discarded weight sum_{k >= chi} s_k^2 / sum s_k^2 of the split
+        double probe = 0.0;  // max_p |A x_p - U S V^H x_p| / (|A|_F |x_p|) of the kept factorization
     };
 
     void run_units(std::vector<Unit>& units) {
@@ -1374,6 +1372,7 @@ struct wqc_tn_state {
                                       j.sweeps, j.sweeps_lo};
                 j.st->split_log.insert(j.st->split_log.end(), e, e + 8);
                 j.st->split_dw.push_back(j.dw);
+                j.st->split_probe.push_back(j.probe);
             }
     }
 
@@ -1586,6 +1585,14 @@ struct wqc_tn_state {
             note_peak();
         }
         return reinterpret_cast<double2*>(probe_p_buf.p);
+    }
+
+    // The larger relative probe residual of job k of a finished batch (|A|_F^2 = fro2).
+    double probe_rel(const SplitBatch& b, int k, double fro2) const {
+        const double scale = std::sqrt(std::max(fro2, 2.2250738585072014e-308));
+        const double xn = std::sqrt((double)b.m0[k]);
+        const double e = std::max(hread[b.hprobe + 2 * (size_t)k], hread[b.hprobe + 2 * (size_t)k + 1]);
+        return std::sqrt(e) / (scale * xn);
     }
 
     // Whether job k of a finished batch passes the probe check (|A|_F^2 = fro2).
@@ -2037,6 +2044,7 @@ struct wqc_tn_state {
             }
             const bool norm_ok =
                 std::fabs(s2 - fro2) <= SVD_CHECK_TOL * std::max(fro2, 2.2250738585072014e-308) + 1e-28;
+            j.probe = probe_rel(b, k, fro2);
             if (!norm_ok || !probes_pass(b, k, fro2)) {
                 j.st->stats.svd_check_failures++;
                 char buf[256];
@@ -2130,6 +2138,7 @@ struct wqc_tn_state {
             }
             const bool svd_ok = std::fabs(s2 - fro2) <= SVD_CHECK_TOL * std::max(fro2, 2.2250738585072014e-308) + 1e-28 &&
                                 probes_pass(b, k, fro2);
+            j.probe = probe_rel(b, k, fro2);
             wqc_tn_stats& sx = j.st->stats;
             if (!svd_ok) sx.svd_check_failures++;
             for (double& s : values)
@@ -3988,6 +3997,7 @@ int wqc_tn_enable_split_log(wqc_tn_state* st, int enable) {
     if (!enable) {
         st->split_log.clear();
         st->split_dw.clear();
+        st->split_probe.clear();
     }
     return WQC_TN_OK;
 }
@@ -4006,6 +4016,13 @@ size_t wqc_tn_split_discarded(const wqc_tn_state* st, double* out, size_t cap_en
     if (!st) return 0;
     const size_t m = st->split_dw.size();
     if (out) std::memcpy(out, st->split_dw.data(), std::min(m, cap_entries) * sizeof(double));
+    return m;
+}
+
+size_t wqc_tn_split_probe_residuals(const wqc_tn_state* st, double* out, size_t cap_entries) {
+    if (!st) return 0;
+    const size_t m = st->split_probe.size();
+    if (out) std::memcpy(out, st->split_probe.data(), std::min(m, cap_entries) * sizeof(double));
     return m;
 }
 

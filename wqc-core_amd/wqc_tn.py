@@ -34,7 +34,7 @@ EXPORTED = (
     "wqc_tn_pauli_expectations", "wqc_tn_observable_expectation", "wqc_tn_sample_counts",
     "wqc_tn_run_slices", "wqc_tn_backward", "wqc_tn_overlap", "wqc_tn_contract_time_split",
     "wqc_tn_site_amplitudes_batch", "wqc_tn_outcome_probabilities", "wqc_tn_copy_sites",
-    "wqc_tn_amplitude_at_compact_zero_noalloc", "wqc_tn_split_discarded",
+    "wqc_tn_amplitude_at_compact_zero_noalloc", "wqc_tn_split_discarded", "wqc_tn_split_probe_residuals",
 )
 
 
@@ -134,6 +134,8 @@ def load():
     L.wqc_tn_split_log.argtypes = [vp, ct.POINTER(ct.c_int32), sz]
     L.wqc_tn_split_discarded.restype = sz
     L.wqc_tn_split_discarded.argtypes = [vp, dp, sz]
+    L.wqc_tn_split_probe_residuals.restype = sz
+    L.wqc_tn_split_probe_residuals.argtypes = [vp, dp, sz]
     _lib = L
     return L
 
@@ -463,6 +465,13 @@ class MpsState:
         m = load().wqc_tn_split_discarded(self._h, None, 0)
         out = np.zeros(max(1, m))
         load().wqc_tn_split_discarded(self._h, _dp(out), m)
+        return out[:m].tolist()
+
+    def split_probe_residuals(self):
+        """Relative probe residual of each unit's kept factorization, in split-log order."""
+        m = load().wqc_tn_split_probe_residuals(self._h, None, 0)
+        out = np.zeros(max(1, m))
+        load().wqc_tn_split_probe_residuals(self._h, _dp(out), m)
         return out[:m].tolist()
 
 
