@@ -884,6 +884,7 @@ struct wqc_tn_state {
         SweepGraph graphs[2];  // 256-thread and 512-thread rounds
         for (int sweep = 0; sweep < MAX_SWEEPS && active > 0; ++sweep) {
             check_cancel();
+            const auto t_sweep = std::chrono::steady_clock::now();
             int wgs = 0;
             for (int j = 0; j < nj; ++j)
                 if (!done[j]) wgs += jpairs[j];
@@ -905,7 +906,9 @@ struct wqc_tn_state {
             const unsigned long long* offv = reinterpret_cast<const unsigned long long*>(hread);
             const int* rot = reinterpret_cast<const int*>(offv + nj);
             if (trace_jacobi) {
-                std::fprintf(stderr, "jacobi sweep %d active %d:", sweep + 1, active);
+                std::fprintf(stderr, "jacobi sweep %d active %d wgs %d%s %.3f ms:", sweep + 1, active, wgs,
+                             wide ? " wide" : "",
+                             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_sweep).count());
                 int shown = 0;
                 for (int j = 0; j < nj && shown < 4; ++j) {
                     if (done[j] || jobs[j].n < max_n) continue;
@@ -1047,6 +1050,7 @@ struct wqc_tn_state {
         const bool bf16 = precision == WQC_TN_PRECISION_MIXED_BF16;
         for (int sweep = 0; sweep < max_sweeps_lo && active > 0; ++sweep) {
             check_cancel();
+            const auto t_sweep = std::chrono::steady_clock::now();
             int wgs = 0;
             for (int j = 0; j < nj; ++j)
                 if (!done[j]) wgs += jpairs[j];
@@ -1069,7 +1073,9 @@ struct wqc_tn_state {
                 std::vector<unsigned long long> offv(nj);
                 HIPC(hipMemcpyAsync(offv.data(), d_off, nj * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
                 sync();
-                std::fprintf(stderr, "low sweep %d active %d:", sweep + 1, active);
+                std::fprintf(stderr, "low sweep %d active %d wgs %d%s %.3f ms:", sweep + 1, active, wgs,
+                             wide ? " wide" : "",
+                             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_sweep).count());
                 int shown = 0;
                 for (int j = 0; j < nj && shown < 4; ++j) {
                     if (done[j] || jobs[j].n < max_n) continue;
