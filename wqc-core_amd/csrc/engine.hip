@@ -296,6 +296,48 @@ struct DevBuf {
     }
 };
 
+// Bump allocator over device chunks of at least 256 MiB, freed together (the adjoint sweep's
+// per-segment record of a forward replay).
+struct TapeArena {
+    std::vector<std::unique_ptr<DevBuf>> chunks;
+    size_t used = 0;
+    double2* take(size_t elems) {
+        const size_t bytes = ((elems * sizeof(double2)) + 255) & ~(size_t)255;
+        if (chunks.empty() || used + bytes > chunks.back()->bytes) {
+            chunks.push_back(std::make_unique<DevBuf>());
+            chunks.back()->ensure(std::max(bytes, (size_t)256 << 20));
+            used = 0;
+        }
+        double2* p = reinterpret_cast<double2*>((char*)chunks.back()->p + used);
+        used += bytes;
+        return p;
+    }
+};
+
+// Forward replay recorded for the adjoint sweep, in execution order: every one-qubit update
+// (site, matrix, the gate it came from) and every two-site unit with its input sites, the
+// Jacobi's rotated rows X = J W (W = theta' or theta'^H, p x q, p <= q), the rotation J (p x p),
+// the row norms (Jacobi order) and the kept rows in order.
+struct TapeOne {
+    int site, tag;
+    double2 u[4];
+};
+struct TapeUnit {
+    int p = 0, dl = 0, bond = 0, dr = 0, pp = 0, qq = 0, c = 0, tag = -1;
+    bool tall = false, done = false;
+    double2 u4[16];
+    std::vector<int> krow;
+    std::vector<double> sig;
+    double2 *L = nullptr, *R = nullptr, *X = nullptr, *J = nullptr;
+};
+struct Tape {
+    std::vector<std::pair<int, int>> order;  // (0, index into ones) or (1, index into units)
+    std::vector<TapeOne> ones;
+    std::vector<TapeUnit> units;
+    TapeArena arena;
+    int tag = -1;  // the gate being replayed
+};
+
 struct OmegaEntry {
     double2* omega_cols = nullptr;  // cols x r row-major: the independent columns of Omega
     int* pos = nullptr;             // r positions
@@ -482,6 +524,7 @@ struct wqc_tn_state {
     std::vector<PendingOne> pend1;
     std::vector<Unit> pend2;
     std::vector<char> touched;
+    Tape* tape = nullptr;  // set while the adjoint sweep replays a segment
 
     ~wqc_tn_state() {
         (void)hipSetDevice(dev ? dev->ordinal : 0);
@@ -741,6 +784,15 @@ struct wqc_tn_state {
             for (int k = 0; k < 4; ++k) jobs[i].u[k] = ops[i].u[k];
             max_elems = std::max(max_elems, s.l * s.r);
         }
+        if (tape)
+            for (const auto& op : ops) {
+                TapeOne t;
+                t.site = op.site;
+                t.tag = tape->tag;
+                for (int k = 0; k < 4; ++k) t.u[k] = op.u[k];
+                tape->order.emplace_back(0, (int)tape->ones.size());
+                tape->ones.push_back(t);
+            }
         OneQubitJob* d = upload(jobs);
         {
             Timer tm(this, 0);
@@ -1246,6 +1298,7 @@ struct wqc_tn_state {
         int sweeps = 0, sweeps_lo = 0;
         double dw = 0.0;  // discarded weight sum_{k >= chi} s_k^2 / sum s_k^2 of the split
         double probe = 0.0;  // max_p |A x_p - U S V^H x_p| / (|A|_F |x_p|) of the kept factorization
+        int rec = -1;        // its entry in the replay tape
     };
 
     void run_units(std::vector<Unit>& units) {
@@ -1323,6 +1376,29 @@ struct wqc_tn_state {
         double* d_fro = PD(o_fro);
         double* d_part = PD(o_part);
 
+        if (tape) {
+            if (use_lq || precision != WQC_TN_PRECISION_FP64)
+                fail(WQC_TN_EXEC_FAILED, "the adjoint replay records plain FP64 splits only");
+            for (int i = 0; i < nu; ++i) {
+                JobInfo& j = J[i];
+                if (j.rsvd || j.st != this) fail(WQC_TN_EXEC_FAILED, "the adjoint replay records exact splits only");
+                TapeUnit t;
+                t.p = j.p;
+                t.dl = j.dl;
+                t.bond = j.bond;
+                t.dr = j.dr;
+                t.tag = tape->tag;
+                std::memcpy(t.u4, units[i].u4, sizeof(t.u4));
+                const size_t nl = (size_t)j.dl * 2 * j.bond, nr = (size_t)j.bond * 2 * j.dr;
+                t.L = tape->arena.take(nl);
+                t.R = tape->arena.take(nr);
+                HIPC(hipMemcpyAsync(t.L, sites[j.p].ptr, nl * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+                HIPC(hipMemcpyAsync(t.R, sites[j.p + 1].ptr, nr * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+                j.rec = (int)tape->units.size();
+                tape->order.emplace_back(1, j.rec);
+                tape->units.push_back(std::move(t));
+            }
+        }
         // 1. merge + U (exact jobs with rows > cols store theta'^H for the row Jacobi).
         {
             Timer tm(this, 1);
@@ -2101,6 +2177,25 @@ struct wqc_tn_state {
             mxl = std::max(mxl, (long)j.rows * (long)c);
             mxr = std::max(mxr, (long)c * j.cols);
         }
+        if (tape)
+            for (int k = 0; k < ne; ++k) {
+                JobInfo& j = J[ids[k]];
+                if (j.rec < 0) continue;
+                TapeUnit& t = tape->units[j.rec];
+                const int nn = jj[k].n, mm = jj[k].m;
+                t.pp = nn;
+                t.qq = mm;
+                t.c = (int)j.chi_kept;
+                t.tall = j.rows > j.cols;
+                t.krow = perms[k];
+                t.sig.assign(hread + b.hoff[k], hread + b.hoff[k] + nn);
+                t.X = tape->arena.take((size_t)nn * mm);
+                t.J = tape->arena.take((size_t)nn * nn);
+                HIPC(hipMemcpy2DAsync(t.X, (size_t)mm * sizeof(double2), jj[k].X, (size_t)jj[k].ldx * sizeof(double2),
+                                      (size_t)mm * sizeof(double2), nn, hipMemcpyDeviceToDevice, stream));
+                HIPC(hipMemcpyAsync(t.J, jj[k].J, (size_t)nn * nn * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+                t.done = true;
+            }
         set_perms(fj, perms);
         {
             Timer tm(this, 4);
@@ -2977,6 +3072,64 @@ struct wqc_tn_state {
         return out;
     }
 
+    // Site cotangents of Re <bra|ket> (this state the ket): cot[q] = scale * L_q^H (Bra_q Rb_{q+1}),
+    // so that d Re <bra|ket> = Re sum conj(cot[q] / scale) . dKet_q over every site, with the
+    // two-state left environments L_q = sum_s Bra_s^H L_{q-1} Ket_s and the conjugated right ones
+    // Rb_q of overlap_ops. cot[q] is sized to the ket's site q.
+    void site_cotangents(const wqc_tn_state& bra, std::vector<std::unique_ptr<DevBuf>>& cot, double scale) {
+        flush();
+        sync();
+        const wqc_tn_state& ket = *this;
+        size_t env = 0, maxbk = 1;
+        std::vector<size_t> off(n + 1);
+        auto lb = [&](int q) { return q < n ? bra.sites[q].l : 1; };
+        auto lk = [&](int q) { return q < n ? ket.sites[q].l : 1; };
+        for (int q = 0; q <= n; ++q) {
+            off[q] = env;
+            env += (size_t)lb(q) * lk(q);
+            maxbk = std::max(maxbk, (size_t)std::max(lb(q), lk(q)));
+        }
+        const size_t fl = 2 * maxbk * maxbk;
+        ews.ensure((2 * env + 2 * fl + 64) * sizeof(double2));
+        note_peak();
+        double2* L = (double2*)ews.p;
+        double2* Rb = L + env;
+        double2* FL = Rb + env;
+        double2* HR = FL + fl;
+        const double2 one = make_double2(1, 0), zero = make_double2(0, 0);
+        HIPC(hipMemcpyAsync(L + off[0], &kOne, sizeof(double2), hipMemcpyHostToDevice, stream));
+        HIPC(hipMemcpyAsync(Rb + off[n], &kOne, sizeof(double2), hipMemcpyHostToDevice, stream));
+        for (int i = 0; i < n; ++i) {
+            tables_reset();
+            const int ql = i, qr = n - 1 - i;
+            const Site &bl = bra.sites[ql], &kl = ket.sites[ql], &br = bra.sites[qr], &kr = ket.sites[qr];
+            std::vector<GemmJob> a{
+                GemmJob{L + off[ql], kl.ptr, FL, bl.l, 2 * kl.r, kl.l, kl.l, 2 * kl.r, 2 * kl.r, 0, 0, one, zero},
+                GemmJob{br.ptr, Rb + off[qr + 1], HR, 2 * br.l, kr.r, br.r, br.r, kr.r, kr.r, 0, 0, one, zero}};
+            std::vector<GemmJob> b{
+                GemmJob{bl.ptr, FL, L + off[ql + 1], bl.r, kl.r, 2 * bl.l, bl.r, kl.r, kl.r, 1, 0, one, zero},
+                GemmJob{HR, kr.ptr, Rb + off[qr], br.l, kr.l, 2 * kr.r, 2 * kr.r, 2 * kr.r, kr.l, 0, 1, one, zero}};
+            wqc::launch_gemm(upload(a), 2, std::max(bl.l, 2 * br.l), std::max(2 * kl.r, kr.r), stream);
+            wqc::launch_gemm(upload(b), 2, std::max(bl.r, br.l), std::max(kl.r, kr.l), stream);
+            check_launch();
+            sync();
+        }
+        cot.resize(n);
+        for (int q = 0; q < n; ++q) {
+            tables_reset();
+            const Site &bs = bra.sites[q], &ks = ket.sites[q];
+            if (!cot[q]) cot[q] = std::make_unique<DevBuf>();
+            cot[q]->ensure((size_t)ks.l * 2 * ks.r * sizeof(double2));
+            double2* c = (double2*)cot[q]->p;
+            // G = Bra_q Rb_{q+1}: (bra.l, 2, ket.r); cot = scale L_q^H G: (ket.l, 2, ket.r).
+            std::vector<GemmJob> g1{gj(bs.ptr, bs.r, 0, Rb + off[q + 1], ks.r, 0, FL, ks.r, 2 * bs.l, ks.r, bs.r, 1.0, 0.0)};
+            std::vector<GemmJob> g2{gj(L + off[q], ks.l, 1, FL, 2 * ks.r, 0, c, 2 * ks.r, ks.l, 2 * ks.r, bs.l, scale, 0.0)};
+            gemm(g1, 5);
+            gemm(g2, 5);
+            sync();
+        }
+    }
+
     // <psi|P_k|psi> of Pauli strings by transfer matrices. The reference evaluates the same
     // sum over the dense statevector, sum_i conj(psi_P(i)) phase_i psi_i
     // (src/expectation.rs:140-167, n <= 20 through contract_to_statevector); here any n.
@@ -3764,6 +3917,7 @@ int wqc_tn_backward(wqc_tn_device* dev, size_t qubit_count, size_t max_bond_dim,
         if (max_bond_dim == 0) fail(WQC_TN_EXEC_FAILED, "MPS max bond dimension must be > 0");
         if (!n_terms) fail(WQC_TN_INVALID_ARG, "observable must have at least one term");
         if (!valid_precision(precision)) fail(WQC_TN_INVALID_ARG, "unknown precision mode");
+        (void)adjoint_bond;  // no adjoint state is carried: the sweep is over site cotangents
         const int n = (int)qubit_count;
         for (size_t j = 0; j < n_terms * qubit_count; ++j) {
             const char ch = labels[j];
@@ -3785,27 +3939,35 @@ int wqc_tn_backward(wqc_tn_device* dev, size_t qubit_count, size_t max_bond_dim,
             }
         }
         const size_t R = rb.size();
-        auto block_end = [&](size_t r) { return r + 1 < R ? rb[r + 1] : count; };
         const size_t k = checkpoint_every ? checkpoint_every : 1;
+        // Segment s: runs [s k, (s + 1) k) and their blocks; its checkpoint is the state at its start.
+        const size_t nseg = (R + k - 1) / k;
+        auto seg_begin = [&](size_t sg) { return rb[sg * k]; };
+        auto seg_end = [&](size_t sg) { return (sg + 1) * k < R ? rb[(sg + 1) * k] : count; };
+        // The differentiated forward: plain FP64 exact splits, one gate per flush, so that a
+        // segment's replay repeats it bit for bit and its tape is in execution order.
         auto phi = std::make_unique<wqc_tn_state>();
-        phi->init(dev, qubit_count, max_bond_dim, precision);
-        auto apply = [&](wqc_tn_state& st, size_t b, size_t e) {
-            for (size_t i = b; i < e; ++i)
-                if (!is_noop(gates[i].kind)) st.queue_gate(gates[i]);
-            st.flush();
+        phi->init(dev, qubit_count, max_bond_dim, WQC_TN_PRECISION_FP64);
+        phi->exact_only = true;
+        phi->use_lq = false;
+        auto apply = [&](size_t b, size_t e) {
+            for (size_t i = b; i < e; ++i) {
+                if (is_noop(gates[i].kind)) continue;
+                if (phi->tape) phi->tape->tag = (int)i;
+                phi->queue_gate(gates[i]);
+                phi->flush();
+            }
         };
-        std::vector<wqc_tn_state::Snapshot> ck(R);
+        std::vector<wqc_tn_state::Snapshot> ck(nseg);
         struct Guard {
             std::vector<wqc_tn_state::Snapshot>& v;
             ~Guard() {
                 for (auto& s : v) s.release();
             }
         } guard{ck};
-        // Forward with checkpoints at the end of every k-th run.
-        for (size_t r = 0; r < R; ++r) {
-            apply(*phi, rb[r], re[r]);
-            if (r % k == 0) phi->snapshot(ck[r]);
-            apply(*phi, re[r], block_end(r));
+        for (size_t sg = 0; sg < nseg; ++sg) {
+            phi->snapshot(ck[sg]);
+            apply(seg_begin(sg), seg_end(sg));
         }
         // Value: sum_j c_j <psi|P_j|psi>.
         {
@@ -3819,80 +3981,166 @@ int wqc_tn_backward(wqc_tn_device* dev, size_t qubit_count, size_t max_bond_dim,
             value[0] = a;
             value[1] = b;
         }
-        // lambda = H psi through the MPO, then carried back through the circuit with exact splits.
-        std::vector<std::vector<double2>> W;
-        std::vector<int> D;
-        pauli_sum_mpo(n, labels, coeffs, n_terms, W, D);
-        size_t dmax = 1;
-        for (int d : D) dmax = std::max(dmax, (size_t)d);
-        const size_t lam_bond = adjoint_bond ? adjoint_bond : 4 * max_bond_dim;
-        auto lam = std::make_unique<wqc_tn_state>();
-        lam->init(dev, qubit_count, lam_bond, WQC_TN_PRECISION_FP64);
-        lam->exact_only = true;
-        lam->set_mpo_product(*phi, W, D);
+        // Cotangents of the final sites, 2 d Re <lambda|psi> / d site with lambda = H psi (the
+        // Pauli sum as an MPO times psi).
+        std::vector<std::unique_ptr<DevBuf>> cot;
+        {
+            std::vector<std::vector<double2>> W;
+            std::vector<int> D;
+            pauli_sum_mpo(n, labels, coeffs, n_terms, W, D);
+            size_t dmax = 1;
+            for (int d : D) dmax = std::max(dmax, (size_t)d);
+            auto lam = std::make_unique<wqc_tn_state>();
+            lam->init(dev, qubit_count, dmax * max_bond_dim, WQC_TN_PRECISION_FP64);
+            lam->set_mpo_product(*phi, W, D);
+            phi->site_cotangents(*lam, cot, 2.0);
+        }
         for (size_t i = 0; i < count; ++i) grad[i] = 0.0;
-        for (size_t rr = R; rr-- > 0;) {
-            // lambda at the end of run rr: undo block rr (CNOT, CZ and CCNOT are self-inverse).
-            for (size_t i = block_end(rr); i-- > re[rr];)
-                if (!is_noop(gates[i].kind)) lam->queue_gate(gates[i]);
-            lam->flush();
-            // phi at the end of run rr: replay from its checkpoint.
-            const size_t c = (rr / k) * k;
-            phi->restore(ck[c]);
-            for (size_t x = c; x < rr; ++x) {
-                apply(*phi, re[x], block_end(x));
-                apply(*phi, rb[x + 1], re[x + 1]);
-            }
-            // dE/dtheta = Im <lambda| W P W^H |phi> for the run's rotations, W = the later gates
-            // on the same qubit within the run.
-            std::vector<std::pair<int, std::array<double2, 4>>> ops;
-            std::vector<size_t> which;
-            for (size_t i = rb[rr]; i < re[rr]; ++i) {
-                const wqc_tn_gate& g = gates[i];
-                if (!is_rotation(g.kind)) continue;
-                double2 P[4];
-                for (int u = 0; u < 4; ++u) P[u] = make_double2(0, 0);
-                if (g.kind == WQC_TN_GATE_RX) P[1] = P[2] = make_double2(1, 0);
-                if (g.kind == WQC_TN_GATE_RY) {
-                    P[1] = make_double2(0, -1);
-                    P[2] = make_double2(0, 1);
+        // Reverse mode over the site tensors, segment by segment from the last: each segment is
+        // replayed from its checkpoint with the tape on, then walked back. A one-qubit update U
+        // on site q passes the cotangent back as U^H cot_q and undoes itself on the site; a
+        // rotation exp(-i t P / 2) adds dE/dt = Im sum_{s,t} P[s,t] M[s,t] / 2 with
+        // M[s,t] = sum conj(cot_q[a,s,b]) site_q[a,t,b]. A unit passes (cot_p, cot_{p+1}) back
+        // through its truncated split (the adjoint of W = U S V^H with the factors U_k sqrt(S),
+        // sqrt(S) V_k^H, including the terms between kept and discarded singular vectors and
+        // the projections onto the null spaces), its gate and its merge, and puts its input
+        // sites back.
+        DevBuf bws;    // one unit's adjoint workspace
+        DevBuf corr;   // the 2x2 correlations of a segment's rotations
+        auto cot_ptr = [&](int q) { return (double2*)cot[q]->p; };
+        for (size_t sgi = nseg; sgi-- > 0;) {
+            phi->restore(ck[sgi]);
+            Tape tape;
+            phi->tape = &tape;
+            apply(seg_begin(sgi), seg_end(sgi));
+            phi->tape = nullptr;
+            phi->sync();
+            size_t nrot = 0;
+            for (const auto& t : tape.ones)
+                if (t.tag >= 0 && is_rotation(gates[t.tag].kind)) ++nrot;
+            corr.ensure(std::max<size_t>(nrot, 1) * 4 * sizeof(double2));
+            std::vector<int> rot_tag;
+            for (size_t e = tape.order.size(); e-- > 0;) {
+                phi->sync();
+                phi->tables_reset();
+                const auto ent = tape.order[e];
+                if (ent.first == 0) {
+                    const TapeOne& t = tape.ones[ent.second];
+                    Site& st = phi->sites[t.site];
+                    if (t.tag >= 0 && is_rotation(gates[t.tag].kind)) {
+                        std::vector<wqc::SiteCorrJob> cj{wqc::SiteCorrJob{
+                            cot_ptr(t.site), st.ptr, (double2*)corr.p + 4 * rot_tag.size(), st.l, st.r}};
+                        wqc::launch_site_corr(phi->upload(cj), 1, phi->stream);
+                        phi->check_launch();
+                        rot_tag.push_back(t.tag);
+                    }
+                    double2 ua[4];
+                    mat2_adj(t.u, ua);
+                    std::vector<OneQubitJob> oj(2);
+                    oj[0].site = cot_ptr(t.site);
+                    oj[1].site = st.ptr;
+                    for (auto& j : oj) {
+                        j.l = st.l;
+                        j.r = st.r;
+                        for (int x = 0; x < 4; ++x) j.u[x] = ua[x];
+                    }
+                    wqc::launch_one_qubit(phi->upload(oj), 2, st.l * st.r, phi->stream);
+                    phi->check_launch();
+                    continue;
                 }
-                if (g.kind == WQC_TN_GATE_RZ) {
-                    P[0] = make_double2(1, 0);
-                    P[3] = make_double2(-1, 0);
+                const TapeUnit& t = tape.units[ent.second];
+                if (!t.done) fail(WQC_TN_EXEC_FAILED, "adjoint replay: a unit's split was not recorded");
+                const int pp = t.pp, qq = t.qq, c = t.c, dl = t.dl, dr = t.dr, bond = t.bond;
+                const size_t nJu = (size_t)pp * c, nC = (size_t)pp * pp, nY = (size_t)pp * qq;
+                const size_t nL = (size_t)2 * dl * bond, nR = (size_t)bond * 2 * dr;
+                bws.ensure((3 * nJu + nC + 3 * nY + nL + nR + 64) * sizeof(double2));
+                double2* Ju = (double2*)bws.p;
+                double2* XV = Ju + nJu;
+                double2* Ec = XV + nJu;
+                double2* Cm = Ec + nJu;
+                double2* Y = Cm + nC;
+                double2* MW = Y + nY;
+                double2* TH = MW + nY;
+                double2* Lb = TH + nY;
+                double2* Rbar = Lb + nL;
+                double2* cP = cot_ptr(t.p);      // (dl, 2, c)
+                double2* cQ = cot_ptr(t.p + 1);  // (c, 2, dr)
+                using WS = wqc_tn_state;
+                // JU = J Abar_W, XV = X Bbar_W^H; for a tall theta' (W = theta'^H) the factors
+                // swap roles: Abar_W = cot_{p+1}^H, Bbar_W = cot_p^H.
+                std::vector<GemmJob> g1;
+                if (!t.tall) {
+                    g1.push_back(WS::gj(t.J, pp, 0, cP, c, 0, Ju, c, pp, c, pp, 1.0, 0.0));
+                    g1.push_back(WS::gj(t.X, qq, 0, cQ, qq, 1, XV, c, pp, c, qq, 1.0, 0.0));
+                } else {
+                    g1.push_back(WS::gj(t.J, pp, 0, cQ, pp, 1, Ju, c, pp, c, pp, 1.0, 0.0));
+                    g1.push_back(WS::gj(t.X, qq, 0, cP, c, 0, XV, c, pp, c, qq, 1.0, 0.0));
                 }
-                double2 Wm[4] = {make_double2(1, 0), make_double2(0, 0), make_double2(0, 0), make_double2(1, 0)};
-                for (size_t j = i + 1; j < re[rr]; ++j) {
-                    if (is_noop(gates[j].kind) || gates[j].q0 != g.q0) continue;
-                    Mat2 u;
-                    unary_matrix(gates[j].kind, gates[j].angle, u);
-                    double2 t[4];
-                    mat2_mul(u, Wm, t);
-                    for (int x = 0; x < 4; ++x) Wm[x] = t[x];
+                phi->gemm(g1, 5);
+                std::vector<int> inv(pp, -1);
+                for (int i = 0; i < c; ++i) inv[t.krow[i]] = i;
+                wqc::SvdAdjJob aj;
+                aj.JU = Ju;
+                aj.XV = XV;
+                aj.sig = phi->upload(t.sig);
+                aj.inv = phi->upload(inv);
+                aj.C = Cm;
+                aj.E = Ec;
+                aj.p = pp;
+                aj.c = c;
+                aj.floor2 = NULL_ROWS_SQ / pp;
+                wqc::launch_svd_adj_coef(aj, phi->stream);
+                phi->check_launch();
+                // Wbar = J^H (C X + E Bbar_W).
+                std::vector<GemmJob> g2{WS::gj(Cm, pp, 0, t.X, qq, 0, Y, qq, pp, qq, pp, 1.0, 0.0)};
+                phi->gemm(g2, 5);
+                std::vector<GemmJob> g3{!t.tall ? WS::gj(Ec, c, 0, cQ, qq, 0, Y, qq, pp, qq, c, 1.0, 1.0)
+                                                : WS::gj(Ec, c, 0, cP, c, 1, Y, qq, pp, qq, c, 1.0, 1.0)};
+                phi->gemm(g3, 5);
+                std::vector<GemmJob> g4{WS::gj(t.J, pp, 1, Y, qq, 0, MW, qq, pp, qq, pp, 1.0, 0.0)};
+                phi->gemm(g4, 5);
+                // theta'bar (2dl x 2dr), then back through the gate and the merge theta = L R.
+                double2* thb = MW;
+                if (t.tall) {
+                    wqc::launch_conj_transpose(MW, TH, pp, qq, phi->stream);
+                    thb = TH;
                 }
-                double2 Wa[4], t[4], O[4];
-                mat2_adj(Wm, Wa);
-                mat2_mul(Wm, P, t);
-                mat2_mul(t, Wa, O);
-                std::array<double2, 4> o;
-                for (int x = 0; x < 4; ++x) o[x] = O[x];
-                ops.emplace_back(g.q0, o);
-                which.push_back(i);
+                wqc::launch_apply4_adj(thb, dl, dr, t.u4, phi->stream);
+                phi->check_launch();
+                std::vector<GemmJob> g5{WS::gj(thb, 2 * dr, 0, t.R, 2 * dr, 1, Lb, bond, 2 * dl, bond, 2 * dr, 1.0, 0.0),
+                                        WS::gj(t.L, bond, 1, thb, 2 * dr, 0, Rbar, 2 * dr, bond, 2 * dr, 2 * dl, 1.0, 0.0)};
+                phi->gemm(g5, 5);
+                // Growing a cotangent or site buffer frees the old one: nothing queued may still
+                // read it.
+                phi->sync();
+                cot[t.p]->ensure(nL * sizeof(double2));
+                cot[t.p + 1]->ensure(nR * sizeof(double2));
+                HIPC(hipMemcpyAsync(cot_ptr(t.p), Lb, nL * sizeof(double2), hipMemcpyDeviceToDevice, phi->stream));
+                HIPC(hipMemcpyAsync(cot_ptr(t.p + 1), Rbar, nR * sizeof(double2), hipMemcpyDeviceToDevice, phi->stream));
+                phi->ensure_site_cap(t.p, nL);
+                phi->ensure_site_cap(t.p + 1, nR);
+                HIPC(hipMemcpyAsync(phi->sites[t.p].ptr, t.L, nL * sizeof(double2), hipMemcpyDeviceToDevice, phi->stream));
+                HIPC(hipMemcpyAsync(phi->sites[t.p + 1].ptr, t.R, nR * sizeof(double2), hipMemcpyDeviceToDevice,
+                                    phi->stream));
+                phi->sites[t.p].l = dl;
+                phi->sites[t.p].r = bond;
+                phi->sites[t.p + 1].l = bond;
+                phi->sites[t.p + 1].r = dr;
             }
-            if (!ops.empty()) {
-                auto vals = phi->overlap_ops(*lam, ops);
-                for (size_t x = 0; x < which.size(); ++x) grad[which[x]] = vals[x].y;
+            phi->sync();
+            if (!rot_tag.empty()) {
+                std::vector<double2> m(4 * rot_tag.size());
+                HIPC(hipMemcpy(m.data(), corr.p, m.size() * sizeof(double2), hipMemcpyDeviceToHost));
+                for (size_t x = 0; x < rot_tag.size(); ++x) {
+                    const int kind = gates[rot_tag[x]].kind;
+                    const double2* M = &m[4 * x];
+                    double v = 0.0;  // Im sum P[s,t] M[s,t]
+                    if (kind == WQC_TN_GATE_RX) v = M[1].y + M[2].y;
+                    if (kind == WQC_TN_GATE_RY) v = -M[1].x + M[2].x;  // P = [[0, -i], [i, 0]]
+                    if (kind == WQC_TN_GATE_RZ) v = M[0].y - M[3].y;
+                    grad[rot_tag[x]] += 0.5 * v;
+                }
             }
-            // lambda at the start of run rr: undo its one-qubit gates.
-            for (size_t i = re[rr]; i-- > rb[rr];) {
-                const wqc_tn_gate& g = gates[i];
-                if (is_noop(g.kind)) continue;
-                Mat2 u, ua;
-                unary_matrix(g.kind, g.angle, u);
-                mat2_adj(u, ua);
-                lam->queue_one(g.q0, ua);
-            }
-            lam->flush();
         }
     });
 }

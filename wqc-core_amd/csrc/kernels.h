@@ -251,6 +251,39 @@ struct OpSiteJob {
 };
 void launch_op_site(const OpSiteJob* jobs, int njobs, long max_half, hipStream_t s);
 
+// Adjoint sweep. out[2s + t] = sum_{a,b} conj(C[a,s,b]) S[a,t,b] for a site cotangent C and
+// a site S, both (l, 2, r).
+struct SiteCorrJob {
+    const double2* C;
+    const double2* S;
+    double2* out;
+    int l, r;
+};
+void launch_site_corr(const SiteCorrJob* jobs, int njobs, hipStream_t s);
+// The 4x4 gate of a unit as a kernel argument.
+struct U4Arg {
+    double2 m[16];
+};
+// T (2dl x 2dr, row-major (a, s, t, b)) <- the cotangent through U4: sum over (s', t') of
+// conj(U4[(s', t')][(s, t)]) T[a, s', t', b], in place.
+void launch_apply4_adj(double2* T, int dl, int dr, const double2 u4[16], hipStream_t s);
+// Coefficients of the adjoint of one truncated split W = U S V^H (W p x q, p <= q; the Jacobi's
+// rotated rows X = J W have norms sig, U = J^H; rows krow[0..c) kept, inv[row] = kept index or
+// -1). JU = J Abar and XV = X Bbar^H (p x c, unscaled) from the cotangents of the factors
+// A = U_k sqrt(S), B = sqrt(S) V_k^H; out: C (p x p) and E (p x c) with
+// Wbar = J^H (C X + E Bbar). Rows with sig^2 <= floor2 count as null.
+struct SvdAdjJob {
+    const double2* JU;
+    const double2* XV;
+    const double* sig;
+    const int* inv;
+    double2* C;
+    double2* E;
+    int p, c;
+    double floor2;
+};
+void launch_svd_adj_coef(const SvdAdjJob& job, hipStream_t s);
+
 // One site of the MPS sampler for `shots` shots: W (shots x 2r) holds v A_s for s = 0, 1,
 // Z (two blocks of shots x r) holds R (v A_s)^T; picks s with the uniform u[k * ustride],
 // sets bit `bit` of bits[k], writes v' = (v A_s) / sqrt(p_s) into vnext (shots x r).

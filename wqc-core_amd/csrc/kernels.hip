@@ -2049,6 +2049,132 @@ __global__ __launch_bounds__(256) void k_op_site(const OpSiteJob* __restrict__ j
     }
 }
 
+// ------------------------------------------------------------------ adjoint sweep
+// 2x2 correlation of a site cotangent with a site: M[s][t] = sum_{a,b} conj(C[a,s,b]) S[a,t,b],
+// one workgroup per job, per-thread partial sums over a strided share of (a, b), then a fixed
+// tree over the threads.
+__global__ __launch_bounds__(256) void k_site_corr(const SiteCorrJob* __restrict__ jobs) {
+    const SiteCorrJob jb = jobs[blockIdx.x];
+    const int tid = threadIdx.x;
+    const long total = (long)jb.l * jb.r;
+    double2 m00 = cz(), m01 = cz(), m10 = cz(), m11 = cz();
+    for (long e = tid; e < total; e += 256) {
+        const long a = e / jb.r, b = e - a * jb.r;
+        const long o0 = (a * 2) * jb.r + b, o1 = o0 + jb.r;
+        const double2 c0 = jb.C[o0], c1 = jb.C[o1], s0 = jb.S[o0], s1 = jb.S[o1];
+        ccmac(m00, c0, s0);
+        ccmac(m01, c0, s1);
+        ccmac(m10, c1, s0);
+        ccmac(m11, c1, s1);
+    }
+    __shared__ double2 red[4][256];
+    red[0][tid] = m00;
+    red[1][tid] = m01;
+    red[2][tid] = m10;
+    red[3][tid] = m11;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) red[k][tid] = cadd(red[k][tid], red[k][tid + w]);
+        __syncthreads();
+    }
+    if (tid < 4) jb.out[tid] = red[tid][0];
+}
+
+// Theta' -> theta cotangent through the two-qubit gate, in place: T[a,s,t,b] <-
+// sum_{s',t'} conj(U4[(s',t')][(s,t)]) T[a,s',t',b] on a (2dl x 2dr) row-major matrix.
+__global__ __launch_bounds__(256) void k_apply4_adj(double2* __restrict__ T, int dl, int dr, U4Arg u) {
+    const long total = (long)dl * dr;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+        const long a = e / dr, b = e - a * dr;
+        const long w = 2L * dr;
+        double2 v[4];
+#pragma unroll
+        for (int st = 0; st < 4; ++st) v[st] = T[(a * 2 + (st >> 1)) * w + (st & 1) * dr + b];
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+            double2 acc = cz();
+#pragma unroll
+            for (int sp = 0; sp < 4; ++sp) ccmac(acc, u.m[sp * 4 + st], v[sp]);
+            T[(a * 2 + (st >> 1)) * w + (st & 1) * dr + b] = acc;
+        }
+    }
+}
+
+// Coefficients of the truncated-split adjoint (see SvdAdjJob). Entry (x, y) of C:
+//   x == y, x kept as i:  P/s - conj(XV[x,i]) / s^3,  P = sbar + (i/2)(Im JU[x,i] - Im XV[x,i]/s)/s,
+//                         sbar = Re(JU[x,i] + XV[x,i]/s) / (2s),  s = sigma_x;
+//   x != y, both kept:    conj(al) / sigma_y - conj(XV[y,ix]) / (sigma_x sigma_y^2) with
+//                         al = (a - b) / (2 (sigma_y + sigma_x)) + (a + b) / (2 (sigma_y - sigma_x)),
+//                         a = conj(JU[x,iy]) - JU[y,ix], b = conj(JV[x,iy]) - JV[y,ix],
+//                         JV[j,i] = XV[j,i] / sigma_j (the second term at its limit for equal values);
+//   otherwise: [y kept as i]  (JU[x,i] s + XV[x,i]) / ((s^2 - sigma_x^2) s),  s = sigma_y
+//         + [x kept as i]  conj(JU[y,i]) / (s^2 - sigma_y^2)
+//                          + [sigma_y^2 > floor2] conj(XV[y,i]) / (s (s^2 - sigma_y^2)),  s = sigma_x,
+// with JU[j,i] = JUraw[j,i] sqrt(s_i) and XV[j,i] = XVraw[j,i] sqrt(s_i) for kept row i; and
+// E[x,i] = 1/sqrt(sigma_x) where row x is kept as i.
+__global__ __launch_bounds__(256) void k_svd_adj_coef(SvdAdjJob jb) {
+    const int x = blockIdx.y * 16 + (threadIdx.x >> 4), y = blockIdx.x * 16 + (threadIdx.x & 15);
+    if (x < jb.p && y < jb.c) {
+        const int i = jb.inv[x];
+        jb.E[(long)x * jb.c + y] = make_double2(i == y ? 1.0 / sqrt(jb.sig[x]) : 0.0, 0.0);
+    }
+    if (x >= jb.p || y >= jb.p) return;
+    double2 val = cz();
+    if (x == y) {
+        const int i = jb.inv[x];
+        if (i >= 0) {
+            const double s = jb.sig[x], rs = sqrt(s);
+            const double2 ju = cscale(jb.JU[(long)x * jb.c + i], rs), xv = cscale(jb.XV[(long)x * jb.c + i], rs);
+            const double sbar = (ju.x + xv.x / s) / (2.0 * s);
+            const double pim = 0.5 * (ju.y - xv.y / s) / s;
+            const double s3 = s * s * s;
+            val = make_double2(sbar / s - xv.x / s3, pim / s + xv.y / s3);
+        }
+    } else if (jb.inv[x] >= 0 && jb.inv[y] >= 0) {
+        // Both rows kept: the pair's coefficient split into its 1/(s_y + s_x) part and its
+        // 1/(s_y - s_x) part, which for (near-)equal values is replaced by its limit (the
+        // cotangents of a rotation inside a degenerate pair cancel: a + b -> 0).
+        const int ix = jb.inv[x], iy = jb.inv[y];
+        const double sx = jb.sig[x], sy = jb.sig[y];
+        const double2 jux = cscale(jb.JU[(long)x * jb.c + iy], sqrt(sy)), juy = cscale(jb.JU[(long)y * jb.c + ix], sqrt(sx));
+        const double2 xvx = cscale(jb.XV[(long)x * jb.c + iy], sqrt(sy)), xvy = cscale(jb.XV[(long)y * jb.c + ix], sqrt(sx));
+        const double2 jvx = cscale(xvx, 1.0 / sx), jvy = cscale(xvy, 1.0 / sy);
+        const double2 a = csub(cconj(jux), juy), b = csub(cconj(jvx), jvy);
+        const double dm = sy - sx, dp = sy + sx;
+        double2 al = cscale(csub(a, b), 0.5 / dp);
+        if (fabs(dm) > 1e-8 * fmax(sx, sy))
+            al = cadd(al, cscale(cadd(a, b), 0.5 / dm));
+        else
+            al = cadd(al, cscale(cadd(cadd(cconj(jux), jvy), cadd(juy, cconj(jvx))), 0.5 / dp));
+        val = csub(cscale(cconj(al), 1.0 / sy), cscale(cconj(xvy), 1.0 / (sx * sy * sy)));
+    } else {
+        const int iy = jb.inv[y], ix = jb.inv[x];
+        if (iy >= 0) {
+            const double s = jb.sig[y], sx = jb.sig[x], rs = sqrt(s);
+            const double d = (s - sx) * (s + sx);
+            if (d != 0.0) {
+                const double2 ju = cscale(jb.JU[(long)x * jb.c + iy], rs), xv = cscale(jb.XV[(long)x * jb.c + iy], rs);
+                val = cadd(val, cscale(cadd(cscale(ju, s), xv), 1.0 / (d * s)));
+            }
+        }
+        if (ix >= 0) {
+            const double s = jb.sig[x], sy = jb.sig[y], rs = sqrt(s);
+            const double d = (s - sy) * (s + sy);
+            if (d != 0.0) {
+                const double2 ju = cscale(jb.JU[(long)y * jb.c + ix], rs);
+                val = cadd(val, cscale(cconj(ju), 1.0 / d));
+                if (sy * sy > jb.floor2) {
+                    const double2 xv = cscale(jb.XV[(long)y * jb.c + ix], rs);
+                    val = cadd(val, cscale(cconj(xv), 1.0 / (s * d)));
+                }
+            }
+        }
+    }
+    jb.C[(long)x * jb.p + y] = val;
+}
+
 // ------------------------------------------------------------------ MPS sampler step
 // One wave per shot: p_s = Re sum_b conj(W_s[b]) Z_s[b] = <v A_s| R |v A_s>, the unnormalized
 // probability of outcome s given the shot's earlier outcomes; s = 0 when u < p_0 / (p_0 + p_1).
@@ -2315,6 +2441,23 @@ void launch_statevector_step(const double2* st, const double2* site, double2* ne
     long total = (long)r * 2 * prefix;
     hipLaunchKernelGGL(k_statevector_step, dim3(blocks_for(total, 256, 8192)), dim3(256), 0, s,
                        st, site, next, l, r, prefix);
+}
+
+void launch_site_corr(const SiteCorrJob* jobs, int njobs, hipStream_t s) {
+    if (njobs <= 0) return;
+    hipLaunchKernelGGL(k_site_corr, dim3(njobs), dim3(256), 0, s, jobs);
+}
+
+void launch_apply4_adj(double2* T, int dl, int dr, const double2 u4[16], hipStream_t s) {
+    U4Arg u;
+    for (int k = 0; k < 16; ++k) u.m[k] = u4[k];
+    hipLaunchKernelGGL(k_apply4_adj, dim3(blocks_for((long)dl * dr, 256, 4096)), dim3(256), 0, s, T, dl, dr, u);
+}
+
+void launch_svd_adj_coef(const SvdAdjJob& job, hipStream_t s) {
+    const int cols = job.p > job.c ? job.p : job.c;
+    dim3 grid((cols + 15) / 16, (job.p + 15) / 16);
+    hipLaunchKernelGGL(k_svd_adj_coef, grid, dim3(256), 0, s, job);
 }
 
 }  // namespace wqc
