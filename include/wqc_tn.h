@@ -242,15 +242,18 @@ int wqc_tn_pauli_expectations(wqc_tn_state *st, const char *labels, size_t n_ter
                               double *out_imag);
 /* Adjoint gradient of E(theta) = sum_k coeffs[k] <psi|P_k|psi> (real coefficients, Pauli
  * labels as wqc_tn_pauli_expectations) with respect to every RX/RY/RZ angle of the circuit; no
- * counterpart in the reference (its expectation mode has no gradients). Forward pass from
- * |0...0> with the state saved at the end of every checkpoint_every-th run of one-qubit gates
- * (0: every run); lambda = H psi as an MPO-MPS product (bond <= adjoint_bond, 0: 4 *
- * max_bond_dim, exact splits only), carried back through the circuit; each run's forward state
- * is replayed from its checkpoint, and a rotation exp(-i theta P / 2) gets
- * dE/dtheta = Im <lambda| W P W^H |phi> (W: the later gates on its qubit in the run, phi and
- * lambda at the run's end). Truncations of the forward pass enter as fixed projections (no
- * derivative of the truncated SVD). grad[i] (count entries) is 0 for non-rotation gates;
- * value = {Re E, Im E}. */
+ * counterpart in the reference (its expectation mode has no gradients). The forward pass is the
+ * plain FP64 form with exact splits (precision is validated, the range finder is not used: its
+ * sketch would have to be differentiated too), truncating to max_bond_dim by the reference's
+ * rule, one gate at a time, with the state saved at the start of every segment of
+ * checkpoint_every runs of one-qubit gates and the gates after them (0: every run). The
+ * gradient is reverse mode over the site tensors of that forward pass, truncations included:
+ * the final site cotangents come from lambda = H psi (an MPO-MPS product), and each segment,
+ * replayed from its checkpoint, is walked back op by op (a one-qubit update passes U^H back; a
+ * unit passes the cotangents of its two sites back through the adjoint of its truncated split
+ * with the sqrt(S) gauge, its gate and its merge). grad[i] (count entries) is dE/dtheta_i, 0 for
+ * non-rotation gates; value = {Re E, Im E}. adjoint_bond is unused (no adjoint state is
+ * carried). */
 int wqc_tn_backward(wqc_tn_device *dev, size_t qubit_count, size_t max_bond_dim, int precision,
                     const wqc_tn_gate *gates, size_t count, const char *labels,
                     const double *coeffs, size_t n_terms, size_t checkpoint_every,

@@ -2984,98 +2984,11 @@ struct wqc_tn_state {
         wbuf.release();
     }
 
-    // ------------------------------------------------------------ <bra| O_q |ket>
-    // For single-site operators ops[k] = (site q, 2x2 O): <bra| O on qubit q |ket> with
-    // two-state left environments Lb_q = sum_s Bra_s^H Lb_{q-1} Ket_s and conjugated right ones;
-    // one transfer step through the operator site (bra transformed by O) and the closing sum.
-    std::vector<double2> overlap_ops(const wqc_tn_state& bra, const std::vector<std::pair<int, std::array<double2, 4>>>& ops) {
-        flush();
-        sync();
-        const wqc_tn_state& ket = *this;
-        size_t env = 0, maxbk = 1, maxs = 1;
-        std::vector<size_t> off(n + 1);
-        auto lb = [&](int q) { return q < n ? bra.sites[q].l : 1; };
-        auto lk = [&](int q) { return q < n ? ket.sites[q].l : 1; };
-        for (int q = 0; q <= n; ++q) {
-            off[q] = env;
-            env += (size_t)lb(q) * lk(q);
-            maxbk = std::max(maxbk, (size_t)std::max(lb(q), lk(q)));
-        }
-        for (int q = 0; q < n; ++q) maxs = std::max(maxs, (size_t)bra.sites[q].l * 2 * bra.sites[q].r);
-        const size_t fl = 2 * maxbk * maxbk;
-        const size_t nops = ops.size();
-        const size_t total = 2 * env + 2 * fl + nops * (maxs + 2 * fl) + nops + 64;
-        ews.ensure(total * sizeof(double2));
-        note_peak();
-        double2* L = (double2*)ews.p;
-        double2* Rb = L + env;
-        double2* FL = Rb + env;
-        double2* HR = FL + fl;
-        double2* opb = HR + fl;                       // transformed bra sites, one per op
-        double2* opf = opb + nops * maxs;             // F = L_q Ket_flat, one per op
-        double2* ope = opf + nops * fl;               // E' per op
-        double2* res = ope + nops * fl;
-        const double2 one = make_double2(1, 0), zero = make_double2(0, 0);
-        HIPC(hipMemcpyAsync(L + off[0], &kOne, sizeof(double2), hipMemcpyHostToDevice, stream));
-        HIPC(hipMemcpyAsync(Rb + off[n], &kOne, sizeof(double2), hipMemcpyHostToDevice, stream));
-        for (int i = 0; i < n; ++i) {
-            tables_reset();
-            const int ql = i, qr = n - 1 - i;
-            const Site &bl = bra.sites[ql], &kl = ket.sites[ql], &br = bra.sites[qr], &kr = ket.sites[qr];
-            std::vector<GemmJob> a{
-                GemmJob{L + off[ql], kl.ptr, FL, bl.l, 2 * kl.r, kl.l, kl.l, 2 * kl.r, 2 * kl.r, 0, 0, one, zero},
-                GemmJob{br.ptr, Rb + off[qr + 1], HR, 2 * br.l, kr.r, br.r, br.r, kr.r, kr.r, 0, 0, one, zero}};
-            std::vector<GemmJob> b{
-                GemmJob{bl.ptr, FL, L + off[ql + 1], bl.r, kl.r, 2 * bl.l, bl.r, kl.r, kl.r, 1, 0, one, zero},
-                GemmJob{HR, kr.ptr, Rb + off[qr], br.l, kr.l, 2 * kr.r, 2 * kr.r, 2 * kr.r, kr.l, 0, 1, one, zero}};
-            wqc::launch_gemm(upload(a), 2, std::max(bl.l, 2 * br.l), std::max(2 * kl.r, kr.r), stream);
-            wqc::launch_gemm(upload(b), 2, std::max(bl.r, br.l), std::max(kl.r, kr.l), stream);
-            check_launch();
-            sync();
-        }
-        std::vector<double2> out(nops, make_double2(0, 0));
-        if (!nops) return out;
-        tables_reset();
-        std::vector<wqc::OpSiteJob> oj;
-        std::vector<GemmJob> fa, fb;
-        std::vector<wqc::CloseJob> cj;
-        long max_half = 1;
-        int mam = 1, man = 1, mbm = 1, mbn = 1;
-        for (size_t k = 0; k < nops; ++k) {
-            const int q = ops[k].first;
-            const Site &bs = bra.sites[q], &ks = ket.sites[q];
-            wqc::OpSiteJob j;
-            j.A = bs.ptr;
-            j.B = opb + k * maxs;
-            j.l = bs.l;
-            j.r = bs.r;
-            for (int u = 0; u < 4; ++u) j.o[u] = ops[k].second[u];
-            oj.push_back(j);
-            max_half = std::max(max_half, (long)bs.l * bs.r);
-            double2* f = opf + k * fl;
-            double2* e = ope + k * fl;
-            fa.push_back(GemmJob{L + off[q], ks.ptr, f, bs.l, 2 * ks.r, ks.l, ks.l, 2 * ks.r, 2 * ks.r, 0, 0, one, zero});
-            fb.push_back(GemmJob{opb + k * maxs, f, e, bs.r, ks.r, 2 * bs.l, bs.r, ks.r, ks.r, 1, 0, one, zero});
-            cj.push_back(wqc::CloseJob{e, Rb + off[q + 1], res + k, (long)bs.r * ks.r});
-            mam = std::max(mam, bs.l);
-            man = std::max(man, 2 * ks.r);
-            mbm = std::max(mbm, bs.r);
-            mbn = std::max(mbn, ks.r);
-        }
-        wqc::launch_op_site(upload(oj), (int)nops, max_half, stream);
-        wqc::launch_gemm(upload(fa), (int)nops, mam, man, stream);
-        wqc::launch_gemm(upload(fb), (int)nops, mbm, mbn, stream);
-        wqc::launch_env_close(upload(cj), (int)nops, stream);
-        check_launch();
-        HIPC(hipMemcpyAsync(out.data(), res, nops * sizeof(double2), hipMemcpyDeviceToHost, stream));
-        sync();
-        return out;
-    }
-
     // Site cotangents of Re <bra|ket> (this state the ket): cot[q] = scale * L_q^H (Bra_q Rb_{q+1}),
     // so that d Re <bra|ket> = Re sum conj(cot[q] / scale) . dKet_q over every site, with the
-    // two-state left environments L_q = sum_s Bra_s^H L_{q-1} Ket_s and the conjugated right ones
-    // Rb_q of overlap_ops. cot[q] is sized to the ket's site q.
+    // two-state left environments L_{q+1} = sum_s Bra_s^H L_q Ket_s (L_0 = 1) and the conjugated
+    // right ones Rb_q = sum_s Bra_s Rb_{q+1} Ket_s^H (Rb_n = 1). cot[q] is sized to the ket's
+    // site q.
     void site_cotangents(const wqc_tn_state& bra, std::vector<std::unique_ptr<DevBuf>>& cot, double scale) {
         flush();
         sync();

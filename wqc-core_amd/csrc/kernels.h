@@ -242,14 +242,6 @@ void launch_overlap_site(const double2* E, const double2* K, const double2* B, d
 // the combined bond index is mps_index * wn + (w - w0). out is (l*wn) x 2 x (r*vn).
 void launch_mpo_site(const double2* A, const double2* W, double2* out, int l, int r, int Dl, int Dr,
                      int w0, int wn, int v0, int vn, hipStream_t s);
-// Bra transform for a single-site operator O (2x2 row-major): B_t = sum_s' conj(O[s', t]) A_s'.
-struct OpSiteJob {
-    const double2* A;
-    double2* B;
-    int l, r;
-    double2 o[4];
-};
-void launch_op_site(const OpSiteJob* jobs, int njobs, long max_half, hipStream_t s);
 
 // Adjoint sweep. out[2s + t] = sum_{a,b} conj(C[a,s,b]) S[a,t,b] for a site cotangent C and
 // a site S, both (l, 2, r).

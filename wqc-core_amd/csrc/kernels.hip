@@ -2031,24 +2031,6 @@ __global__ __launch_bounds__(256) void k_mpo_site(const double2* __restrict__ A,
     }
 }
 
-__global__ __launch_bounds__(256) void k_op_site(const OpSiteJob* __restrict__ jobs) {
-    const OpSiteJob jb = jobs[blockIdx.y];
-    const long half = (long)jb.l * jb.r;
-    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < half; e += (long)gridDim.x * 256) {
-        const long a = e / jb.r, b = e % jb.r;
-        const long i0 = a * 2 * jb.r + b, i1 = i0 + jb.r;
-        const double2 v0 = jb.A[i0], v1 = jb.A[i1];
-        double2 b0 = cz(), b1 = cz();
-        // B_0 = conj(O00) A_0 + conj(O10) A_1, B_1 = conj(O01) A_0 + conj(O11) A_1
-        ccmac(b0, jb.o[0], v0);
-        ccmac(b0, jb.o[2], v1);
-        ccmac(b1, jb.o[1], v0);
-        ccmac(b1, jb.o[3], v1);
-        jb.B[i0] = b0;
-        jb.B[i1] = b1;
-    }
-}
-
 // ------------------------------------------------------------------ adjoint sweep
 // 2x2 correlation of a site cotangent with a site: M[s][t] = sum_{a,b} conj(C[a,s,b]) S[a,t,b],
 // one workgroup per job, per-thread partial sums over a strided share of (a, b), then a fixed
@@ -2332,10 +2314,6 @@ void launch_mpo_site(const double2* A, const double2* W, double2* out, int l, in
     const long total = (long)l * wn * 2 * r * vn;
     hipLaunchKernelGGL(k_mpo_site, dim3(blocks_for(total, 256, 4096)), dim3(256), 0, s, A, W, out, l, r, Dl,
                        Dr, w0, wn, v0, vn);
-}
-
-void launch_op_site(const OpSiteJob* jobs, int njobs, long max_half, hipStream_t s) {
-    hipLaunchKernelGGL(k_op_site, dim3(blocks_for(max_half, 256, 1024), njobs), dim3(256), 0, s, jobs);
 }
 
 void launch_sample_step(const double2* W, const double2* Z, int r, long shots, const double* u,

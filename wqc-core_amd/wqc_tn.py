@@ -223,7 +223,8 @@ def contract_time_split(qubit_count, gates, split, max_bond_dim, precision: int 
 def backward(qubit_count, max_bond_dim, gates, terms, checkpoint_every=2, adjoint_bond=0,
              precision: int = PRECISION_FP64, device=None):
     """Adjoint gradient of sum_k c_k <psi|P_k|psi> over (label, real coeff) terms with respect
-    to every RX/RY/RZ angle; returns (value, grad) with grad[i] for gates[i]."""
+    to every RX/RY/RZ angle, truncated splits included (reverse mode over the site tensors of the
+    plain FP64 exact-split forward pass); returns (value, grad) with grad[i] for gates[i]."""
     L = load()
     dev = device or default_device()
     garr = gate_array(gates)
