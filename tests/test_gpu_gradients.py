@@ -142,12 +142,13 @@ def test_config5_untruncated_vs_parameter_shift(wt):
     say(f"config5 chi=256: worst |adjoint - shift| = {worst:.2e}")
 
 
-@pytest.mark.parametrize("n,chi,depth,seed", [(8, 4, 6, 1), (10, 6, 6, 2), (12, 5, 8, 3), (12, 8, 12, 3)])
-def test_truncating_circuits_vs_finite_differences(wt, n, chi, depth, seed):
+@pytest.mark.parametrize("n,chi,depth,seed,every", [(8, 4, 6, 1, 2), (10, 6, 6, 2, 1), (12, 5, 8, 3, 3),
+                                                    (12, 8, 12, 3, 0)])
+def test_truncating_circuits_vs_finite_differences(wt, n, chi, depth, seed, every):
     rng = C.SplitMix64(0x7A + seed)
     gates = random_rotation_circuit(rng, n, depth)
     terms = random_terms(rng, n, 6)
-    val, grad = wt.backward(n, chi, gates, terms, checkpoint_every=2)
+    val, grad = wt.backward(n, chi, gates, terms, checkpoint_every=every)
     st = wt.MpsState(n, chi)
     st.apply_circuit(gates)
     trunc = st.stats()["truncated_splits"]
