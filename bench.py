@@ -300,6 +300,12 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy 256 MiB device to device three times: the reference kernel whose "
                          "FETCH_SIZE / WRITE_SIZE readings scale the counter runs")
+    ap.add_argument("--slices-pass", action="store_true",
+                    help="run the 256-slice batch (widths 12-60, depths 8-32, chi 16-256) once as one "
+                         "batch and once more timed (for kernel traces), no JSON line")
+    ap.add_argument("--backward-pass", action="store_true",
+                    help="run the adjoint gradient of the 32-qubit, 8-layer ansatz (512 angles, "
+                         "chi 64, checkpoints every 2 runs) once and once more timed, no JSON line")
     args = ap.parse_args()
     if args.copy_probe:
         # Three device-to-device copies of 256 MiB through the HIP runtime (its copy kernel).
@@ -326,6 +332,23 @@ def main():
     import wqc_tn as wt
 
     wt.load()
+    if args.slices_pass:
+        slices = C.config4()
+        for label in ("first", "timed"):
+            t0 = time.perf_counter()
+            amps, codes = wt.run_slices(slices, workers=0)
+            print(f"# slice batch ({label}): {len(slices)} slices in {time.perf_counter() - t0:.3f} s, "
+                  f"{sum(1 for c in codes if c == wt.OK)} OK", flush=True)
+        return
+    if args.backward_pass:
+        n5, chi5, gates5, _, tf = C.config5()
+        terms = [(lab, c) for c, lab in tf]
+        for label in ("first", "timed"):
+            t0 = time.perf_counter()
+            val, grad = wt.backward(n5, chi5, gates5, terms, checkpoint_every=2)
+            print(f"# backward ({label}): {len(grad)} gates in {time.perf_counter() - t0:.3f} s, "
+                  f"E = {val.real:.12f}", flush=True)
+        return
     n, chi, gates = C.config3(n=args.qubits, depth=args.depth)
     units = C.two_qubit_units(gates, n)
     print(f"# workload: {n}q depth {args.depth} chi {chi}: {units} two-qubit units", file=sys.stderr,

@@ -542,6 +542,7 @@ struct wqc_tn_state {
         gather.release();
         ov_buf.release();
         env_buf.release();
+        amp_buf.release();
         probe_buf.release();
         probe_p_buf.release();
         for (auto& sp : spans) {
@@ -560,7 +561,7 @@ struct wqc_tn_state {
     void note_peak() {
         uint64_t now = site_bytes + ws.bytes + ws32.bytes + ews.bytes + tab.bytes + splitk_buf.bytes + gather.bytes +
                        jstats_buf.bytes + trace_buf.bytes + ov_buf.bytes + env_buf.bytes + probe_buf.bytes +
-                       probe_p_buf.bytes;
+                       probe_p_buf.bytes + amp_buf.bytes;
         if (now > stats.peak_vram_bytes) stats.peak_vram_bytes = now;
     }
 
@@ -2632,18 +2633,62 @@ struct wqc_tn_state {
 
     // <0...0|psi> of another state's sites (a slice container of the batch), computed with this
     // state's stream and buffers.
-    std::pair<double, double> amplitude_of(wqc_tn_state& o) {
-        std::swap(sites, o.sites);
-        std::swap(n, o.n);
-        try {
-            auto a = amplitude_zero();
-            std::swap(sites, o.sites);
-            std::swap(n, o.n);
-            return a;
-        } catch (...) {
-            std::swap(sites, o.sites);
-            std::swap(n, o.n);
-            throw;
+    // <0...0|psi> of many states together: position q of every state's vector-times-site chain
+    // runs in one launch pair (each state's vector ping-pongs in its own two buffers), so a
+    // batch of slices costs one launch pair per site position instead of one per site.
+    DevBuf amp_buf;
+    void amplitudes_batch(const std::vector<wqc_tn_state*>& states, double* out) {
+        flush();
+        sync();
+        tables_reset();
+        const size_t ns = states.size();
+        size_t mx = 2;
+        int maxn = 0;
+        for (auto* s : states) {
+            if (!s) continue;
+            maxn = std::max(maxn, s->n);
+            for (const auto& x : s->sites) mx = std::max(mx, (size_t)std::max(x.l, x.r) + 2);
+        }
+        const size_t part = (mx / wqc::kVecMatChunk + 2) * mx;
+        const size_t per = 2 * mx + part;
+        amp_buf.ensure(std::max<size_t>(ns, 1) * per * sizeof(double2) + 4096);
+        note_peak();
+        double2* base = (double2*)amp_buf.p;
+        std::vector<double2> ones(ns, kOne);
+        HIPC(hipMemcpy2DAsync(base, per * sizeof(double2), ones.data(), sizeof(double2), sizeof(double2), ns,
+                              hipMemcpyHostToDevice, stream));
+        std::vector<int> vlen(ns, 1), cur(ns, 0);
+        Timer tm(this, 5);
+        for (int q = 0; q < maxn; ++q) {
+            std::vector<VecMatJob> jobs;
+            int mo = 1, mi = 1;
+            for (size_t i = 0; i < ns; ++i) {
+                if (!states[i] || q >= states[i]->n) continue;
+                const Site& x = states[i]->sites[q];
+                double2* b = base + i * per;
+                jobs.push_back(VecMatJob{x.ptr, b + cur[i] * mx, b + (1 - cur[i]) * mx, x.l, x.r, vlen[i], 0, 0, 0,
+                                         b + 2 * mx});
+                mo = std::max(mo, x.r);
+                mi = std::max(mi, std::min(x.l, vlen[i]));
+                vlen[i] = x.r;
+                cur[i] ^= 1;
+            }
+            wqc::launch_vecmat_batch(upload(jobs), (int)jobs.size(), mo, mi, stream);
+            check_launch();
+            klaunch[5]++;
+        }
+        // Every chain ends in a vector of length 1: element 0 of the buffer it was last written to.
+        std::vector<double2> h(2 * ns);
+        HIPC(hipMemcpy2DAsync(h.data(), 2 * sizeof(double2), base, per * sizeof(double2), sizeof(double2), ns,
+                              hipMemcpyDeviceToHost, stream));
+        HIPC(hipMemcpy2DAsync(h.data() + 1, 2 * sizeof(double2), base + mx, per * sizeof(double2), sizeof(double2),
+                              ns, hipMemcpyDeviceToHost, stream));
+        sync();
+        for (size_t i = 0; i < ns; ++i) {
+            if (!states[i]) continue;
+            const double2 v = h[2 * i + cur[i]];
+            out[2 * i] = v.x;
+            out[2 * i + 1] = v.y;
         }
     }
 
@@ -3645,12 +3690,9 @@ bool run_slices_batched(wqc_tn_device* dev, const wqc_tn_slice* slices, size_t n
                 k = e;
             }
         }
-        for (size_t i = 0; i < n_slices; ++i) {
-            if (!st[i]) continue;
-            auto a = ex->amplitude_of(*st[i]);
-            out_amps[2 * i] = a.first;
-            out_amps[2 * i + 1] = a.second;
-        }
+        std::vector<wqc_tn_state*> states(n_slices);
+        for (size_t i = 0; i < n_slices; ++i) states[i] = st[i].get();
+        ex->amplitudes_batch(states, out_amps);
     } catch (const Failure& f) {
         if (f.code == WQC_TN_CANCELLED) throw;
         return false;

@@ -211,6 +211,9 @@ void launch_lq_extract(const LqJob* jobs, int njobs, int max_n, hipStream_t s);
 void launch_lq_init_m(const LqJob* jobs, int njobs, long max_elems, hipStream_t s);
 constexpr int kVecMatChunk = 16;  // rows per workgroup of the left pass
 void launch_vecmat(const VecMatJob* job, int out_len, int in_len, int right, hipStream_t s);
+// Left passes of njobs independent vector-times-site products in one launch pair (max_out: the
+// largest site r, max_in: the largest min(l, vlen)); each job has its own part buffer.
+void launch_vecmat_batch(const VecMatJob* jobs, int njobs, int max_out, int max_in, hipStream_t s);
 void launch_fill_int(int* p, int value, int count, hipStream_t s);
 void launch_conj_transpose(const double2* src, double2* dst, int rows, int cols, hipStream_t s);
 void launch_add_identity(double2* C, int rows, int cols, int ldc, hipStream_t s);

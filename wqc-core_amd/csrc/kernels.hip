@@ -1854,15 +1854,18 @@ __global__ __launch_bounds__(1024) void k_householder(const HouseJob* __restrict
 }
 
 // ------------------------------------------------------------------ vector x site
-// Left pass: workgroup (x, y) sums rows y*kVecMatChunk.. of v[a] site[a, s, o] for 256
-// consecutive o (coalesced along the row); a second kernel adds the chunks in order.
+// Left pass: workgroup (x, y, z) sums rows y*kVecMatChunk.. of v[a] site[a, s, o] of job z for
+// 256 consecutive o (coalesced along the row); a second kernel adds the chunks in order. A
+// job's output length is its site's r and its chunk count ceil(min(l, vlen) / kVecMatChunk).
 // Right pass: one wave per output row o, coalesced along the row, wave reduction.
-__global__ __launch_bounds__(256) void k_vecmat_left(const VecMatJob* __restrict__ jobp, int out_len) {
-    const VecMatJob jb = *jobp;
+__global__ __launch_bounds__(256) void k_vecmat_left(const VecMatJob* __restrict__ jobs) {
+    const VecMatJob jb = jobs[blockIdx.z];
+    const int out_len = jb.r;
     const int o = blockIdx.x * 256 + threadIdx.x;
     if (o >= out_len) return;
     const int lim = jb.l < jb.vlen ? jb.l : jb.vlen;
     const int a0 = blockIdx.y * kVecMatChunk;
+    if (blockIdx.y > 0 && a0 >= lim) return;
     const int a1 = a0 + kVecMatChunk < lim ? a0 + kVecMatChunk : lim;
     const long rs = 2L * jb.r;
     double2 acc = cz();
@@ -1875,10 +1878,13 @@ __global__ __launch_bounds__(256) void k_vecmat_left(const VecMatJob* __restrict
     jb.part[(long)blockIdx.y * out_len + o] = acc;
 }
 
-__global__ __launch_bounds__(256) void k_vecmat_sum(const VecMatJob* __restrict__ jobp, int out_len, int nchunk) {
-    const VecMatJob jb = *jobp;
+__global__ __launch_bounds__(256) void k_vecmat_sum(const VecMatJob* __restrict__ jobs) {
+    const VecMatJob jb = jobs[blockIdx.z];
+    const int out_len = jb.r;
     const int o = blockIdx.x * 256 + threadIdx.x;
     if (o >= out_len) return;
+    const int lim = jb.l < jb.vlen ? jb.l : jb.vlen;
+    const int nchunk = (lim + kVecMatChunk - 1) / kVecMatChunk;
     double2 acc = cz();
     for (int c = 0; c < nchunk; ++c) acc = cadd(acc, jb.part[(long)c * out_len + o]);
     jb.vout[o] = acc;
@@ -2393,9 +2399,14 @@ void launch_vecmat(const VecMatJob* job, int out_len, int in_len, int right, hip
         hipLaunchKernelGGL(k_vecmat_right, dim3((out_len + 3) / 4), dim3(256), 0, s, job, out_len);
         return;
     }
-    const int nchunk = in_len > 0 ? (in_len + kVecMatChunk - 1) / kVecMatChunk : 1;
-    hipLaunchKernelGGL(k_vecmat_left, dim3((out_len + 255) / 256, nchunk), dim3(256), 0, s, job, out_len);
-    hipLaunchKernelGGL(k_vecmat_sum, dim3((out_len + 255) / 256), dim3(256), 0, s, job, out_len, nchunk);
+    launch_vecmat_batch(job, 1, out_len, in_len, s);
+}
+
+void launch_vecmat_batch(const VecMatJob* jobs, int njobs, int max_out, int max_in, hipStream_t s) {
+    if (njobs <= 0) return;
+    const int nchunk = max_in > 0 ? (max_in + kVecMatChunk - 1) / kVecMatChunk : 1;
+    hipLaunchKernelGGL(k_vecmat_left, dim3((max_out + 255) / 256, nchunk, njobs), dim3(256), 0, s, jobs);
+    hipLaunchKernelGGL(k_vecmat_sum, dim3((max_out + 255) / 256, 1, njobs), dim3(256), 0, s, jobs);
 }
 
 void launch_conj_transpose(const double2* src, double2* dst, int rows, int cols, hipStream_t s) {
