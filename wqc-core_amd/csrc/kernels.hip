@@ -670,7 +670,28 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round(const Ja
         // Every wave reads the flag before thread 0 clears it for the next sweep.
         const int again = s_any;
         __syncthreads();
-        if (!again) break;
+        if (!again || sweep + 1 >= inner_cap) break;
+        // The next sweep is a no-op unless some pair passes rot_params64's test on the current
+        // G in either orientation (an inactive rotation leaves G and T unchanged, so every
+        // later step of the sweep sees the same entries): skip it then, with the same result.
+        if (tid == 0) s_any = 0;
+        __syncthreads();
+        {
+            bool act = false;
+            for (int e = tid; e < JP * JP; e += NT) {
+                const int i = e / JP, j = e % JP;
+                if (i == j) continue;
+                const double2 c = Gc[i][j];
+                const double g2 = fma(c.x, c.x, c.y * c.y);
+                const double a = Gc[i][i].x, d = Gc[j][j].x;
+                if (g2 > 0.0 && fmin(a, d) > nu2 && g2 > tol * tol * (a * d)) act = true;
+            }
+            if (act) s_any = 1;
+        }
+        __syncthreads();
+        const int next = s_any;
+        __syncthreads();
+        if (!next) break;
         if (tid == 0) s_any = 0;
         __syncthreads();
     }
