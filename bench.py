@@ -302,7 +302,7 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE readings scale the counter runs")
     ap.add_argument("--slices-pass", action="store_true",
                     help="run the 256-slice batch (widths 12-60, depths 8-32, chi 16-256) once as one "
-                         "batch and once more timed (for kernel traces), no JSON line")
+                         "batch and once more timed (for kernel traces) in each form --form selects, no JSON line")
     ap.add_argument("--backward-pass", action="store_true",
                     help="run the adjoint gradient of the 32-qubit, 8-layer ansatz (512 angles, "
                          "chi 64, checkpoints every 2 runs) once and once more timed, no JSON line")
@@ -334,11 +334,16 @@ def main():
     wt.load()
     if args.slices_pass:
         slices = C.config4()
-        for label in ("first", "timed"):
-            t0 = time.perf_counter()
-            amps, codes = wt.run_slices(slices, workers=0)
-            print(f"# slice batch ({label}): {len(slices)} slices in {time.perf_counter() - t0:.3f} s, "
-                  f"{sum(1 for c in codes if c == wt.OK)} OK", flush=True)
+        forms = [("fp64", wt.PRECISION_FP64), ("mixed", wt.PRECISION_MIXED_FP32),
+                 ("mixed_bf16", wt.PRECISION_MIXED_BF16)]
+        for name, prec in forms:
+            if args.form not in ("all", name) and not (args.form == "both" and name != "mixed_bf16"):
+                continue
+            for label in ("first", "timed"):
+                t0 = time.perf_counter()
+                amps, codes = wt.run_slices(slices, precision=prec, workers=0)
+                print(f"# slice batch {name} ({label}): {len(slices)} slices in {time.perf_counter() - t0:.3f} s, "
+                      f"{sum(1 for c in codes if c == wt.OK)} OK", flush=True)
         return
     if args.backward_pass:
         n5, chi5, gates5, _, tf = C.config5()

@@ -139,3 +139,31 @@ def test_config4_slices_vs_oracle(wt):
             assert done >= 4, cap
     finally:
         om.set_qr_rank_tol(0.0)
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+def test_config4_mixed_forms(wt, prec):
+    """Config 4 in the mixed forms (FP32 or split-BF16 early Jacobi sweeps, FP64 finish, FP64
+    truncation): the 256-slice batch runs every slice (status OK, finite amplitude), and on the
+    slices that never reach the range finder (chi <= 64: theta at most 128 x 128, below the
+    range finder's size threshold) the amplitude matches the FP64 batch to 1e-12 of the state's
+    norm (the states are normalized, so this bounds the absolute error of <0...0|psi>; measured
+    9.6e-15)."""
+    slices = C.config4()
+    amps64, codes64 = wt.run_slices(slices, precision=wt.PRECISION_FP64, workers=0)
+    t = time.time()
+    amps, codes = wt.run_slices(slices, precision=prec, workers=0)
+    say("config4 precision %d: 256 slices batched %.2f s" % (prec, time.time() - t))
+    assert all(c == wt.OK for c in codes) and all(c == wt.OK for c in codes64)
+    assert np.all(np.isfinite(amps))
+    worst_abs, worst_rel, checked = 0.0, 0.0, 0
+    for i, (n, chi, gates) in enumerate(slices):
+        if chi > 64:
+            continue
+        checked += 1
+        worst_abs = max(worst_abs, abs(amps[i] - amps64[i]))
+        worst_rel = max(worst_rel, rel(amps[i], amps64[i]))
+    say("config4 precision %d vs FP64 on %d range-finder-free slices: worst |diff| %.1e, worst rel %.1e"
+        % (prec, checked, worst_abs, worst_rel))
+    assert checked >= 100
+    assert worst_abs <= 1e-12
