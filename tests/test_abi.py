@@ -77,10 +77,13 @@ def header_structs():
     return out
 
 
+GUIDE = os.path.join(ROOT, "INTEGRATION.md")  # the integration guide with the Rust binding
+
+
 def rust_block():
-    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    text = open(GUIDE).read()
     m = re.search(r"## 2\. FFI declarations.*?```rust\n(.*?)```", text, flags=re.S)
-    assert m, "INTEGRATION.md has no FFI declaration block"
+    assert m, "the integration guide has no FFI declaration block"
     return m.group(1)
 
 
@@ -94,7 +97,7 @@ def rust_structs():
 
 
 def test_rust_structs_match_header():
-    """Every #[repr(C)] struct of the Rust binding in INTEGRATION.md has the header's fields, in
+    """Every #[repr(C)] struct of the integration guide's Rust binding has the header's fields, in
     order and with the matching width: a missing field would let wqc_tn_get_stats write past a
     Rust caller's struct."""
     h, r = header_structs(), rust_structs()
