@@ -634,6 +634,7 @@ struct wqc_tn_state {
     bool trace_jacobi = false;
     double stall_off = STALL_OFF;
     bool use_lq = true;
+    bool use_lq2 = true;  // a second factorization R = L2 Q2 before the Jacobi (with use_lq)
     bool use_splitk = true;  // WQC_TN_SPLITK=0 turns split-K GEMMs off
     bool use_wide = true;    // WQC_TN_JACOBI_WIDE=0 keeps every Jacobi round at 256 threads
     DevBuf trace_buf;
@@ -652,6 +653,8 @@ struct wqc_tn_state {
         // decisions); WQC_TN_LQ=0/1 overrides either.
         use_lq = prec != WQC_TN_PRECISION_FP64;
         if (const char* e = std::getenv("WQC_TN_LQ")) use_lq = std::atoi(e) != 0;
+        use_lq2 = use_lq;
+        if (const char* e = std::getenv("WQC_TN_LQ2")) use_lq2 = use_lq && std::atoi(e) != 0;
         if (const char* e = std::getenv("WQC_TN_SPLITK")) use_splitk = std::atoi(e) != 0;
         if (const char* e = std::getenv("WQC_TN_JACOBI_WIDE")) use_wide = std::atoi(e) != 0;
         if (const char* e = std::getenv("WQC_TN_GRAPH")) use_graph = std::atoi(e) != 0;
@@ -1293,6 +1296,7 @@ struct wqc_tn_state {
         size_t o_theta, o_thetaT, o_J, o_sig;
         size_t o_Y0, o_Z, o_Y, o_V, o_T, o_Z2, o_B, o_JB, o_sigB, o_W, o_W2;
         size_t o_lqU = 0, o_lqUT = 0, o_lqUTh = 0, o_lqR = 0, o_lqW = 0, o_lqT = 0, o_lqG = 0;  // LQ preconditioning
+        size_t o_lq2U = 0, o_lq2UT = 0, o_lq2UTh = 0, o_lq2R = 0;  // its second factorization (of R)
         OmegaEntry* om;
         size_t chi_kept;
         bool fallback;
@@ -1346,6 +1350,12 @@ struct wqc_tn_state {
                 j.o_lqW = take(n_ex * wqc::LQ_SB * C16);
                 j.o_lqT = take((size_t)wqc::LQ_SB * wqc::LQ_SB * C16);
                 j.o_lqG = take((size_t)wqc::LQ_SB * wqc::LQ_SB * C16);
+                if (use_lq2) {
+                    j.o_lq2U = take(n_ex * n_ex * C16);
+                    j.o_lq2UT = take(n_ex * n_ex * C16);
+                    j.o_lq2UTh = take(n_ex * n_ex * C16);
+                    j.o_lq2R = take(n_ex * n_ex * C16);
+                }
             }
             if (j.rsvd) {
                 j.om = &omega_for(dev, rows, cols, j.ell);
@@ -1818,11 +1828,13 @@ struct wqc_tn_state {
     // Householder LQ of every eligible Jacobi job (X = L Q, panels of LQ_NB rows, trailing
     // rows updated by two GEMMs per panel); the job then rotates the rows of R = L^H.
     void lq_precondition(std::vector<JobInfo>& J, const std::vector<SplitBatch*>& batches, char* base,
-                         std::vector<wqc::LqJob>& lq, std::vector<double2*>& lqw) {
+                         std::vector<wqc::LqJob>& lq, std::vector<double2*>& lqw, std::vector<wqc::LqJob>& lq2,
+                         std::vector<double2*>& lqw2) {
         auto P = [&](size_t o) { return reinterpret_cast<double2*>(base + o); };
         lq.clear();
         lqw.clear();
-        int max_n = 0;
+        lq2.clear();
+        lqw2.clear();
         for (auto* b : batches) {
             const size_t nb = b->ids.size();
             b->pre.assign(nb, 0);
@@ -1853,11 +1865,48 @@ struct wqc_tn_state {
                 x.X = q.R;
                 x.m = x.n;
                 x.ldx = x.n;
-                max_n = std::max(max_n, q.n);
             }
         }
         if (lq.empty()) return;
+        lq_run(lq, lqw);
+        // Second factorization R = L2 Q2 of every preconditioned job, the Jacobi then rotating
+        // the rows of R2 = L2^H (n x n) instead of R's: R2 is nearer diagonal (a further step of
+        // the unshifted QR iteration), which on the splits with a graded tail of singular values
+        // through the cutoff takes 6 FP64 sweeps instead of 18. Every preconditioned job has
+        // its buffers, so lq2[k] belongs to lq[k].
+        if (!use_lq2) return;
+        for (auto* b : batches)
+            for (size_t i = 0; i < b->ids.size(); ++i) {
+                if (!b->pre[i]) continue;
+                JobInfo& j = J[b->ids[i]];
+                JacobiJob& x = b->jj[i];
+                if (!j.o_lq2U) fail(WQC_TN_EXEC_FAILED, "second LQ factorization without its buffers");
+                wqc::LqJob q;
+                q.X = x.X;  // R, factored in place
+                q.Ubc = P(j.o_lq2U);
+                q.UTc = P(j.o_lq2UT);
+                q.UThc = P(j.o_lq2UTh);
+                q.R = P(j.o_lq2R);
+                q.J = x.J;
+                q.T64 = P(j.o_lqT);
+                q.Gu = P(j.o_lqG);
+                q.n = x.n;
+                q.m = x.n;
+                q.ldx = x.ldx;
+                q.pad = 0;
+                lq2.push_back(q);
+                lqw2.push_back(P(j.o_lqW));
+                x.X = q.R;
+            }
+        lq_run(lq2, lqw2);
+    }
+
+    // The factorization's fixed sequence of launches for the jobs of lq (each X = L Q in place,
+    // R = L^H extracted).
+    void lq_run(const std::vector<wqc::LqJob>& lq, const std::vector<double2*>& lqw) {
         Timer tm(this, 6);
+        int max_n = 0;
+        for (const auto& q : lq) max_n = std::max(max_n, q.n);
         const int nl = (int)lq.size();
         wqc::LqJob* dl = upload(lq);
         const int NB = wqc::LQ_NB, SB = wqc::LQ_SB;
@@ -2003,9 +2052,9 @@ struct wqc_tn_state {
                 }
             gemm(g, 4);
         }
-        std::vector<wqc::LqJob> lq;
-        std::vector<double2*> lqw;
-        lq_precondition(J, batches, base, lq, lqw);
+        std::vector<wqc::LqJob> lq, lq2;
+        std::vector<double2*> lqw, lqw2;
+        lq_precondition(J, batches, base, lq, lqw, lq2, lqw2);
         std::vector<JacobiJob> all;
         std::vector<long> offs;
         for (auto* b : batches) {
@@ -2014,7 +2063,28 @@ struct wqc_tn_state {
         }
         if (all.empty()) return;
         jacobi_any(all, PD(0), offs, d_flags);
-        lq_form_m(lq, lqw);
+        if (lq2.empty()) {
+            lq_form_m(lq, lqw);
+        } else {
+            // X = L Q = R^H Q and R = L2 Q2, so X = (Q2^H J^H)(Y2 Q) with Y2 = J R2 the rotated rows:
+            // the plain form's X = J_eff^H X_eff with J_eff = J Q2 (into R's buffer) and
+            // X_eff = [Y2 0] Q (into X's own buffer), both by the reflectors applied in reverse.
+            lq_form_m(lq2, lqw2);
+            for (size_t k = 0; k < lq.size(); ++k) lq[k].J = lq2[k].R;
+            lq_form_m(lq, lqw);
+            size_t k = 0;
+            for (auto* b : batches)
+                for (size_t i = 0; i < b->ids.size(); ++i) {
+                    if (!b->pre[i]) continue;
+                    JacobiJob& x = b->jj[i];
+                    x.J = lq2[k].X;
+                    x.X = b->orig[i];
+                    x.ldx = b->orig_ld[i];
+                    x.m = b->m0[i];
+                    b->pre[i] = 0;
+                    ++k;
+                }
+        }
         // Rebuilt probes U S V^H x = F^H (G x): (F, G) = (J, rotated rows) for a plain job
         // (A = J^H X_final), (rotated rows of R, M) for an LQ-preconditioned one (A = Y^H M).
         {
