@@ -800,6 +800,49 @@ __device__ __forceinline__ f4v mfma_bf16(bf8v a, bf8v b, f4v c) {
 // NT = 512 for rounds with at most one block pair per CU, as in k_jacobi_round. GB: the Gram
 // matrix on BF16 MFMA (split-BF16, FP32 accumulation) instead of FP32 MFMA; the eigensolve and
 // the rotations stay FP32.
+// Rotation parameters of the low rounds in FP32 (the rotation is applied in FP32 and J is made
+// unitary in FP64 once, after the last low round, so parameters accurate to FP32 suffice):
+// |c| by hypotf and the tolerance against sqrt(a) sqrt(d), so nothing is squared and no entry
+// of a small row underflows; hardware reciprocal and reciprocal square root (about 1 ulp).
+struct Rot32 {
+    int act;
+    float cs, tg;
+    float2 m01, m10;
+};
+
+__device__ __forceinline__ Rot32 rot_params32(float a, float d, float2 c, float tol, float low) {
+    Rot32 r;
+    const float g = hypotf(c.x, c.y);
+    const bool act = g > 0.f && fminf(a, d) > low && g > tol * (sqrtf(a) * sqrtf(d));
+    const float gs = act ? g : 1.f;
+    const float rg = __builtin_amdgcn_rcpf(gs);
+    const float zeta = 0.5f * (d - a) * rg;
+    const float w = fmaf(zeta, zeta, 1.f);
+    const float t = copysignf(__builtin_amdgcn_rcpf(fabsf(zeta) + w * __builtin_amdgcn_rsqf(w)), zeta);
+    const float cs = __builtin_amdgcn_rsqf(fmaf(t, t, 1.f));
+    const float sn = cs * t;
+    const float phx = c.x * rg, phy = c.y * rg;
+    r.act = act ? 1 : 0;
+    r.cs = act ? cs : 1.f;
+    r.m01 = act ? make_float2(-phx * sn, -phy * sn) : make_float2(0.f, 0.f);
+    r.m10 = act ? make_float2(phx * sn, -phy * sn) : make_float2(0.f, 0.f);
+    r.tg = act ? t * gs : 0.f;
+    return r;
+}
+
+// As rot_from_lane: the row-block rotation from the lane that derived it; tg stays the lane's
+// own (used only on a diagonal tile, whose row rotation is the lane's own).
+__device__ __forceinline__ Rot32 rot_from_lane32(const Rot32& own, int src) {
+    Rot32 r;
+    r.act = __shfl(own.act, src, 64);
+    r.cs = __shfl(own.cs, src, 64);
+    r.tg = own.tg;
+    r.m01.x = __shfl(own.m01.x, src, 64);
+    r.m01.y = __shfl(own.m01.y, src, 64);
+    r.m10 = make_float2(-r.m01.x, r.m01.y);
+    return r;
+}
+
 template <int NT, bool GB>
 __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round_lo(const JacobiLoJob* __restrict__ jobs,
                                                          int round, float tol_base, double nu2_base,
@@ -1051,7 +1094,6 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round_lo(const
     auto fa = [](float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); };
     auto fs = [](float2 a, float k) { return make_float2(a.x * k, a.y * k); };
     auto fc = [](float2 a) { return make_float2(a.x, -a.y); };
-    auto f2 = [](double2 a) { return make_float2((float)a.x, (float)a.y); };
     for (int sweep = 0; sweep < inner_cap; ++sweep) {
         ++sweeps_done;
         // The pair's two blocks come out of their previous rounds mutually orthogonal inside;
@@ -1069,15 +1111,13 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round_lo(const
                     circle_pair(step, bk, p1, q1);
                     circle_pair(step, bk2, p2, q2);
                 }
-                // Parameters in FP64 from the FP32 Gram entries (no underflow of |G_pq|^2).
-                const float2 c2 = Gc[p2][q2];
-                const Rot64 r2 = rot_params64(Gc[p2][p2].x, Gc[q2][q2].x, make_double2(c2.x, c2.y), tol, low);
-                const Rot64 r1 = rot_from_lane(r2, bk & 15);
+                const Rot32 r2 = rot_params32(Gc[p2][p2].x, Gc[q2][q2].x, Gc[p2][q2], tol, low);
+                const Rot32 r1 = rot_from_lane32(r2, bk & 15);
                 float2 h00 = Gc[p1][p2], h01 = Gc[p1][q2], h10 = Gc[q1][p2], h11 = Gc[q1][q2];
                 float2 t00 = Tc[p1][p2], t01 = Tc[p1][q2], t10 = Tc[q1][p2], t11 = Tc[q1][q2];
                 if (r1.act) {
-                    const float cs = (float)r1.cs;
-                    const float2 m01 = f2(r1.m01), m10 = f2(r1.m10);
+                    const float cs = r1.cs;
+                    const float2 m01 = r1.m01, m10 = r1.m10;
                     const float2 x00 = fa(fs(h00, cs), fm(m01, h10)), x01 = fa(fs(h01, cs), fm(m01, h11));
                     const float2 x10 = fa(fm(m10, h00), fs(h10, cs)), x11 = fa(fm(m10, h01), fs(h11, cs));
                     h00 = x00;
@@ -1092,8 +1132,8 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round_lo(const
                     t11 = y11;
                 }
                 if (r2.act) {
-                    const float cs = (float)r2.cs;
-                    const float2 c01 = fc(f2(r2.m01)), c10 = fc(f2(r2.m10));
+                    const float cs = r2.cs;
+                    const float2 c01 = fc(r2.m01), c10 = fc(r2.m10);
                     const float2 n00 = fa(fs(h00, cs), fm(h01, c01)), n01 = fa(fm(h00, c10), fs(h01, cs));
                     const float2 n10 = fa(fs(h10, cs), fm(h11, c01)), n11 = fa(fm(h10, c10), fs(h11, cs));
                     h00 = n00;
@@ -1102,8 +1142,8 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round_lo(const
                     h11 = n11;
                 }
                 if (bk == bk2 && r1.act) {
-                    h00 = make_float2((float)(Gc[p1][p1].x - r1.tg), 0.f);
-                    h11 = make_float2((float)(Gc[q1][q1].x + r1.tg), 0.f);
+                    h00 = make_float2(Gc[p1][p1].x - r1.tg, 0.f);
+                    h11 = make_float2(Gc[q1][q1].x + r1.tg, 0.f);
                     h01 = make_float2(0.f, 0.f);
                     h10 = h01;
                     s_any = 1;
