@@ -484,27 +484,34 @@ def test_mixed_form_truncating_and_range_finder(wt, prec):
 
 
 # ---------------------------------------------------------------- LQ-preconditioned splits
-@pytest.mark.parametrize("dl,dr,bond,chi,lq_env", [
-    (48, 96, 80, 96, None), (96, 48, 80, 96, None), (70, 70, 70, 64, None), (40, 300, 60, 128, None),
-    (40, 550, 50, 128, None),   # 80 x 1100: wider than the LQ panels take, plain rows
-    (64, 100, 90, 80, "1"),     # the FP64 form with WQC_TN_LQ=1
+LQ1 = {"WQC_TN_LQ2": "0"}
+
+
+@pytest.mark.parametrize("dl,dr,bond,chi,env", [
+    (48, 96, 80, 96, {}), (96, 48, 80, 96, {}), (70, 70, 70, 64, {}), (40, 300, 60, 128, {}),
+    (40, 550, 50, 128, {}),     # 80 x 1100: wider than the LQ panels take, plain rows
+    (48, 96, 80, 96, LQ1), (96, 48, 80, 96, LQ1), (70, 70, 70, 64, LQ1), (40, 300, 60, 128, LQ1),
+    (64, 100, 90, 80, {"WQC_TN_LQ": "1"}),                      # the FP64 form with WQC_TN_LQ=1
+    (64, 100, 90, 80, {"WQC_TN_LQ": "1", "WQC_TN_LQ2": "0"}),
 ])
-def test_lq_preconditioned_split_vs_oracle(wt, dl, dr, bond, chi, lq_env):
-    """The mixed form rotates R = L^H of a Householder LQ (n >= 64 rows, at most 1024 columns)
-    and swaps the factor roles; the truncated products, bonds and sqrt(sigma) gauge must match
-    the oracle's exact split as the plain path does (tall, wide, square, truncating, too wide
-    for the panels, and the FP64 form with the preconditioning forced on)."""
+def test_lq_preconditioned_split_vs_oracle(wt, dl, dr, bond, chi, env):
+    """The mixed form factors each split's matrix twice (X = L Q, then L^H = L2 Q2), rotates the
+    rows of R2 = L2^H and forms the factors in the plain layout (J Q2, [Y2 0] Q); with
+    WQC_TN_LQ2=0 it factors once, rotates R = L^H and swaps the factor roles. Either way the
+    truncated products, bonds and sqrt(sigma) gauge must match the oracle's exact split as the
+    plain path does (tall, wide, square, truncating, too wide for the panels, and the FP64 form
+    with the preconditioning forced on)."""
     import os
 
     rng = np.random.default_rng(dl * 7919 + dr * 31 + chi)
     sites = random_mps_sites(rng, [1, dl, bond, dr, 1], 0.05)
-    prec = wt.PRECISION_MIXED_FP32 if lq_env is None else wt.PRECISION_FP64
-    if lq_env is not None:
-        os.environ["WQC_TN_LQ"] = lq_env
+    prec = wt.PRECISION_FP64 if "WQC_TN_LQ" in env else wt.PRECISION_MIXED_FP32
+    os.environ.update(env)
     try:
         g = wt.MpsState(4, chi, prec)
     finally:
-        os.environ.pop("WQC_TN_LQ", None)
+        for key in env:
+            os.environ.pop(key, None)
     o = om.OracleMps(4, chi)
     for i, s in enumerate(sites):
         g.set_site(i, s)
@@ -517,7 +524,7 @@ def test_lq_preconditioned_split_vs_oracle(wt, dl, dr, bond, chi, lq_env):
     pg = g.site(1).reshape(-1, k) @ g.site(2).reshape(k, -1)
     po = o.site(1).reshape(-1, k) @ o.site(2).reshape(k, -1)
     err = np.max(np.abs(pg - po)) / np.max(np.abs(po))
-    say("lq split", dl, dr, bond, chi, lq_env, "k", k, "product err", err)
+    say("lq split", dl, dr, bond, chi, env, "k", k, "product err", err)
     assert err <= TOL
     ln = np.linalg.norm(g.site(1).reshape(-1, k), axis=0)
     rn = np.linalg.norm(g.site(2).reshape(k, -1), axis=1)
