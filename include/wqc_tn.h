@@ -336,8 +336,9 @@ int wqc_tn_get_stats(const wqc_tn_state *st, wqc_tn_stats *out);
 
 /* Device time (ms) accumulated per kernel family since the state was created, for
  * measurement: index 0 = one-qubit, 1 = merge, 2 = range-finder GEMMs, 3 = Jacobi,
- * 4 = factors, 5 = readouts, 6 = LQ preconditioning of the Jacobi. Returns the number of
- * entries written. */
+ * 4 = factors, 5 = readouts, 6 = LQ preconditioning of the Jacobi, 7 = the mixed forms'
+ * change of basis between their FP32 and FP64 sweeps (Newton-Schulz on J, X <- J X).
+ * Returns the number of entries written. */
 int wqc_tn_kernel_times(const wqc_tn_state *st, double *ms, size_t len);
 int wqc_tn_enable_timing(wqc_tn_state *st, int enable);
 /* Launch counts per kernel family (same indexing as wqc_tn_kernel_times). */

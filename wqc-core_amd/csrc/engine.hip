@@ -272,15 +272,143 @@ void resolve_u4(const Mat4 u, bool ctrl_is_first, double2* u4) {
 // ---------------------------------------------------------------- device buffers
 // Grow-only device buffer; freed by release() or when its owner goes away (the owner makes
 // its device current first). Not copyable: one owner per allocation.
+// Device blocks that a device's states have let go, kept for the next buffer of about that
+// size on the same device. A state grows its workspaces as the bonds grow, callers typically
+// run one circuit per state, and hipFree waits for the whole device: without the cache every
+// circuit pays a few dozen reallocations and a hundred frees, each with the device drained.
+// A block carries an event, recorded on the stream of its last user when that user may still
+// have work queued on it; the stream that takes the block waits for that event on the device.
+// The cache holds at most kHeldMax bytes; it is emptied when a hipMalloc runs out of memory
+// (and the allocation tried again) and when the device handle is closed.
+struct BlockPool {
+    struct Block {
+        void* p = nullptr;
+        size_t bytes = 0;
+        hipEvent_t ev = nullptr;
+    };
+    static constexpr size_t kHeldMax = size_t(48) << 30;
+    std::mutex mu;
+    std::multimap<size_t, Block> blocks;
+    size_t held = 0;
+
+    static size_t round_up(size_t want) {
+        const size_t g = want < (size_t(1) << 20) ? size_t(64) << 10 : size_t(1) << 20;
+        return (std::max<size_t>(want, 1) + g - 1) / g * g;
+    }
+    // A block of at least `want` bytes for work queued on stream s: a cached one of at most
+    // twice that (plus 1 MiB), else a new one.
+    Block get(size_t want, hipStream_t s) {
+        want = round_up(want);
+        Block b;
+        bool hit = false;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            auto it = blocks.lower_bound(want);
+            if (it != blocks.end() && it->first <= 2 * want + (size_t(1) << 20)) {
+                b = it->second;
+                held -= b.bytes;
+                blocks.erase(it);
+                hit = true;
+            }
+        }
+        if (hit) {
+            if (b.ev) {
+                const hipError_t e = s ? hipStreamWaitEvent(s, b.ev, 0) : hipEventSynchronize(b.ev);
+                if (e != hipSuccess) {
+                    put(b, nullptr);
+                    hip_check(e, "waiting for a cached block");
+                }
+            }
+            return b;
+        }
+        b.bytes = want;
+        hipError_t e = hipMalloc(&b.p, want);
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            release_all();
+            e = hipMalloc(&b.p, want);
+        }
+        hip_check(e, "hipMalloc");
+        return b;
+    }
+    // Returns a block; s: the stream whose queued work may still use it (null: none may).
+    void put(Block b, hipStream_t s) noexcept {
+        if (!b.p) return;
+        if (s) {
+            bool ok = b.ev || hipEventCreateWithFlags(&b.ev, hipEventDisableTiming) == hipSuccess;
+            ok = ok && hipEventRecord(b.ev, s) == hipSuccess;
+            if (!ok) {
+                (void)hipStreamSynchronize(s);
+                (void)hipFree(b.p);
+                if (b.ev) (void)hipEventDestroy(b.ev);
+                return;
+            }
+        }
+        std::vector<Block> drop;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            blocks.emplace(b.bytes, b);
+            held += b.bytes;
+            while (held > kHeldMax && !blocks.empty()) {
+                auto it = std::prev(blocks.end());
+                drop.push_back(it->second);
+                held -= it->first;
+                blocks.erase(it);
+            }
+        }
+        for (auto& d : drop) {
+            (void)hipFree(d.p);
+            if (d.ev) (void)hipEventDestroy(d.ev);
+        }
+    }
+    size_t held_bytes() {
+        std::lock_guard<std::mutex> lk(mu);
+        return held;
+    }
+    void release_all() noexcept {
+        std::multimap<size_t, Block> all;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            all.swap(blocks);
+            held = 0;
+        }
+        for (auto& kv : all) {
+            (void)hipFree(kv.second.p);
+            if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
+        }
+    }
+    ~BlockPool() { release_all(); }
+};
+
+// A device buffer that only grows (its contents are not kept when it does). Bound to a pool
+// and a stream, it takes its blocks from the device's cache and returns them there, growing by
+// at least half its size each time; unbound, it calls hipMalloc and hipFree itself.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    BlockPool* pool = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
+    void bind(BlockPool* bp, hipStream_t s) {
+        release();
+        pool = bp;
+        stream = s;
+    }
     void ensure(size_t want) {
         if (want <= bytes) return;
+        if (pool) {
+            const size_t sz = std::max(want, bytes + bytes / 2);
+            release();
+            const BlockPool::Block b = pool->get(sz, stream);
+            p = b.p;
+            bytes = b.bytes;
+            ev = b.ev;
+            return;
+        }
         if (p) HIPC(hipFree(p));
         p = nullptr;
         bytes = 0;
@@ -289,10 +417,16 @@ struct DevBuf {
         HIPC(hipMalloc(&p, sz));
         bytes = sz;
     }
-    void release() {
-        if (p) (void)hipFree(p);
+    // synced: no work queued on the stream uses the buffer any more.
+    void release(bool synced = false) {
+        if (pool) {
+            if (p) pool->put(BlockPool::Block{p, bytes, ev}, synced ? nullptr : stream);
+        } else if (p) {
+            (void)hipFree(p);
+        }
         p = nullptr;
         bytes = 0;
+        ev = nullptr;
     }
 };
 
@@ -366,6 +500,33 @@ struct wqc_tn_device {
     // and, with WQC_TN_TUNE_CACHE=<file>, in that file across processes.
     std::map<std::array<int, 6>, int> gemm_tile;
     std::string tune_path;
+    // Device blocks and pinned host staging that closed states have let go, for the next state.
+    BlockPool pool;
+    std::mutex pin_mu;
+    std::vector<std::pair<void*, size_t>> pinned;
+    void* get_pinned(size_t bytes) {
+        {
+            std::lock_guard<std::mutex> lk(pin_mu);
+            for (size_t i = 0; i < pinned.size(); ++i)
+                if (pinned[i].second == bytes) {
+                    void* p = pinned[i].first;
+                    pinned.erase(pinned.begin() + i);
+                    return p;
+                }
+        }
+        void* p = nullptr;
+        HIPC(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+        return p;
+    }
+    void put_pinned(void* p, size_t bytes) noexcept {  // no queued copy may still use p
+        if (!p) return;
+        std::lock_guard<std::mutex> lk(pin_mu);
+        if (pinned.size() < 16) {
+            pinned.emplace_back(p, bytes);
+            return;
+        }
+        (void)hipHostFree(p);
+    }
     void load_tune_cache() {
         if (tune_path.empty()) return;
         FILE* f = std::fopen(tune_path.c_str(), "r");
@@ -385,6 +546,8 @@ struct wqc_tn_device {
     }
     ~wqc_tn_device() {
         (void)hipSetDevice(ordinal);
+        for (auto& pp : pinned) (void)hipHostFree(pp.first);
+        pool.release_all();
         for (auto& kv : omega) {
             if (kv.second.omega_cols) (void)hipFree(kv.second.omega_cols);
             if (kv.second.pos) (void)hipFree(kv.second.pos);
@@ -462,6 +625,8 @@ struct Site {
     size_t cap = 0;  // complex elements
     int l = 1, r = 1;
     bool owned = true;  // false: carved from the state's site arena (freed with it)
+    size_t block = 0;   // owned and from the device's block cache: the block's bytes
+    hipEvent_t ev = nullptr;  // that block's event
 };
 
 // One two-site unit: sites p and p+1, resolved 4x4 unitary.
@@ -509,8 +674,8 @@ struct wqc_tn_state {
     wqc_tn_stats stats{};
     uint64_t site_bytes = 0;
     bool timing = false;
-    double kms[7] = {0, 0, 0, 0, 0, 0, 0};
-    uint64_t klaunch[7] = {0, 0, 0, 0, 0, 0, 0};
+    double kms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t klaunch[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     struct TimedSpan {
         hipEvent_t a, b;
         int fam;
@@ -529,30 +694,40 @@ struct wqc_tn_state {
     ~wqc_tn_state() {
         (void)hipSetDevice(dev ? dev->ordinal : 0);
         if (stream) (void)hipStreamSynchronize(stream);
-        for (auto& s : sites)
-            if (s.ptr && s.owned) (void)hipFree(s.ptr);
-        site_arena.release();
-        ws.release();
-        ws32.release();
-        ews.release();
-        tab.release();
-        jstats_buf.release();
-        trace_buf.release();
-        splitk_buf.release();
-        gather.release();
-        ov_buf.release();
-        env_buf.release();
-        amp_buf.release();
-        probe_buf.release();
-        probe_p_buf.release();
+        for (auto& s : sites) free_site(s, true);
+        for (DevBuf* b : state_bufs()) b->release(true);
         for (auto& sp : spans) {
             (void)hipEventDestroy(sp.a);
             (void)hipEventDestroy(sp.b);
         }
         for (auto e : event_pool) (void)hipEventDestroy(e);
-        if (htab) (void)hipHostFree(htab);
-        if (hread) (void)hipHostFree(hread);
+        if (dev) {
+            dev->put_pinned(htab, htab_cap);
+            dev->put_pinned(hread, hread_cap * sizeof(double));
+        } else {
+            if (htab) (void)hipHostFree(htab);
+            if (hread) (void)hipHostFree(hread);
+        }
         if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // The state's own device buffers (bound to the device's block cache when the state has a
+    // stream).
+    std::vector<DevBuf*> state_bufs() {
+        return {&site_arena, &ws, &ws32, &ews, &tab, &jstats_buf, &trace_buf, &splitk_buf,
+                &gather, &ov_buf, &env_buf, &amp_buf, &probe_buf, &probe_p_buf};
+    }
+    // Gives an owned site's memory back (synced: no queued work reads it any more).
+    void free_site(Site& s, bool synced) {
+        if (s.ptr && s.owned) {
+            if (s.block)
+                dev->pool.put(BlockPool::Block{s.ptr, s.block, s.ev}, synced ? nullptr : stream);
+            else
+                (void)hipFree(s.ptr);
+        }
+        s.ptr = nullptr;
+        s.block = 0;
+        s.ev = nullptr;
     }
 
     // ------------------------------------------------------------ helpers
@@ -577,15 +752,27 @@ struct wqc_tn_state {
         if (elems <= s.cap) return;
         HIPC(hipStreamSynchronize(stream));
         double2* np = nullptr;
-        HIPC(hipMalloc(&np, elems * sizeof(double2)));
+        BlockPool::Block b;
+        if (stream) {
+            b = dev->pool.get(elems * sizeof(double2), stream);
+            np = static_cast<double2*>(b.p);
+        } else {
+            HIPC(hipMalloc(&np, elems * sizeof(double2)));
+        }
         if (s.ptr) {
-            HIPC(hipMemcpy(np, s.ptr, std::min(s.cap, elems) * sizeof(double2), hipMemcpyDeviceToDevice));
-            if (s.owned) HIPC(hipFree(s.ptr));
+            if (stream)
+                HIPC(hipMemcpyAsync(np, s.ptr, std::min(s.cap, elems) * sizeof(double2), hipMemcpyDeviceToDevice,
+                                    stream));
+            else
+                HIPC(hipMemcpy(np, s.ptr, std::min(s.cap, elems) * sizeof(double2), hipMemcpyDeviceToDevice));
+            free_site(s, false);
         }
         site_bytes += s.owned ? (elems - s.cap) * sizeof(double2) : elems * sizeof(double2);
         s.ptr = np;
         s.cap = elems;
         s.owned = true;
+        s.block = b.bytes;
+        s.ev = b.ev;
         note_peak();
     }
 
@@ -684,11 +871,12 @@ struct wqc_tn_state {
         precision = prec;
         HIPC(hipSetDevice(dev->ordinal));
         HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        for (DevBuf* b : state_bufs()) b->bind(&dev->pool, stream);
         htab_cap = 8u << 20;
-        HIPC(hipHostMalloc((void**)&htab, htab_cap, hipHostMallocDefault));
+        htab = static_cast<char*>(dev->get_pinned(htab_cap));
         tab.ensure(htab_cap);
         hread_cap = 1u << 20;
-        HIPC(hipHostMalloc((void**)&hread, hread_cap * sizeof(double), hipHostMallocDefault));
+        hread = static_cast<double*>(dev->get_pinned(hread_cap * sizeof(double)));
         sites.assign(n, Site{});
         std::vector<double2> init = {make_double2(1, 0), make_double2(0, 0)};
         for (int i = 0; i < n; ++i) {
@@ -758,8 +946,8 @@ struct wqc_tn_state {
         }
         for (auto& s : old)
             if (s.ptr && s.owned) {
-                HIPC(hipFree(s.ptr));
                 site_bytes -= s.cap * sizeof(double2);
+                free_site(s, true);
             }
         static const double2 zero_state[2] = {make_double2(1, 0), make_double2(0, 0)};
         for (int i = 0; i < n; ++i) {
@@ -1160,7 +1348,7 @@ struct wqc_tn_state {
         // J = (complex128) J32, made unitary to FP64 precision by two Newton-Schulz steps
         // J <- J (3I - J^H J) / 2 (the FP32 rounds leave it unitary to about 1e-6).
         {
-            Timer tm(this, 2);
+            Timer tm(this, 7);
             std::vector<const float2*> js(nj);
             std::vector<double2*> jd(nj);
             std::vector<long> jc(nj);
@@ -1190,13 +1378,13 @@ struct wqc_tn_state {
                     g2.push_back(gj(src_j, nn, 0, w, nn, 0, dst_j, nn, nn, nn, nn, -0.5, 1.5));
                 }
                 copy_batch(cs, cd, cc);
-                gemm(g1);
-                gemm(g2);
+                gemm(g1, 7);
+                gemm(g2, 7);
             }
         }
         // X <- J X in FP64 (through a scratch copy), then the FP64 rounds.
         {
-            Timer tm(this, 2);
+            Timer tm(this, 7);
             std::vector<GemmJob> g;
             std::vector<const double*> cs;
             std::vector<double*> cd;
@@ -1210,7 +1398,7 @@ struct wqc_tn_state {
                                jobs[k].m, jobs[k].n, 1, 0));
             }
             copy_batch(cs, cd, cc);
-            gemm(g);
+            gemm(g, 7);
         }
         jacobi(jobs, d_sigma, sig_off, d_flags);
         last_sweeps_lo = lo_count;
@@ -3368,10 +3556,11 @@ int wqc_tn_state_new(wqc_tn_device* dev, size_t qubit_count, size_t max_bond_dim
         if (qubit_count > (1u << 20)) fail(WQC_TN_INVALID_ARG, "qubit count too large");
         if (!valid_precision(precision)) fail(WQC_TN_INVALID_ARG, "unknown precision mode");
         // ensure_mps_workspace_budget analogue (src/tn/mps.rs:26-48): N chi^2 32 bytes
-        // against free device memory.
+        // against free device memory (the blocks the device's cache holds count as free).
         HIPC(hipSetDevice(dev->ordinal));
         size_t fr = 0, tot = 0;
         HIPC(hipMemGetInfo(&fr, &tot));
+        fr += dev->pool.held_bytes();
         unsigned __int128 need = (unsigned __int128)qubit_count * max_bond_dim * max_bond_dim * 32;
         if (need > fr)
             fail(WQC_TN_OUT_OF_MEMORY, "insufficient device memory: required " +
@@ -4252,7 +4441,7 @@ int wqc_tn_get_stats(const wqc_tn_state* st, wqc_tn_stats* out) {
 
 int wqc_tn_kernel_times(const wqc_tn_state* st, double* ms, size_t len) {
     if (!st || !ms) return 0;
-    size_t k = std::min<size_t>(len, 7);
+    size_t k = std::min<size_t>(len, 8);
     for (size_t i = 0; i < k; ++i) ms[i] = st->kms[i];
     return (int)k;
 }
@@ -4265,7 +4454,7 @@ int wqc_tn_enable_timing(wqc_tn_state* st, int enable) {
 
 int wqc_tn_kernel_launches(const wqc_tn_state* st, uint64_t* count, size_t len) {
     if (!st || !count) return 0;
-    size_t k = std::min<size_t>(len, 7);
+    size_t k = std::min<size_t>(len, 8);
     for (size_t i = 0; i < k; ++i) count[i] = st->klaunch[i];
     return (int)k;
 }

@@ -436,7 +436,7 @@ class MpsState:
     def enable_timing(self, on=True):
         _check(load().wqc_tn_enable_timing(self._h, 1 if on else 0))
 
-    FAMILIES = ("one_qubit", "merge", "range_finder", "jacobi", "factors", "readout", "lq")
+    FAMILIES = ("one_qubit", "merge", "range_finder", "jacobi", "factors", "readout", "lq", "basis")
 
     def kernel_times(self):
         """Device milliseconds per kernel family (timing must be enabled)."""
