@@ -126,6 +126,39 @@ def test_truncating_brickwork_vs_oracle(wt, n, depth, chi):
     assert rel(a, b) <= TOL
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_reused_device_blocks_give_identical_results(wt, prec):
+    """States take their sites and workspaces from the device's block cache, which hands back
+    blocks other states have written (nothing clears them). A truncating brickwork run on a
+    fresh state, again after states of other sizes have come and gone (so its buffers are
+    blocks those states filled), and a third time while another state is still alive must give
+    the same bonds, counters and amplitude bit for bit."""
+    n, depth, chi = 24, 16, 24
+    gates = C.brickwork(n, depth, 4242)
+
+    def run():
+        st = wt.MpsState(n, chi, prec)
+        st.apply_circuit(gates)
+        out = (st.bond_dims(), st.amplitude_at_compact_zero(), st.stats()["truncated_splits"])
+        st.free()
+        return out
+
+    first = run()
+    assert first[2] > 0
+    for m, c in ((40, 64), (12, 8), (30, 48)):
+        other = wt.MpsState(m, c, prec)
+        other.apply_circuit(C.brickwork(m, 12, m))
+        other.free()
+    again = run()
+    held = wt.MpsState(30, 48, prec)
+    held.apply_circuit(C.brickwork(30, 12, 30))
+    third = run()
+    held.free()
+    say("block reuse", prec, first[1], again[1], third[1])
+    assert again == first
+    assert third == first
+
+
 def test_disjoint_layer_equals_sequential_bitwise(wt):
     # src/tn/mps.rs:1772-1793, and disjoint batching inside apply_circuit.
     prep = C.brickwork(8, 4, 5)

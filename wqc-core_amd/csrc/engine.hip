@@ -773,6 +773,10 @@ struct wqc_tn_state {
         s.owned = true;
         s.block = b.bytes;
         s.ev = b.ev;
+        // Callers write the site right away, some through the null stream (which does not
+        // wait for this stream): the copy above and a wait for the block's previous user end
+        // here.
+        if (stream) HIPC(hipStreamSynchronize(stream));
         note_peak();
     }
 
@@ -1861,7 +1865,8 @@ struct wqc_tn_state {
                 }
             sync();
             probe_p_buf.ensure(P.size() * sizeof(double2));
-            HIPC(hipMemcpy(probe_p_buf.p, P.data(), P.size() * sizeof(double2), hipMemcpyHostToDevice));
+            HIPC(hipMemcpyAsync(probe_p_buf.p, P.data(), P.size() * sizeof(double2), hipMemcpyHostToDevice, stream));
+            sync();
             probe_m = cap;
             note_peak();
         }
