@@ -661,6 +661,13 @@ struct wqc_tn_state {
     int precision = WQC_TN_PRECISION_FP64;
     std::vector<Site> sites;
     hipStream_t stream = nullptr;
+    // Streams for the further groups of a Jacobi batch's jobs (group 0 runs on `stream`),
+    // forked from and joined back into `stream` by ev_fork / ev_join; ev_sweep[g] marks group
+    // g's latest sweep.
+    static constexpr int kMaxGroups = 4;
+    hipStream_t streams_x[kMaxGroups - 1] = {};
+    hipStream_t stream2 = nullptr;  // streams_x[0]
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_sweep[kMaxGroups] = {};
     DevBuf ws;        // split workspace
     DevBuf ws32;      // mixed-precision rows and the J X product
     DevBuf ews;       // expectation environments and transfer buffers
@@ -694,6 +701,8 @@ struct wqc_tn_state {
     ~wqc_tn_state() {
         (void)hipSetDevice(dev ? dev->ordinal : 0);
         if (stream) (void)hipStreamSynchronize(stream);
+        for (hipStream_t x : streams_x)
+            if (x) (void)hipStreamSynchronize(x);
         for (auto& s : sites) free_site(s, true);
         for (DevBuf* b : state_bufs()) b->release(true);
         for (auto& sp : spans) {
@@ -708,6 +717,12 @@ struct wqc_tn_state {
             if (htab) (void)hipHostFree(htab);
             if (hread) (void)hipHostFree(hread);
         }
+        for (hipEvent_t e : {ev_fork, ev_join})
+            if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : ev_sweep)
+            if (e) (void)hipEventDestroy(e);
+        for (hipStream_t x : streams_x)
+            if (x) (void)hipStreamDestroy(x);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -838,6 +853,7 @@ struct wqc_tn_state {
         }
         if (const char* e = std::getenv("WQC_TN_JACOBI_TRACE")) trace_jacobi = std::atoi(e) != 0;
         if (const char* e = std::getenv("WQC_TN_JACOBI_STALL")) stall_off = std::max(0.0, std::atof(e));
+        if (const char* e = std::getenv("WQC_TN_JACOBI_GROUPS")) jacobi_groups = std::min(kMaxGroups, std::max(1, std::atoi(e)));
         // LQ preconditioning: on in the mixed form, off in the plain FP64 form (whose Jacobi on
         // the rows of theta' keeps small singular values relatively accurate; R = L^H carries
         // the LQ's rounding at the scale of the largest rows, which moves near-cutoff
@@ -875,6 +891,10 @@ struct wqc_tn_state {
         precision = prec;
         HIPC(hipSetDevice(dev->ordinal));
         HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        for (int g = 0; g + 1 < jacobi_groups; ++g) HIPC(hipStreamCreateWithFlags(&streams_x[g], hipStreamNonBlocking));
+        stream2 = streams_x[0];
+        for (hipEvent_t* e : {&ev_fork, &ev_join}) HIPC(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        for (int g = 0; g < jacobi_groups; ++g) HIPC(hipEventCreateWithFlags(&ev_sweep[g], hipEventDisableTiming));
         for (DevBuf* b : state_bufs()) b->bind(&dev->pool, stream);
         htab_cap = 8u << 20;
         htab = static_cast<char*>(dev->get_pinned(htab_cap));
@@ -1085,114 +1105,237 @@ struct wqc_tn_state {
     // writes the row norms into sigma (device, at sig_off[j]). Returns sweeps used.
     std::vector<int> last_sweeps, last_sweeps_lo;  // per job of the latest jacobi call
 
+    // The jobs of a Jacobi call in groups that sweep independently, each on its own stream
+    // (the state's stream and streams_x): a round waits for its slowest workgroup, and the
+    // other groups' rounds fill the slots its early finishers leave. Each group has its own job
+    // table, flags, pinned readback and sweep graphs; the host serves the groups in turn,
+    // issuing a group's next sweep as soon as it has read that group's flags.
+    struct SweepGroup {
+        std::vector<int> ids;  // the caller's job indices
+        hipStream_t s = nullptr;
+        int nj = 0;
+        const void* dj = nullptr;  // device job table of the group
+        int* d_done = nullptr;
+        int* d_rot = nullptr;
+        unsigned long long* d_off = nullptr;
+        char* h = nullptr;  // pinned readback: offmax (nj u64) then rotated (nj int)
+        std::vector<int> done, jpairs;
+        std::vector<double> prev_off;
+        int active = 0, sweep = 0;
+        bool in_flight = false;
+        SweepGraph graphs[2];  // 256-thread and 512-thread rounds
+        hipEvent_t ev = nullptr;
+    };
+    // Groups per Jacobi batch (WQC_TN_JACOBI_GROUPS, 1 to kMaxGroups). Measured on config 3:
+    // FP64 form 1456 / 1389 / 1374 / 1357 ms per circuit with 1 / 2 / 3 / 4, mixed form 693 /
+    // 690 / 690 / 689 ms.
+    int jacobi_groups = kMaxGroups;
+
+    // Splits nj jobs (block pairs jp[j]) into min(jacobi_groups, nj) groups balanced by block
+    // pairs (largest first, each to the lightest group). Sets up flags (in trace_buf,
+    // cleared), readback regions and events.
+    std::vector<std::unique_ptr<SweepGroup>> make_groups(const std::vector<int>& jp, size_t hread_off) {
+        const int nj = (int)jp.size();
+        const int ng = std::max(1, std::min(jacobi_groups, nj));
+        std::vector<std::unique_ptr<SweepGroup>> gs;
+        for (int g = 0; g < ng; ++g) {
+            gs.push_back(std::make_unique<SweepGroup>());
+            gs.back()->s = g == 0 ? stream : streams_x[g - 1];
+            gs.back()->ev = ev_sweep[g];
+        }
+        std::vector<int> order(nj);
+        for (int j = 0; j < nj; ++j) order[j] = j;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return jp[x] > jp[y]; });
+        std::vector<long> load(ng, 0);
+        for (int j : order) {
+            int g = 0;
+            for (int k = 1; k < ng; ++k)
+                if (load[k] < load[g]) g = k;
+            load[g] += jp[j];
+            gs[g]->ids.push_back(j);
+        }
+        for (auto& g : gs) std::sort(g->ids.begin(), g->ids.end());
+        // Flags: per group offmax (u64), rotated (int), done (int), 256-byte aligned.
+        size_t off = 0;
+        std::vector<size_t> fo(ng);
+        for (int g = 0; g < ng; ++g) {
+            fo[g] = off;
+            const size_t n = gs[g]->ids.size();
+            off += ((n * (sizeof(unsigned long long) + 2 * sizeof(int))) + 255) & ~(size_t)255;
+        }
+        trace_buf.ensure(off);
+        note_peak();
+        wqc::launch_fill_int(reinterpret_cast<int*>(trace_buf.p), 0, (int)(off / sizeof(int)), stream);
+        size_t ho = hread_off;
+        for (int g = 0; g < ng; ++g) {
+            SweepGroup& G = *gs[g];
+            G.nj = (int)G.ids.size();
+            char* base = (char*)trace_buf.p + fo[g];
+            G.d_off = reinterpret_cast<unsigned long long*>(base);
+            G.d_rot = reinterpret_cast<int*>(G.d_off + G.nj);
+            G.d_done = G.d_rot + G.nj;
+            G.h = reinterpret_cast<char*>(hread) + ho;
+            ho += ((size_t)G.nj * (sizeof(unsigned long long) + sizeof(int)) + 255) & ~(size_t)255;
+            if (ho > hread_cap * sizeof(double)) fail(WQC_TN_EXEC_FAILED, "readback buffer too small");
+            G.done.assign(G.nj, 0);
+            G.prev_off.assign(G.nj, 0.0);
+            G.active = G.nj;
+            for (int k = 0; k < G.nj; ++k) G.jpairs.push_back(jp[G.ids[k]]);
+        }
+        return gs;
+    }
+
+    // Runs the groups' sweeps to the end: issue(g, wide) queues one sweep of group g on the
+    // state's stream member (set to g's stream meanwhile) and its flag readback into g.h;
+    // after(g) reads them and updates g.done / g.active. The other streams are forked from the
+    // state's stream before and joined into it after.
+    template <class Issue, class After>
+    void sweep_groups(std::vector<std::unique_ptr<SweepGroup>>& gs, int max_sweeps, Issue&& issue, After&& after) {
+        const hipStream_t main = stream;
+        if (gs.size() > 1) {
+            HIPC(hipEventRecord(ev_fork, main));
+            for (auto& g : gs)
+                if (g->s != main) HIPC(hipStreamWaitEvent(g->s, ev_fork, 0));
+        }
+        struct Restore {
+            wqc_tn_state* st;
+            hipStream_t s;
+            ~Restore() { st->stream = s; }
+        } restore{this, main};
+        const int cus = dev->prop.multiProcessorCount;
+        auto launch = [&](SweepGroup& g) {
+            check_cancel();
+            int wgs = 0;
+            for (auto& o : gs)
+                for (int k = 0; k < o->nj; ++k)
+                    if (!o->done[k]) wgs += o->jpairs[k];
+            const bool wide = use_wide && wgs <= cus;
+            stream = g.s;
+            issue(g, wide);
+            HIPC(hipEventRecord(g.ev, g.s));
+            stream = main;
+            g.in_flight = true;
+        };
+        for (auto& g : gs)
+            if (g->active > 0 && max_sweeps > 0) launch(*g);
+        for (;;) {
+            bool any = false;
+            for (auto& gp : gs) {
+                SweepGroup& g = *gp;
+                if (!g.in_flight) continue;
+                any = true;
+                HIPC(hipEventSynchronize(g.ev));
+                g.in_flight = false;
+                stream = g.s;
+                after(g);
+                stream = main;
+                ++g.sweep;
+                if (g.active > 0 && g.sweep < max_sweeps) launch(g);
+            }
+            if (!any) break;
+        }
+        for (auto& g : gs)
+            if (g->s != main) {
+                HIPC(hipEventRecord(ev_join, g->s));
+                HIPC(hipStreamWaitEvent(main, ev_join, 0));
+            }
+    }
+
     void jacobi(std::vector<JacobiJob>& jobs, double* d_sigma, const std::vector<long>& sig_off,
                 int* d_flags) {
+        (void)d_flags;
         const int nj = (int)jobs.size();
         last_sweeps.assign(nj, 0);
         if (!nj) return;
-        JacobiJob* dj = upload(jobs);
-        int max_n = 1, max_m = 1;
-        for (auto& j : jobs) {
-            max_n = std::max(max_n, j.n);
-            max_m = std::max(max_m, j.m);
-        }
-        const int nb = (max_n + 15) / 16;
-        const int nbe = nb + (nb & 1);
-        const int rounds = nbe - 1, pairs = nbe / 2;
+        JacobiJob* dj_all = upload(jobs);
+        int max_n = 1;
+        for (auto& j : jobs) max_n = std::max(max_n, j.n);
         // Per job: tolerance JACOBI_TOL sqrt(m), null-row floor NULL_ROWS_SQ / n (in the kernel).
         const double tol = JACOBI_TOL;
         const double nu2 = NULL_ROWS_SQ;
-        int* d_done = d_flags + nj;
         unsigned long long* d_jstats = jstats_dev();
-        wqc::launch_fill_int(d_done, 0, nj, stream);
-        std::vector<int> done(nj, 0);
-        int active = nj;
         // Largest relative off-diagonal |G_ij| / sqrt(G_ii G_jj) seen before rotation, per job
         // and sweep (always collected): a job whose value is at most stall_off and did not
         // halve since the previous sweep has reached the rounding floor of its rotations
         // (quadratic convergence would have taken it far below) and stops.
         // WQC_TN_JACOBI_TRACE=1 prints it per sweep for the widest jobs, and the history of
         // the jobs that needed the most sweeps.
-        // The per-sweep flags live together: offmax (nj u64) then rotated (nj int), cleared by
-        // one fill and read back by one copy.
-        trace_buf.ensure((size_t)nj * (sizeof(unsigned long long) + sizeof(int)));
-        unsigned long long* d_off = reinterpret_cast<unsigned long long*>(trace_buf.p);
-        int* d_rot = reinterpret_cast<int*>(d_off + nj);
-        std::vector<double> prev_off(nj, 0.0);
         std::vector<std::vector<double>> hist(trace_jacobi ? nj : 0);
-        // Block pairs per job: a sweep whose unfinished jobs have no more pairs than the device
-        // has CUs runs its rounds on the 512-thread form (one workgroup per CU either way, and
-        // the round lasts as long as one workgroup).
+        // Block pairs per job: a sweep whose unfinished jobs have no more block pairs than the
+        // device has CUs runs its rounds on the 512-thread form (one workgroup per CU either way,
+        // and the round lasts as long as one workgroup).
         std::vector<int> jpairs(nj);
         for (int j = 0; j < nj; ++j) {
             const int b = (jobs[j].n + 15) / 16;
             jpairs[j] = (b + (b & 1)) / 2;
         }
-        const int cus = dev->prop.multiProcessorCount;
-        SweepGraph graphs[2];  // 256-thread and 512-thread rounds
-        for (int sweep = 0; sweep < MAX_SWEEPS && active > 0; ++sweep) {
-            check_cancel();
-            const auto t_sweep = std::chrono::steady_clock::now();
-            int wgs = 0;
-            for (int j = 0; j < nj; ++j)
-                if (!done[j]) wgs += jpairs[j];
-            const bool wide = use_wide && wgs <= cus;
-            {
-                Timer tm(this, 3);
-                run_sweep(graphs[wide ? 1 : 0], [&] {
-                    wqc::launch_fill_int(reinterpret_cast<int*>(d_off), 0, 3 * nj, stream);
-                    for (int r = 0; r < rounds; ++r)
-                        wqc::launch_jacobi_round(dj, nj, pairs, r, tol, nu2, inner_cap, d_rot, d_done, d_jstats,
-                                                 d_off, wide, stream);
-                });
-                klaunch[3] += rounds;
-            }
-            stats.jacobi_sweeps += active;
-            HIPC(hipMemcpyAsync(hread, d_off, nj * (sizeof(unsigned long long) + sizeof(int)), hipMemcpyDeviceToHost,
+        auto gs = make_groups(jpairs, 0);
+        for (auto& g : gs) {
+            std::vector<JacobiJob> sub;
+            for (int j : g->ids) sub.push_back(jobs[j]);
+            g->dj = upload(sub);
+        }
+        Timer tm(this, 3);
+        sweep_groups(gs, MAX_SWEEPS, [&](SweepGroup& g, bool wide) {
+            int gmax = 1;
+            for (int j : g.ids) gmax = std::max(gmax, jobs[j].n);
+            const int nb = (gmax + 15) / 16;
+            const int nbe = nb + (nb & 1);
+            const int rounds = nbe - 1, pairs = nbe / 2;
+            const JacobiJob* dj = static_cast<const JacobiJob*>(g.dj);
+            run_sweep(g.graphs[wide ? 1 : 0], [&] {
+                wqc::launch_fill_int(reinterpret_cast<int*>(g.d_off), 0, 3 * g.nj, stream);
+                for (int r = 0; r < rounds; ++r)
+                    wqc::launch_jacobi_round(dj, g.nj, pairs, r, tol, nu2, inner_cap, g.d_rot, g.d_done, d_jstats,
+                                             g.d_off, wide, stream);
+            });
+            klaunch[3] += rounds;
+            stats.jacobi_sweeps += g.active;
+            HIPC(hipMemcpyAsync(g.h, g.d_off, g.nj * (sizeof(unsigned long long) + sizeof(int)), hipMemcpyDeviceToHost,
                                 stream));
-            sync();
-            const unsigned long long* offv = reinterpret_cast<const unsigned long long*>(hread);
-            const int* rot = reinterpret_cast<const int*>(offv + nj);
+        }, [&](SweepGroup& g) {
+            const unsigned long long* offv = reinterpret_cast<const unsigned long long*>(g.h);
+            const int* rot = reinterpret_cast<const int*>(offv + g.nj);
             if (trace_jacobi) {
-                std::fprintf(stderr, "jacobi sweep %d active %d wgs %d%s %.3f ms:", sweep + 1, active, wgs,
-                             wide ? " wide" : "",
-                             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_sweep).count());
+                std::fprintf(stderr, "jacobi sweep %d stream %p active %d:", g.sweep + 1, (void*)g.s, g.active);
                 int shown = 0;
-                for (int j = 0; j < nj && shown < 4; ++j) {
-                    if (done[j] || jobs[j].n < max_n) continue;
+                for (int k = 0; k < g.nj && shown < 4; ++k) {
+                    const int j = g.ids[k];
+                    if (g.done[k] || jobs[j].n < max_n) continue;
                     double v;
-                    std::memcpy(&v, &offv[j], sizeof v);
+                    std::memcpy(&v, &offv[k], sizeof v);
                     std::fprintf(stderr, " [%dx%d] %.2e", jobs[j].n, jobs[j].m, v);
                     ++shown;
                 }
                 std::fprintf(stderr, "\n");
             }
             bool changed = false;
-            for (int j = 0; j < nj; ++j)
-                if (!done[j]) last_sweeps[j]++;
-            for (int j = 0; j < nj; ++j) {
-                if (done[j]) continue;
+            for (int k = 0; k < g.nj; ++k) {
+                if (g.done[k]) continue;
+                const int j = g.ids[k];
+                last_sweeps[j]++;
                 double v;
-                std::memcpy(&v, &offv[j], sizeof v);
-                const bool stalled = rot[j] && v <= stall_off && prev_off[j] > 0.0 && v >= 0.5 * prev_off[j];
-                prev_off[j] = v;
-                if (trace_jacobi) hist[j].push_back(rot[j] ? v : 0.0);
-                if (!rot[j] || stalled) {
-                    done[j] = 1;
-                    active--;
+                std::memcpy(&v, &offv[k], sizeof v);
+                const bool stalled = rot[k] && v <= stall_off && g.prev_off[k] > 0.0 && v >= 0.5 * g.prev_off[k];
+                g.prev_off[k] = v;
+                if (trace_jacobi) hist[j].push_back(rot[k] ? v : 0.0);
+                if (!rot[k] || stalled) {
+                    g.done[k] = 1;
+                    g.active--;
                     changed = true;
                     if (stalled) stats.jacobi_stalled++;
                 }
             }
             // The rounds read the done flags at one address: the new flags are copied there.
-            if (changed && active > 0)
-                HIPC(hipMemcpyAsync(d_done, upload_ints(done), nj * sizeof(int), hipMemcpyDeviceToDevice, stream));
-        }
+            if (changed && g.active > 0)
+                HIPC(hipMemcpyAsync(g.d_done, upload_ints(g.done), g.nj * sizeof(int), hipMemcpyDeviceToDevice, stream));
+        });
         // A job still rotating after MAX_SWEEPS is counted; as the reference's Jacobi at its sweep
         // cap (src/tn/svd_guard.rs:165-229, then svd_reproduces), its factorization is kept when
         // it passes the norm and probe checks, which the caller applies to every split.
-        for (int j = 0; j < nj; ++j)
-            if (!done[j]) stats.jacobi_unconverged++;
+        for (auto& g : gs)
+            for (int k = 0; k < g->nj; ++k)
+                if (!g->done[k]) stats.jacobi_unconverged++;
         if (trace_jacobi) {
             std::vector<int> order(nj);
             for (int j = 0; j < nj; ++j) order[j] = j;
@@ -1206,7 +1349,7 @@ struct wqc_tn_state {
         }
         std::vector<long> offs = sig_off;
         long* doff = upload(offs);
-        wqc::launch_row_norms(dj, nj, max_n, d_sigma, doff, stream);
+        wqc::launch_row_norms(dj_all, nj, max_n, d_sigma, doff, stream);
         check_launch();
     }
 
