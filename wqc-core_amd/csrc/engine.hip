@@ -1384,7 +1384,7 @@ struct wqc_tn_state {
         std::vector<long> counts(nj);
         std::vector<wqc::JacobiLoJob> lj(nj);
         long mx = 1;
-        int max_n = 1, max_m = 1;
+        int max_n = 1;
         for (int k = 0; k < nj; ++k) {
             src[k] = jobs[k].X;
             dst[k] = reinterpret_cast<float2*>(b32 + o32[k]);
@@ -1397,7 +1397,6 @@ struct wqc_tn_state {
             lj[k].ldx = jobs[k].m;
             lj[k].pad = 0;
             max_n = std::max(max_n, jobs[k].n);
-            max_m = std::max(max_m, jobs[k].m);
         }
         wqc::launch_to_f32(upload(src), upload(dst), upload(counts), nj, mx, stream);
         // J32 = (complex64) J, the identity the split set up.
@@ -1415,82 +1414,74 @@ struct wqc_tn_state {
             wqc::launch_to_f32(upload(js), upload(jd), upload(jc), nj, jm, stream);
         }
         check_launch();
-        wqc::JacobiLoJob* dj = upload(lj);
-        const int nb = (max_n + 15) / 16;
-        const int nbe = nb + (nb & 1);
-        const int rounds = nbe - 1, pairs = nbe / 2;
         const float tol = (float)tol_lo;
         const double nu2 = NULL_ROWS_SQ;
-        int* d_done = d_flags + nj;
         unsigned long long* d_jstats = jstats_dev();
-        wqc::launch_fill_int(d_done, 0, nj, stream);
-        std::vector<int> done(nj, 0);
-        int active = nj;
-        // offmax (nj u64, collected only when tracing) then rotated (nj int), as in jacobi().
-        trace_buf.ensure((size_t)nj * (sizeof(unsigned long long) + sizeof(int)));
-        unsigned long long* d_off = trace_jacobi ? reinterpret_cast<unsigned long long*>(trace_buf.p) : nullptr;
-        int* d_rot = reinterpret_cast<int*>(reinterpret_cast<unsigned long long*>(trace_buf.p) + nj);
-        // Block pairs per job, for the choice of the 512-thread form (as in jacobi()).
+        // Block pairs per job, for the groups and the choice of the 512-thread form (as in
+        // jacobi()).
         std::vector<int> jpairs(nj);
         for (int j = 0; j < nj; ++j) {
             const int b = (jobs[j].n + 15) / 16;
             jpairs[j] = (b + (b & 1)) / 2;
         }
-        const int cus = dev->prop.multiProcessorCount;
-        SweepGraph graphs[2];  // 256-thread and 512-thread rounds
+        auto gs = make_groups(jpairs, 0);
+        for (auto& g : gs) {
+            std::vector<wqc::JacobiLoJob> sub;
+            for (int j : g->ids) sub.push_back(lj[j]);
+            g->dj = upload(sub);
+        }
         const bool bf16 = precision == WQC_TN_PRECISION_MIXED_BF16;
-        for (int sweep = 0; sweep < max_sweeps_lo && active > 0; ++sweep) {
-            check_cancel();
-            const auto t_sweep = std::chrono::steady_clock::now();
-            int wgs = 0;
-            for (int j = 0; j < nj; ++j)
-                if (!done[j]) wgs += jpairs[j];
-            const bool wide = use_wide && wgs <= cus;
-            {
-                Timer tm(this, 3);
-                run_sweep(graphs[wide ? 1 : 0], [&] {
-                    if (d_off)
-                        wqc::launch_fill_int(reinterpret_cast<int*>(d_off), 0, 3 * nj, stream);
-                    else
-                        wqc::launch_fill_int(d_rot, 0, nj, stream);
+        {
+            Timer tm(this, 3);
+            sweep_groups(gs, max_sweeps_lo, [&](SweepGroup& g, bool wide) {
+                int gmax = 1;
+                for (int j : g.ids) gmax = std::max(gmax, jobs[j].n);
+                const int nb = (gmax + 15) / 16;
+                const int nbe = nb + (nb & 1);
+                const int rounds = nbe - 1, pairs = nbe / 2;
+                const wqc::JacobiLoJob* dj = static_cast<const wqc::JacobiLoJob*>(g.dj);
+                // offmax (collected only when tracing) and rotated, cleared together.
+                unsigned long long* d_off = trace_jacobi ? g.d_off : nullptr;
+                run_sweep(g.graphs[wide ? 1 : 0], [&] {
+                    wqc::launch_fill_int(reinterpret_cast<int*>(g.d_off), 0, 3 * g.nj, stream);
                     for (int r = 0; r < rounds; ++r)
-                        wqc::launch_jacobi_round_lo(dj, nj, pairs, r, tol, nu2, (float)floor_lo, inner_cap_lo, d_rot,
-                                                    d_done, d_jstats, d_off, wide, bf16, stream);
+                        wqc::launch_jacobi_round_lo(dj, g.nj, pairs, r, tol, nu2, (float)floor_lo, inner_cap_lo,
+                                                    g.d_rot, g.d_done, d_jstats, d_off, wide, bf16, stream);
                 });
                 klaunch[3] += rounds;
-            }
-            stats.jacobi_sweeps_lo += active;
-            if (d_off) {
-                std::vector<unsigned long long> offv(nj);
-                HIPC(hipMemcpyAsync(offv.data(), d_off, nj * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
-                sync();
-                std::fprintf(stderr, "low sweep %d active %d wgs %d%s %.3f ms:", sweep + 1, active, wgs,
-                             wide ? " wide" : "",
-                             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_sweep).count());
-                int shown = 0;
-                for (int j = 0; j < nj && shown < 4; ++j) {
-                    if (done[j] || jobs[j].n < max_n) continue;
-                    double v;
-                    std::memcpy(&v, &offv[j], sizeof v);
-                    std::fprintf(stderr, " [%dx%d] %.2e", jobs[j].n, jobs[j].m, v);
-                    ++shown;
+                stats.jacobi_sweeps_lo += g.active;
+                HIPC(hipMemcpyAsync(g.h, g.d_off, g.nj * (sizeof(unsigned long long) + sizeof(int)),
+                                    hipMemcpyDeviceToHost, stream));
+            }, [&](SweepGroup& g) {
+                const unsigned long long* offv = reinterpret_cast<const unsigned long long*>(g.h);
+                const int* rot = reinterpret_cast<const int*>(offv + g.nj);
+                if (trace_jacobi) {
+                    std::fprintf(stderr, "low sweep %d stream %p active %d:", g.sweep + 1, (void*)g.s, g.active);
+                    int shown = 0;
+                    for (int k = 0; k < g.nj && shown < 4; ++k) {
+                        const int j = g.ids[k];
+                        if (g.done[k] || jobs[j].n < max_n) continue;
+                        double v;
+                        std::memcpy(&v, &offv[k], sizeof v);
+                        std::fprintf(stderr, " [%dx%d] %.2e", jobs[j].n, jobs[j].m, v);
+                        ++shown;
+                    }
+                    std::fprintf(stderr, "\n");
                 }
-                std::fprintf(stderr, "\n");
-            }
-            HIPC(hipMemcpyAsync(hread, d_rot, nj * sizeof(int), hipMemcpyDeviceToHost, stream));
-            sync();
-            const int* rot = reinterpret_cast<const int*>(hread);
-            bool changed = false;
-            for (int j = 0; j < nj; ++j)
-                if (!done[j]) lo_count[j]++;
-            for (int j = 0; j < nj; ++j)
-                if (!done[j] && !rot[j]) {
-                    done[j] = 1;
-                    active--;
-                    changed = true;
+                bool changed = false;
+                for (int k = 0; k < g.nj; ++k) {
+                    if (g.done[k]) continue;
+                    lo_count[g.ids[k]]++;
+                    if (!rot[k]) {
+                        g.done[k] = 1;
+                        g.active--;
+                        changed = true;
+                    }
                 }
-            if (changed && active > 0)
-                HIPC(hipMemcpyAsync(d_done, upload_ints(done), nj * sizeof(int), hipMemcpyDeviceToDevice, stream));
+                if (changed && g.active > 0)
+                    HIPC(hipMemcpyAsync(g.d_done, upload_ints(g.done), g.nj * sizeof(int), hipMemcpyDeviceToDevice,
+                                        stream));
+            });
         }
         // J = (complex128) J32, made unitary to FP64 precision by two Newton-Schulz steps
         // J <- J (3I - J^H J) / 2 (the FP32 rounds leave it unitary to about 1e-6).
