@@ -666,7 +666,6 @@ struct wqc_tn_state {
     // g's latest sweep.
     static constexpr int kMaxGroups = 4;
     hipStream_t streams_x[kMaxGroups - 1] = {};
-    hipStream_t stream2 = nullptr;  // streams_x[0]
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_sweep[kMaxGroups] = {};
     DevBuf ws;        // split workspace
     DevBuf ws32;      // mixed-precision rows and the J X product
@@ -854,6 +853,8 @@ struct wqc_tn_state {
         if (const char* e = std::getenv("WQC_TN_JACOBI_TRACE")) trace_jacobi = std::atoi(e) != 0;
         if (const char* e = std::getenv("WQC_TN_JACOBI_STALL")) stall_off = std::max(0.0, std::atof(e));
         if (const char* e = std::getenv("WQC_TN_JACOBI_GROUPS")) jacobi_groups = std::min(kMaxGroups, std::max(1, std::atoi(e)));
+        if (const char* e = std::getenv("WQC_TN_JACOBI_GROUPS_LO"))
+            jacobi_groups_lo = std::min(kMaxGroups, std::max(1, std::atoi(e)));
         // LQ preconditioning: on in the mixed form, off in the plain FP64 form (whose Jacobi on
         // the rows of theta' keeps small singular values relatively accurate; R = L^H carries
         // the LQ's rounding at the scale of the largest rows, which moves near-cutoff
@@ -891,10 +892,10 @@ struct wqc_tn_state {
         precision = prec;
         HIPC(hipSetDevice(dev->ordinal));
         HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        for (int g = 0; g + 1 < jacobi_groups; ++g) HIPC(hipStreamCreateWithFlags(&streams_x[g], hipStreamNonBlocking));
-        stream2 = streams_x[0];
+        const int ngroups = std::max(jacobi_groups, jacobi_groups_lo);
+        for (int g = 0; g + 1 < ngroups; ++g) HIPC(hipStreamCreateWithFlags(&streams_x[g], hipStreamNonBlocking));
         for (hipEvent_t* e : {&ev_fork, &ev_join}) HIPC(hipEventCreateWithFlags(e, hipEventDisableTiming));
-        for (int g = 0; g < jacobi_groups; ++g) HIPC(hipEventCreateWithFlags(&ev_sweep[g], hipEventDisableTiming));
+        for (int g = 0; g < ngroups; ++g) HIPC(hipEventCreateWithFlags(&ev_sweep[g], hipEventDisableTiming));
         for (DevBuf* b : state_bufs()) b->bind(&dev->pool, stream);
         htab_cap = 8u << 20;
         htab = static_cast<char*>(dev->get_pinned(htab_cap));
@@ -1130,13 +1131,16 @@ struct wqc_tn_state {
     // FP64 form 1456 / 1389 / 1374 / 1357 ms per circuit with 1 / 2 / 3 / 4, mixed form 693 /
     // 690 / 690 / 689 ms.
     int jacobi_groups = kMaxGroups;
+    // The mixed forms' low sweeps in one group (WQC_TN_JACOBI_GROUPS_LO, 1 to kMaxGroups): four
+    // gave config 3 nothing (693 / 691 ms) and made the 256-slice batch slower (1.21 -> 1.32 s).
+    int jacobi_groups_lo = 1;
 
     // Splits nj jobs (block pairs jp[j]) into min(jacobi_groups, nj) groups balanced by block
     // pairs (largest first, each to the lightest group). Sets up flags (in trace_buf,
     // cleared), readback regions and events.
-    std::vector<std::unique_ptr<SweepGroup>> make_groups(const std::vector<int>& jp, size_t hread_off) {
+    std::vector<std::unique_ptr<SweepGroup>> make_groups(const std::vector<int>& jp, size_t hread_off, int groups) {
         const int nj = (int)jp.size();
-        const int ng = std::max(1, std::min(jacobi_groups, nj));
+        const int ng = std::max(1, std::min(groups, nj));
         std::vector<std::unique_ptr<SweepGroup>> gs;
         for (int g = 0; g < ng; ++g) {
             gs.push_back(std::make_unique<SweepGroup>());
@@ -1269,7 +1273,7 @@ struct wqc_tn_state {
             const int b = (jobs[j].n + 15) / 16;
             jpairs[j] = (b + (b & 1)) / 2;
         }
-        auto gs = make_groups(jpairs, 0);
+        auto gs = make_groups(jpairs, 0, jacobi_groups);
         for (auto& g : gs) {
             std::vector<JacobiJob> sub;
             for (int j : g->ids) sub.push_back(jobs[j]);
@@ -1424,7 +1428,7 @@ struct wqc_tn_state {
             const int b = (jobs[j].n + 15) / 16;
             jpairs[j] = (b + (b & 1)) / 2;
         }
-        auto gs = make_groups(jpairs, 0);
+        auto gs = make_groups(jpairs, 0, jacobi_groups_lo);
         for (auto& g : gs) {
             std::vector<wqc::JacobiLoJob> sub;
             for (int j : g->ids) sub.push_back(lj[j]);
