@@ -278,21 +278,25 @@ __device__ __forceinline__ void pair_order(int tid, const int* s_row, const Key*
     int out = tid;
     if (rk >= 0) {
         const Key key = diag[tid * stride];
-        int rank = 0;
-        for (int j = 0; j < JP; ++j) {
-            const int rj = s_row[j];
-            if (j == tid || rj < 0) continue;
-            const Key kj = diag[j * stride];
-            if (kj > key || (kj == key && rj < rk)) ++rank;
-        }
+        int rank = 0, na = 0, nb = 0;
         for (int j = 0; j < JP; ++j) {
             const int rj = s_row[j];
             if (rj < 0) continue;
-            int pos = 0;
-            for (int i = 0; i < JP; ++i)
-                if (s_row[i] >= 0 && s_row[i] < rj) ++pos;
-            if (pos == rank) out = j;
+            if (j < JB)
+                ++na;
+            else
+                ++nb;
+            if (j == tid) continue;
+            const Key kj = diag[j * stride];
+            if (kj > key || (kj == key && rj < rk)) ++rank;
         }
+        // The slot holding the rank-th smallest row index. Each half's valid rows are a prefix
+        // of it in ascending order, and the half of the lower block comes first.
+        const bool a_first = (na == 0 || nb == 0) ? na > 0 : s_row[0] < s_row[JB];
+        if (a_first)
+            out = rank < na ? rank : JB + rank - na;
+        else
+            out = rank < nb ? JB + rank : rank - nb;
     }
     dst[tid] = out;
 }
@@ -558,14 +562,20 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round(const Ja
                         off = true;
                         if ((i < JB) == (j < JB)) intra = true;
                     }
-                    if (offmax) worst = fmax(worst, sqrt(g2 / (di * dj)));
+                    if (offmax) worst = fmax(worst, g2 / (di * dj));
                 }
             }
         }
         if (off) s_any = 1;
         if (intra) s_intra = 1;
-        // Non-negative doubles order as their bit patterns.
-        if (offmax && worst > 0.0) atomicMax(&s_off, (unsigned long long)__double_as_longlong(worst));
+        if (offmax) {
+            // The largest squared ratio over the wave first (one LDS atomic per wave), then its
+            // square root: sqrt is monotonic, so this is exactly the largest relative
+            // off-diagonal. Non-negative doubles order as their bit patterns.
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) worst = fmax(worst, __shfl_xor(worst, o, 64));
+            if (lane == 0 && worst > 0.0) atomicMax(&s_off, (unsigned long long)__double_as_longlong(sqrt(worst)));
+        }
     }
     __syncthreads();
     if (offmax && tid == 0 && s_off) atomicMax(&offmax[job], s_off);
@@ -1039,10 +1049,11 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round_lo(const
     // Rows whose squared norm is below rel_floor of the pair's largest are left to the FP64
     // finish: FP32 rotations add noise of about 1e-7 of the largest row to them, so they could
     // not reach the relative tolerance here.
-    if (tid == 0) {
-        float mx = 0.f;
-        for (int i = 0; i < JP; ++i) mx = fmaxf(mx, G[i][i].x);
-        s_low = fmaxf((float)nu2, mx * rel_floor);
+    if (wave == 0) {
+        float mx = lane < JP ? G[lane][lane].x : 0.f;
+#pragma unroll
+        for (int o = JP / 2; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+        if (lane == 0) s_low = fmaxf((float)nu2, mx * rel_floor);
     }
     __syncthreads();
     const float low = s_low;
