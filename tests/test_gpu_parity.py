@@ -566,6 +566,46 @@ def test_lq_preconditioned_split_vs_oracle(wt, dl, dr, bond, chi, env):
     g.free()
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_jacobi_job_groups_agree(wt, prec):
+    """A Jacobi batch's jobs sweep as independent groups on their own streams
+    (WQC_TN_JACOBI_GROUPS, and WQC_TN_JACOBI_GROUPS_LO for the mixed form's low sweeps). A
+    truncating brickwork on 20 qubits, whose layers batch up to ten splits, run with one group
+    and with four (both settings at 4 for the mixed form) must keep identical bonds and split
+    counters, agree to 1e-11 on the amplitude (the groups change which rounds take the
+    512-thread form, so the last bits may differ), match the oracle to the parity bar, and
+    give the same bits when run twice with four groups."""
+    import os
+
+    n, depth, chi = 20, 14, 12
+    gates = C.brickwork(n, depth, 2024)
+
+    def run(groups):
+        os.environ["WQC_TN_JACOBI_GROUPS"] = groups
+        os.environ["WQC_TN_JACOBI_GROUPS_LO"] = groups
+        try:
+            g = wt.MpsState(n, chi, prec)
+        finally:
+            os.environ.pop("WQC_TN_JACOBI_GROUPS", None)
+            os.environ.pop("WQC_TN_JACOBI_GROUPS_LO", None)
+        g.apply_circuit(gates)
+        st = g.stats()
+        out = (g.bond_dims(), g.amplitude_at_compact_zero(), st["two_qubit_units"], st["truncated_splits"])
+        g.free()
+        return out
+
+    one, four, again = run("1"), run("4"), run("4")
+    o = om.OracleMps(n, chi)
+    o.apply_circuit(gates)
+    want = o.amplitude_at_compact_zero()
+    say("job groups (precision %d): one" % prec, one[1], "four", four[1], "oracle", want)
+    assert four == again
+    assert one[0] == four[0] == o.bond_dims()
+    assert one[2:] == four[2:] and four[3] > 0
+    assert rel(four[1], one[1]) <= 1e-11
+    assert rel(four[1], want) <= TOL
+
+
 @pytest.mark.parametrize("prec,bar", [(0, 1e-12), (1, 1e-11), (2, 1e-11)])
 def test_wide_and_bulk_rounds_agree(wt, prec, bar):
     """The Jacobi rounds (FP64, and the mixed form's low-precision ones) have a 512-thread form
