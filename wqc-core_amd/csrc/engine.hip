@@ -2292,8 +2292,9 @@ struct wqc_tn_state {
             }
             call(gg);
             pre.back().push_back(Act{1, b0, 0, 0});
+            // (U T)^H and T U depend only on T: one launch for both.
+            gt.insert(gt.end(), gh.begin(), gh.end());
             call(gt);
-            call(gh);
             // Rows below the block.
             call(g1);
             call(g2);
