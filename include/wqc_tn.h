@@ -140,6 +140,9 @@ int wqc_tn_engine_status(char *active, size_t cap, uint64_t *vram_budget_bytes);
 /* Open device `ordinal` (0 = first visible GPU). Replaces GpuMpsDevice::try_new
  * (src/tn/gpu/device.rs:68-214). */
 int wqc_tn_device_open(int ordinal, wqc_tn_device **out);
+/* Frees the device's cache of device blocks and pinned staging that freed states gave back
+ * (at most 48 GiB of device memory; it saves the next states of the device hipMalloc and
+ * hipFree, which waits for the whole device). Every state of the device must be freed first. */
 void wqc_tn_device_close(wqc_tn_device *dev);
 
 /* ---- state lifetime ------------------------------------------------------------ */
