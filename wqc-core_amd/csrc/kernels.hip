@@ -1748,11 +1748,14 @@ __global__ __launch_bounds__(256) void k_lq_tblock(const LqJob* __restrict__ job
     if (nbs <= 0) return;
     __shared__ double2 T[LQ_SB][LQ_SB + 1];
     __shared__ double2 W[LQ_SB / 2][LQ_SB / 2 + 1];
+    // Gu staged in LDS: the merges below read it along dependent accumulation chains.
+    __shared__ double2 G[LQ_SB][LQ_SB + 1];
     const int tid = threadIdx.x;
     for (int e = tid; e < LQ_SB * LQ_SB; e += 256) {
         const int i = e / LQ_SB, k = e % LQ_SB;
         const bool diag_block = i / LQ_NB == k / LQ_NB && k >= i;
         T[i][k] = (i < nbs && k < nbs && diag_block) ? jb.T64[e] : cz();
+        G[i][k] = (i < nbs && k < nbs) ? jb.Gu[e] : cz();
     }
     __syncthreads();
     for (int w = LQ_NB; w < LQ_SB; w *= 2) {
@@ -1763,7 +1766,7 @@ __global__ __launch_bounds__(256) void k_lq_tblock(const LqJob* __restrict__ job
             for (int e = tid; e < w * bw; e += 256) {
                 const int i = e / bw, k = e % bw;
                 double2 acc = cz();
-                for (int l = 0; l <= k; ++l) cmac(acc, jb.Gu[(a + i) * LQ_SB + a + w + l], T[a + w + l][a + w + k]);
+                for (int l = 0; l <= k; ++l) cmac(acc, G[a + i][a + w + l], T[a + w + l][a + w + k]);
                 W[i][k] = acc;
             }
             __syncthreads();
