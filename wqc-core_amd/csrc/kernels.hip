@@ -623,7 +623,6 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round(const Ja
                 const Rot64 r2 = rot_params64(Gc[p2][p2].x, Gc[q2][q2].x, Gc[p2][q2], tol, nu2);
                 const Rot64 r1 = rot_from_lane(r2, bk & 15);
                 double2 h00 = Gc[p1][p2], h01 = Gc[p1][q2], h10 = Gc[q1][p2], h11 = Gc[q1][q2];
-                double2 t00 = Tc[p1][p2], t01 = Tc[p1][q2], t10 = Tc[q1][p2], t11 = Tc[q1][q2];
                 if (r1.act) {
                     const double cs = r1.cs;
                     const double2 m01 = r1.m01, m10 = r1.m10;
@@ -635,14 +634,12 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round(const Ja
                     h01 = x01;
                     h10 = x10;
                     h11 = x11;
-                    const double2 y00 = cadd(cscale(t00, cs), cmul(m01, t10));
-                    const double2 y01 = cadd(cscale(t01, cs), cmul(m01, t11));
-                    const double2 y10 = cadd(cmul(m10, t00), cscale(t10, cs));
-                    const double2 y11 = cadd(cmul(m10, t01), cscale(t11, cs));
-                    t00 = y00;
-                    t01 = y01;
-                    t10 = y10;
-                    t11 = y11;
+                    // T (updated in place) changes only with the row rotation.
+                    const double2 t00 = Tc[p1][p2], t01 = Tc[p1][q2], t10 = Tc[q1][p2], t11 = Tc[q1][q2];
+                    Tc[p1][p2] = cadd(cscale(t00, cs), cmul(m01, t10));
+                    Tc[p1][q2] = cadd(cscale(t01, cs), cmul(m01, t11));
+                    Tc[q1][p2] = cadd(cmul(m10, t00), cscale(t10, cs));
+                    Tc[q1][q2] = cadd(cmul(m10, t01), cscale(t11, cs));
                 }
                 if (r2.act) {
                     const double cs = r2.cs;
@@ -667,10 +664,6 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round(const Ja
                 Gn[p1][q2] = h01;
                 Gn[q1][p2] = h10;
                 Gn[q1][q2] = h11;
-                Tc[p1][p2] = t00;
-                Tc[p1][q2] = t01;
-                Tc[q1][p2] = t10;
-                Tc[q1][q2] = t11;
             }
             __syncthreads();
             double2 (*sw)[JP + 1] = Gc;
