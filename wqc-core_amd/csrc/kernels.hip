@@ -876,8 +876,9 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round_lo(const
 
     __shared__ float2 G[JP][JP + 1];
     __shared__ float2 T[JP][JP + 1];
-    // Partial Gram tiles of two waves; afterwards the second G and T buffers of the eigensolve.
-    __shared__ __align__(16) float s_aux[NW == 8 ? 4 * 6 * 4 * 64 : 4 * JP * (JP + 1)];
+    // Partial Gram tiles of two waves (four with eight waves); afterwards the second G buffer of
+    // the eigensolve.
+    __shared__ __align__(16) float s_aux[NW == 8 ? 4 * 6 * 4 * 64 : 2 * 6 * 4 * 64];
     __shared__ int s_row[JP];
     __shared__ int s_any;
     __shared__ int s_intra;
@@ -885,7 +886,6 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round_lo(const
     __shared__ int s_dst[JP];
     float (*Gp)[6][4][64] = reinterpret_cast<float (*)[6][4][64]>(s_aux);
     float2 (*G2)[JP + 1] = reinterpret_cast<float2 (*)[JP + 1]>(s_aux);
-    float2 (*T2)[JP + 1] = reinterpret_cast<float2 (*)[JP + 1]>(s_aux + 2 * JP * (JP + 1));
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid < JP) {
@@ -1089,9 +1089,8 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round_lo(const
     __syncthreads();
 
     float2 (*Gc)[JP + 1] = G;
-    float2 (*Tc)[JP + 1] = T;
+    float2 (*const Tc)[JP + 1] = T;  // updated in place: a step's tiles are disjoint
     float2 (*Gn)[JP + 1] = G2;
-    float2 (*Tn)[JP + 1] = T2;
     int sweeps_done = 0;
     const int bk = tid >> 4, bk2 = tid & 15;
     auto fm = [](float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
@@ -1118,7 +1117,6 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round_lo(const
                 const Rot32 r2 = rot_params32(Gc[p2][p2].x, Gc[q2][q2].x, Gc[p2][q2], tol, low);
                 const Rot32 r1 = rot_from_lane32(r2, bk & 15);
                 float2 h00 = Gc[p1][p2], h01 = Gc[p1][q2], h10 = Gc[q1][p2], h11 = Gc[q1][q2];
-                float2 t00 = Tc[p1][p2], t01 = Tc[p1][q2], t10 = Tc[q1][p2], t11 = Tc[q1][q2];
                 if (r1.act) {
                     const float cs = r1.cs;
                     const float2 m01 = r1.m01, m10 = r1.m10;
@@ -1128,12 +1126,12 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round_lo(const
                     h01 = x01;
                     h10 = x10;
                     h11 = x11;
-                    const float2 y00 = fa(fs(t00, cs), fm(m01, t10)), y01 = fa(fs(t01, cs), fm(m01, t11));
-                    const float2 y10 = fa(fm(m10, t00), fs(t10, cs)), y11 = fa(fm(m10, t01), fs(t11, cs));
-                    t00 = y00;
-                    t01 = y01;
-                    t10 = y10;
-                    t11 = y11;
+                    // T changes only with the row rotation.
+                    const float2 t00 = Tc[p1][p2], t01 = Tc[p1][q2], t10 = Tc[q1][p2], t11 = Tc[q1][q2];
+                    Tc[p1][p2] = fa(fs(t00, cs), fm(m01, t10));
+                    Tc[p1][q2] = fa(fs(t01, cs), fm(m01, t11));
+                    Tc[q1][p2] = fa(fm(m10, t00), fs(t10, cs));
+                    Tc[q1][q2] = fa(fm(m10, t01), fs(t11, cs));
                 }
                 if (r2.act) {
                     const float cs = r2.cs;
@@ -1156,18 +1154,11 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round_lo(const
                 Gn[p1][q2] = h01;
                 Gn[q1][p2] = h10;
                 Gn[q1][q2] = h11;
-                Tn[p1][p2] = t00;
-                Tn[p1][q2] = t01;
-                Tn[q1][p2] = t10;
-                Tn[q1][q2] = t11;
             }
             __syncthreads();
             float2 (*sw)[JP + 1] = Gc;
             Gc = Gn;
             Gn = sw;
-            sw = Tc;
-            Tc = Tn;
-            Tn = sw;
         }
         const int again = s_any;
         __syncthreads();
