@@ -1220,23 +1220,31 @@ struct wqc_tn_state {
             stream = main;
             g.in_flight = true;
         };
-        for (auto& g : gs)
-            if (g->active > 0 && max_sweeps > 0) launch(*g);
-        for (;;) {
-            bool any = false;
-            for (auto& gp : gs) {
-                SweepGroup& g = *gp;
-                if (!g.in_flight) continue;
-                any = true;
-                HIPC(hipEventSynchronize(g.ev));
-                g.in_flight = false;
-                stream = g.s;
-                after(g);
-                stream = main;
-                ++g.sweep;
-                if (g.active > 0 && g.sweep < max_sweeps) launch(g);
+        try {
+            for (auto& g : gs)
+                if (g->active > 0 && max_sweeps > 0) launch(*g);
+            for (;;) {
+                bool any = false;
+                for (auto& gp : gs) {
+                    SweepGroup& g = *gp;
+                    if (!g.in_flight) continue;
+                    any = true;
+                    HIPC(hipEventSynchronize(g.ev));
+                    g.in_flight = false;
+                    stream = g.s;
+                    after(g);
+                    stream = main;
+                    ++g.sweep;
+                    if (g.active > 0 && g.sweep < max_sweeps) launch(g);
+                }
+                if (!any) break;
             }
-            if (!any) break;
+        } catch (...) {
+            // A cancellation or a failed call: let the other groups' queued sweeps end before
+            // the state's buffers are used again.
+            for (auto& g : gs)
+                if (g->s != main) (void)hipStreamSynchronize(g->s);
+            throw;
         }
         for (auto& g : gs)
             if (g->s != main) {
