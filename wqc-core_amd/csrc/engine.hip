@@ -344,21 +344,30 @@ struct BlockPool {
                 return;
             }
         }
-        std::vector<Block> drop;
+        bool kept = true;
+        std::multimap<size_t, Block> drop;  // filled by node moves, which allocate nothing
         {
             std::lock_guard<std::mutex> lk(mu);
-            blocks.emplace(b.bytes, b);
-            held += b.bytes;
+            try {
+                blocks.emplace(b.bytes, b);
+                held += b.bytes;
+            } catch (...) {
+                kept = false;  // no host memory for the entry: the block is freed below
+            }
             while (held > kHeldMax && !blocks.empty()) {
-                auto it = std::prev(blocks.end());
-                drop.push_back(it->second);
-                held -= it->first;
-                blocks.erase(it);
+                auto node = blocks.extract(std::prev(blocks.end()));
+                held -= node.key();
+                drop.insert(std::move(node));
             }
         }
-        for (auto& d : drop) {
-            (void)hipFree(d.p);
-            if (d.ev) (void)hipEventDestroy(d.ev);
+        if (!kept) {
+            if (s) (void)hipStreamSynchronize(s);
+            (void)hipFree(b.p);
+            if (b.ev) (void)hipEventDestroy(b.ev);
+        }
+        for (auto& kv : drop) {
+            (void)hipFree(kv.second.p);
+            if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
         }
     }
     size_t held_bytes() {
