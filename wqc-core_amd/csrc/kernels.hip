@@ -781,13 +781,13 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void k_jacobi_round(const Ja
 }
 
 // ------------------------------------------------------------------ mixed-precision round
-// Low-precision phase of the mixed split: the rows live in complex64 (X32), the
-// accumulated rotation J in complex128. Per block pair: Gram on FP32 MFMA
-// (v_mfma_f32_16x16x4f32), eigensolve of the 32x32 Gram matrix in FP32, then the rotation
-// T is lifted to FP64 and made unitary to FP64 precision by two Newton-Schulz steps
-// T <- T (3I - T^H T) / 2, applied to X32 in FP32 and to J in FP64. J is therefore unitary
-// to FP64 precision whatever the FP32 rounding, and X32 tracks J X to FP32 precision; the
-// FP64 finish recomputes J X exactly and polishes it with the FP64 rounds.
+// Low-precision phase of the mixed split: the rows (X32) and the accumulated rotation (J32)
+// live in complex64. Per block pair: the Gram on FP32 MFMA (v_mfma_f32_16x16x4f32), or with
+// GB in split-BF16 on BF16 MFMA with FP32 accumulation; the eigensolve of the 32x32 Gram
+// matrix in FP32; the rotation T applied to X32 and J32 on FP32 MFMA. J32 stays unitary to
+// about FP32 precision; after the last low round the host lifts it to complex128 and makes it
+// unitary to FP64 precision by Newton-Schulz steps, recomputes X = J X_orig in FP64, and the
+// FP64 rounds finish from there.
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ f4v mfma_f32(float a, float b, f4v c) {
